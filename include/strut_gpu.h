@@ -1,0 +1,158 @@
+/*
+ * strut_gpu.h -- C ABI of libstrut_gpu, the MI355X (gfx950) implementation
+ * of Strut's nonlinear fiber-section state determination.
+ *
+ * Plain C: integers, doubles, pointers and sizes only.  One opaque context
+ * owns every device buffer and one HIP stream; calls on one context are not
+ * thread-safe; several contexts may live in one process.  No call aborts the
+ * process: every entry point returns an sgpu_status, and sgpu_last_error()
+ * holds the message of the last failure on that context.
+ *
+ * Each declaration names the reference interface it replaces (paths are
+ * relative to the Strut repository root, src/mojo/...).
+ */
+#ifndef STRUT_GPU_H
+#define STRUT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGPU_ABI_VERSION 1
+
+typedef struct sgpu_ctx sgpu_ctx;
+
+/* Status codes.  Negative values are API or runtime errors; positive values
+ * are numerical outcomes the host maps onto the reference's messages. */
+typedef enum {
+    SGPU_OK = 0,
+    SGPU_ERR_ARG = -1,       /* bad argument (size, index, null pointer)   */
+    SGPU_ERR_HIP = -2,       /* HIP runtime error                          */
+    SGPU_ERR_NODEV = -3,     /* no usable gfx950 device                    */
+    SGPU_ERR_STATE = -4,     /* call out of order (e.g. before finalize)   */
+    SGPU_SINGULAR_SECTION = 1 /* section |det k| <= 1e-40 (fiber2d.mojo:3074-3077) */
+} sgpu_status;
+
+/* Material type tags: tag_types.mojo:136-142 (UniMaterialTypeTag). */
+enum {
+    SGPU_ELASTIC = 0,
+    SGPU_STEEL01 = 1,
+    SGPU_CONCRETE01 = 2,
+    SGPU_STEEL02 = 3,
+    SGPU_CONCRETE02 = 4
+};
+
+int sgpu_abi_version(void);
+
+/* Create a context on HIP device `device`.  Replaces nothing in the
+ * reference (the reference has no device); owns all state the reference
+ * keeps in RunCaseState (solver/run_case/loader.mojo:68). */
+int sgpu_ctx_create(int device, sgpu_ctx **out);
+int sgpu_ctx_destroy(sgpu_ctx *ctx);
+const char *sgpu_last_error(const sgpu_ctx *ctx);
+/* The context's hipStream_t, for callers that order their own work on it. */
+void *sgpu_stream(sgpu_ctx *ctx);
+int sgpu_synchronize(sgpu_ctx *ctx);
+
+/* Upload the uniaxial material table: n definitions, tags in `types`,
+ * 11 parameters each (p0..p10) already normalised as
+ * solver/run_case/loader.mojo:963-1096 does.  Replaces the
+ * List[UniMaterialDef] of core.mojo:48-118. */
+int sgpu_material_defs_upload(sgpu_ctx *ctx, int n, const int32_t *types, const double *params);
+
+/* Create a fiber-section definition from its discretised fibers (input
+ * order, as the patch/layer generators of fiber2d.mojo:3504-3878 and
+ * fiber3d.mojo:258-470 emit them).  dim = 2 ignores z.  Replaces
+ * _build_fiber_section2d_def (fiber2d.mojo:3323-3455) and
+ * _build_fiber_section3d_def (fiber3d.mojo:157-186). */
+int sgpu_section_def_create(sgpu_ctx *ctx, int dim, int n_fibers, const double *y, const double *z,
+                            const double *area, const int32_t *mat_index, int32_t *def_id);
+
+/* Allocate n runtime instances of a definition (one per element
+ * integration point); returns the first instance index.  Replaces
+ * fiber_section2d_runtime_alloc_instances (fiber2d.mojo:961-1006) and the
+ * per-element state allocation of loader.mojo:1760-1869. */
+int sgpu_section_instances_alloc(sgpu_ctx *ctx, int32_t def_id, int n, int32_t *first_instance);
+
+/* Freeze definitions and instances, upload the slot tables and the initial
+ * material states (UniMaterialState(mat_def), core.mojo:203-263).  No
+ * definition or instance may be added afterwards. */
+int sgpu_sections_finalize(sgpu_ctx *ctx);
+
+/* Static facts of a definition: out[0] fiber count, out[1] slot count,
+ * out[2] y centroid, out[3] z centroid, out[4] initial flexibility valid,
+ * out[5..7] initial 2D flexibility {f00, f01, f11}
+ * (fiber2d.mojo:3448-3454). */
+int sgpu_section_def_info(sgpu_ctx *ctx, int32_t def_id, double *out8);
+
+/* One section evaluation point: a (definition, instance) pair and its
+ * section deformation.  2D uses {eps0, kappa_z}; 3D uses
+ * {eps0, kappa_z, kappa_y}.  Mirrors FiberSection2dBatchPoint
+ * (fiber2d.mojo:621-676). */
+typedef struct {
+    int32_t def_id;
+    int32_t instance;
+    double eps0;
+    double kappa_z;
+    double kappa_y;
+} sgpu_section_point;
+
+/* Section response.  2D: s = {N, Mz, 0}, k = {k11, k12, k22, 0, 0, 0},
+ * f = {f00, f01, f11, 0, 0, 0}.  3D: s = {N, My, Mz},
+ * k = {k11, k12, k13, k22, k23, k33}, f = the same entries of k^-1.
+ * valid = 0 when |det k| <= 1e-40.  Mirrors FiberSection2dResponse and
+ * FiberSection2dBatchResult (fiber2d.mojo:586-612, :679-705) and
+ * FiberSection3dResponse (fiber3d.mojo:98-143). */
+typedef struct {
+    double s[3];
+    double k[6];
+    double f[6];
+    int32_t valid;
+    int32_t pad_;
+} sgpu_section_result;
+
+/* Evaluate n points: trial strains, fiber trial states, section resultants,
+ * tangents and flexibilities.  Fiber trial states are always computed from
+ * the committed states, so points must name distinct instances within one
+ * call.  Replaces fiber_section2d_set_trial_from_offset
+ * (fiber2d.mojo:3982-4024), fiber_section2d_batch_eval_same_def
+ * (:3042-3086) and fiber_section3d_set_trial_from_offset
+ * (fiber3d.mojo:769-836).  Host buffers; synchronous. */
+int sgpu_section_eval(sgpu_ctx *ctx, int n, const sgpu_section_point *points,
+                      sgpu_section_result *results);
+
+/* Same with device-resident points/results; asynchronous on the context's
+ * stream.  Used by hosts that keep the Newton loop on the device. */
+int sgpu_section_eval_device(sgpu_ctx *ctx, int n, const sgpu_section_point *d_points,
+                             sgpu_section_result *d_results);
+
+/* Commit (trial -> committed) or revert (committed -> trial) every fiber of
+ * every instance.  Replaces fiber_section2d_commit_runtime_all /
+ * revert_trial_runtime_all (fiber2d.mojo:4191-4252) and
+ * fiber_section3d_commit_all / revert_trial_all (fiber3d.mojo:905-926).
+ * Asynchronous on the context's stream. */
+int sgpu_sections_commit_all(sgpu_ctx *ctx);
+int sgpu_sections_revert_all(sgpu_ctx *ctx);
+
+/* Commit or revert listed instances only.  Replaces
+ * fiber_section2d_commit_from_offset / revert_trial_from_offset
+ * (fiber2d.mojo:4059-4128) and fiber_section3d_commit / revert_trial
+ * (fiber3d.mojo:873-902).  Host lists; synchronous. */
+int sgpu_sections_commit(sgpu_ctx *ctx, int n, const int32_t *def_ids, const int32_t *instances);
+int sgpu_sections_revert(sgpu_ctx *ctx, int n, const int32_t *def_ids, const int32_t *instances);
+
+/* Read the per-fiber state of one instance in input fiber order:
+ * out[3*i..3*i+2] = {eps, sig, tangent}; which = 0 trial, 1 committed.
+ * Elastic fibers report zeros (the 2D reference keeps no state for them). */
+int sgpu_fiber_state_read(sgpu_ctx *ctx, int32_t def_id, int32_t instance, int which, double *out);
+
+/* Total real fibers over all instances (the unit of the fiber-update
+ * count, benchmarks_micro.mojo:250) and total slots. */
+int sgpu_fiber_totals(sgpu_ctx *ctx, int64_t *fibers, int64_t *slots);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STRUT_GPU_H */
