@@ -1,0 +1,31 @@
+/* Internal types of the oracle's element and transient layers (test
+ * infrastructure only). */
+#ifndef STRUT_ORACLE_INTERNAL_H
+#define STRUT_ORACLE_INTERNAL_H
+
+#include "strut_oracle.h"
+
+#define OR_MAX_NIP 10
+
+typedef struct {
+    long section_evals;   /* section point evaluations */
+    long local_iters;     /* forceBeamColumn local iterations */
+    long subdivisions;    /* subdivision cutbacks */
+    long unconverged;     /* element states left unconverged */
+    long fallbacks;       /* initial-tangent fallbacks */
+} or_stats;
+
+typedef struct {
+    or_sec2d *sec;
+    int first_inst;       /* section instance of integration point 0 */
+    int nip;
+    double xi[OR_MAX_NIP], w[OR_MAX_NIP]; /* on [0,1], weights halved */
+    double L, inv_L, c, s; /* cached geometry (force_beam_column2d.mojo:383-408) */
+    or_stats *stats;
+} or_elem2d;
+
+int or_sec2d_all_elastic(const or_sec2d *s);
+int or_fbc2d_state(const or_elem2d *el, double *q_state, const double u_global[6], double k_out[36],
+                   double f_out[6]);
+
+#endif

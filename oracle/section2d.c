@@ -231,6 +231,8 @@ int or_sec2d_alloc_instances(or_sec2d *s, int n) {
 double or_sec2d_ybar(const or_sec2d *s) { return s->y_bar; }
 int or_sec2d_fiber_count(const or_sec2d *s) { return s->fiber_count; }
 int or_sec2d_nonlinear_only(const or_sec2d *s) { return s->ne == 0; }
+/* force_beam_column2d.mojo:327-331 (_fiber_section2d_all_materials_elastic) */
+int or_sec2d_all_elastic(const or_sec2d *s) { return s->nn == 0; }
 
 int or_sec2d_initial_flex(const or_sec2d *s, double *f3) {
     f3[0] = s->if00;

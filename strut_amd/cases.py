@@ -111,3 +111,192 @@ def synthetic_section_deformation(n: int, step: int, seed: int, dim: int) -> Tup
     kz = amp_k * np.sin(w * t + ph[1]) * (1.0 + 0.25 * math.sin(0.05 * t))
     ky = 0.0 * kz if dim == 2 else 0.6 * amp_k * np.sin(0.8 * w * t + ph[2])
     return eps0, kz, ky
+
+
+# ---------------------------------------------------------------- motions
+_MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed: int):
+    """SplitMix64 stream (Steele, Lea, Flood 2014): portable across numpy
+    versions, so the same seed yields the same record everywhere."""
+    state = seed & _MASK64
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & _MASK64
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+        yield z ^ (z >> 31)
+
+
+def gaussian_noise(seed: int, n: int):
+    """Box-Muller normals from SplitMix64 uniforms on (0, 1]."""
+    import numpy as np
+
+    gen = splitmix64(seed)
+    out = np.empty(n)
+    for i in range(0, n, 2):
+        u1 = ((next(gen) >> 11) + 1) * (1.0 / 9007199254740992.0)
+        u2 = ((next(gen) >> 11) + 1) * (1.0 / 9007199254740992.0)
+        r = math.sqrt(-2.0 * math.log(u1))
+        out[i] = r * math.cos(2.0 * math.pi * u2)
+        if i + 1 < n:
+            out[i + 1] = r * math.sin(2.0 * math.pi * u2)
+    return out
+
+
+def ground_motion(seed: int, dt: float, n_steps: int, pga_g: float, f_lo: float = 0.1, f_hi: float = 25.0,
+                  rise: float = 2.0, strong: float = 6.0, decay: float = 2.0):
+    """Band-limited white noise (4th-order Butterworth band-pass applied
+    forward and backward) under a trapezoidal envelope, scaled to a peak
+    ground acceleration of pga_g * 9.81 m/s^2.  Returns n_steps + 1 samples
+    at spacing dt starting at t = 0."""
+    import numpy as np
+    from scipy import signal
+
+    n = n_steps + 1
+    raw = gaussian_noise(seed, n)
+    fs = 1.0 / dt
+    hi = min(f_hi, 0.45 * fs)
+    sos = signal.butter(4, [f_lo, hi], btype="bandpass", fs=fs, output="sos")
+    band = signal.sosfiltfilt(sos, raw)
+    t = np.arange(n) * dt
+    env = np.clip(np.minimum(t / rise, (rise + strong + decay - t) / decay), 0.0, 1.0)
+    acc = band * env
+    peak = np.max(np.abs(acc))
+    return acc * (pga_g * 9.81 / peak)
+
+
+# ---------------------------------------------------------------- frames
+def _section_flex_elastic(cells, mats):
+    """Initial section stiffness terms (EA, ES, EI) about the centroid."""
+    import numpy as np
+
+    a = cells["area"]
+    e = np.array([_init_tangent(mats.defs[m]) for m in cells["mat"]])
+    yb = np.sum(a * cells["y"]) / np.sum(a)
+    yr = cells["y"] - yb
+    ks = e * a
+    return ks.sum(), (-ks * yr).sum(), (ks * yr * yr).sum()
+
+
+def _init_tangent(d):
+    from .model import CONCRETE01, CONCRETE02, ELASTIC, STEEL01, STEEL02
+
+    if d.mat_type == ELASTIC:
+        return d.p[0]
+    if d.mat_type in (STEEL01, STEEL02):
+        return d.p[1]
+    return 2.0 * d.p[0] / d.p[1]
+
+
+def rc_frame2d_case(stories: int, bays: int, steps: int, dt: float, seed: int, pga_g: float,
+                    story_h: float = 3.0, bay_w: float = 6.0, floor_mass: float = 50.0, nip: int = 5,
+                    strong: float = 6.0, tol: float = 1.0e-8, max_iters: int = 25, name: str = "") -> dict:
+    """2D RC moment frame in Strut's JSON schema: forceBeamColumn2d columns
+    and beams (Lobatto, nip points, Linear), fixed bases, lumped floor masses
+    on ux and uy of every floor node, Rayleigh 5 % on modes 1 and 3
+    (alphaM + betaKInit, from the initial stiffness), Newmark 1/2-1/4,
+    Newton / NormUnbalance / BandGeneral, UniformExcitation in x."""
+    import numpy as np
+
+    from .model import MaterialTable, discretize_section
+
+    nodes = []
+    nid = 1
+    node_at = {}
+    for f in range(stories + 1):
+        for c in range(bays + 1):
+            node = {"id": nid, "x": c * bay_w, "y": f * story_h}
+            if f == 0:
+                node["constraints"] = [1, 2, 3]
+            nodes.append(node)
+            node_at[(f, c)] = nid
+            nid += 1
+    elements = []
+    eid = 1
+    for f in range(stories):
+        for c in range(bays + 1):
+            elements.append({"id": eid, "type": "forceBeamColumn2d", "nodes": [node_at[(f, c)], node_at[(f + 1, c)]],
+                             "section": 1, "geomTransf": "Linear", "integration": "Lobatto", "num_int_pts": nip})
+            eid += 1
+    for f in range(1, stories + 1):
+        for c in range(bays):
+            elements.append({"id": eid, "type": "forceBeamColumn2d", "nodes": [node_at[(f, c)], node_at[(f, c + 1)]],
+                             "section": 2, "geomTransf": "Linear", "integration": "Lobatto", "num_int_pts": nip})
+            eid += 1
+    masses = []
+    for f in range(1, stories + 1):
+        for c in range(bays + 1):
+            for dof in (1, 2):
+                masses.append({"node": node_at[(f, c)], "dof": dof, "value": floor_mass})
+    materials = rc_materials()
+    sections = [rc_section(1, dim=2, **FRAME2D_COLUMN), rc_section(2, dim=2, **FRAME2D_BEAM)]
+    # Rayleigh from the initial (elastic-section) stiffness, modes 1 and 3
+    mt = MaterialTable(materials)
+    sec_k = {s["id"]: _section_flex_elastic(discretize_section(s, mt), mt) for s in sections}
+    n_nodes = len(nodes)
+    K = np.zeros((3 * n_nodes, 3 * n_nodes))
+    idx = {n["id"]: i for i, n in enumerate(nodes)}
+    for el in elements:
+        i, j = idx[el["nodes"][0]], idx[el["nodes"][1]]
+        xi, yi, xj, yj = nodes[i]["x"], nodes[i]["y"], nodes[j]["x"], nodes[j]["y"]
+        L = math.hypot(xj - xi, yj - yi)
+        c, s = (xj - xi) / L, (yj - yi) / L
+        EA, ES, EI = sec_k[el["section"]]
+        det = EA * EI - ES * ES
+        f00, f01, f11 = EI / det, -ES / det, EA / det
+        # uniform section flexibility integrates exactly: F = L * int b^T f b
+        F = np.array([[f00 * L, f01 * L * (-0.5), f01 * L * 0.5],
+                      [f01 * L * (-0.5), f11 * L / 3.0, -f11 * L / 6.0],
+                      [f01 * L * 0.5, -f11 * L / 6.0, f11 * L / 3.0]])
+        kb = np.linalg.inv(F)
+        A = np.array([[-1, 0, 0, 1, 0, 0], [0, 1 / L, 1, 0, -1 / L, 0], [0, 1 / L, 0, 0, -1 / L, 1]], dtype=float)
+        kl = A.T @ kb @ A
+        T = np.zeros((6, 6))
+        for b in (0, 3):
+            T[b:b + 2, b:b + 2] = [[c, s], [-s, c]]
+            T[b + 2, b + 2] = 1.0
+        kg = T.T @ kl @ T
+        dofs = [3 * i, 3 * i + 1, 3 * i + 2, 3 * j, 3 * j + 1, 3 * j + 2]
+        K[np.ix_(dofs, dofs)] += kg
+    Mv = np.zeros(3 * n_nodes)
+    for m in masses:
+        Mv[3 * idx[m["node"]] + m["dof"] - 1] += m["value"]
+    free = [k for k in range(3 * n_nodes) if k // 3 >= bays + 1]
+    Kf = K[np.ix_(free, free)]
+    mf = Mv[free]
+    dyn = mf > 0
+    # static condensation of the massless rotations
+    Kdd = Kf[np.ix_(dyn, dyn)] - Kf[np.ix_(dyn, ~dyn)] @ np.linalg.solve(Kf[np.ix_(~dyn, ~dyn)], Kf[np.ix_(~dyn, dyn)])
+    Md = mf[dyn]
+    lam = np.sort(np.linalg.eigvalsh(Kdd / np.sqrt(np.outer(Md, Md))))
+    w1, w3 = math.sqrt(lam[0]), math.sqrt(lam[2])
+    zeta = 0.05
+    alpha_m = 2.0 * zeta * w1 * w3 / (w1 + w3)
+    beta_k = 2.0 * zeta / (w1 + w3)
+    acc = ground_motion(seed, dt, steps, pga_g, strong=strong)
+    return {
+        "schema_version": "1.0",
+        "metadata": {"name": name or f"rc_frame2d_{stories}x{bays}_seed{seed}", "units": "kN-m-t-s"},
+        "model": {"ndm": 2, "ndf": 3},
+        "nodes": nodes,
+        "materials": materials,
+        "sections": sections,
+        "elements": elements,
+        "masses": masses,
+        "time_series": [{"type": "Path", "tag": 1, "dt": dt, "values": [float(v) for v in acc], "factor": 1.0}],
+        "pattern": {"type": "UniformExcitation", "tag": 1, "direction": 1, "accel": 1},
+        "rayleigh": {"alphaM": alpha_m, "betaKInit": beta_k},
+        "analysis": {"type": "transient_nonlinear", "steps": steps, "dt": dt, "tol": tol, "max_iters": max_iters,
+                     "system": "BandGeneral", "numberer": "RCM", "constraints": "Plain", "algorithm": "Newton",
+                     "test_type": "NormUnbalance", "integrator": {"type": "Newmark", "gamma": 0.5, "beta": 0.25}},
+        "recorders": [{"type": "node_displacement", "nodes": [node_at[(stories, 0)]], "dofs": [1, 2, 3],
+                       "output": "roof"}],
+    }
+
+
+def config0_case(steps: int = 1000, seed: int = 0) -> dict:
+    """3 stories x 1 bay, 3 m / 6 m, 50 t floor masses, dt 0.01 s, PGA 0.4 g,
+    2 s rise / 6 s strong / 2 s decay."""
+    return rc_frame2d_case(3, 1, steps=steps, dt=0.01, seed=seed, pga_g=0.4, strong=6.0)

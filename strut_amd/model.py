@@ -15,6 +15,7 @@ device library are the ones Strut builds from the same JSON:
 
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from typing import Dict, List, Sequence, Tuple
 
@@ -259,3 +260,90 @@ def discretize_section(section: dict, materials: MaterialTable) -> np.ndarray:
     for i, c in enumerate(cells):
         out[i] = c
     return out
+
+
+# ---------------------------------------------------------------- beam integration
+def _legendre_p(n: int, x: float):
+    """(P_n, P_{n-1}, P'_n) as beam_integration.mojo:21-45 computes them."""
+    if n == 0:
+        return 1.0, 0.0, 0.0
+    if n == 1:
+        return x, 1.0, 1.0
+    pnm2, pnm1 = 1.0, x
+    for k in range(2, n + 1):
+        kf = float(k)
+        pk = ((2.0 * kf - 1.0) * x * pnm1 - (kf - 1.0) * pnm2) / kf
+        pnm2, pnm1 = pnm1, pk
+    pn, pn1 = pnm1, pnm2
+    denom = x * x - 1.0
+    if abs(denom) <= 1.0e-18:
+        x_off = x - 1.0e-12 if x >= 0.0 else x + 1.0e-12
+        return pn, pn1, _legendre_p(n, x_off)[2]
+    return pn, pn1, float(n) * (x * pn - pn1) / denom
+
+
+def beam_integration_rule(integration: str, n: int):
+    """Integration points on [0, 1] and weights (halved), as
+    beam_integration_rule (beam_integration.mojo:224-241) builds them: the
+    3- and 5-point Lobatto rules are the decimal literals of :63-91."""
+    if integration == "Lobatto":
+        if n < 2:
+            raise ValueError("Lobatto integration requires num_int_pts >= 2")
+        if n == 3:
+            x, w = [-1.0, 0.0, 1.0], [1.0 / 3.0, 4.0 / 3.0, 1.0 / 3.0]
+        elif n == 5:
+            x = [-1.0, -0.6546536707079772, 0.0, 0.6546536707079772, 1.0]
+            w = [0.1, 0.5444444444444444, 0.7111111111111111, 0.5444444444444444, 0.1]
+        else:
+            m, nf = n - 1, float(n)
+            x, w = [0.0] * n, [0.0] * n
+            x[0], x[-1] = -1.0, 1.0
+            w[0] = 2.0 / (nf * (nf - 1.0))
+            w[-1] = w[0]
+            for i in range(1, n - 1):
+                root = math.cos(3.141592653589793 * float(i) / float(n - 1))
+                for _ in range(80):
+                    pm, _, dpm = _legendre_p(m, root)
+                    denom = 1.0 - root * root
+                    if abs(denom) <= 1.0e-18:
+                        break
+                    ddpm = (2.0 * root * dpm - float(m * (m + 1)) * pm) / denom
+                    if abs(ddpm) <= 1.0e-24:
+                        break
+                    delta = dpm / ddpm
+                    root -= delta
+                    if abs(delta) <= 1.0e-14:
+                        break
+                x[i] = root
+                pm_root = _legendre_p(m, root)[0]
+                w[i] = 2.0 / (nf * (nf - 1.0) * pm_root * pm_root)
+            order = sorted(range(n), key=lambda k: x[k])
+            x, w = [x[k] for k in order], [w[k] for k in order]
+    elif integration == "Legendre":
+        if n < 1:
+            raise ValueError("Legendre integration requires num_int_pts >= 1")
+        if n == 1:
+            x, w = [0.0], [2.0]
+        else:
+            x, w = [0.0] * n, [0.0] * n
+            for i in range((n + 1) // 2):
+                root = math.cos(3.141592653589793 * (float(i) + 0.75) / (float(n) + 0.5))
+                dpn = 0.0
+                for _ in range(80):
+                    pn, _, dpn = _legendre_p(n, root)
+                    if abs(dpn) <= 1.0e-24:
+                        break
+                    delta = pn / dpn
+                    root -= delta
+                    if abs(delta) <= 1.0e-14:
+                        break
+                one_minus = 1.0 - root * root
+                weight = 2.0 / (one_minus * dpn * dpn)
+                x[i], w[i] = -root, weight
+                x[n - 1 - i], w[n - 1 - i] = root, weight
+            order = sorted(range(n), key=lambda k: x[k])
+            x, w = [x[k] for k in order], [w[k] for k in order]
+    else:
+        raise ValueError(f"unsupported beam integration: {integration}")
+    return [0.5 * (xi + 1.0) for xi in x], [wi * 0.5 for wi in w]
+
