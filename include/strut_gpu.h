@@ -152,6 +152,84 @@ int sgpu_fiber_state_read(sgpu_ctx *ctx, int32_t def_id, int32_t instance, int w
  * count, benchmarks_micro.mojo:250) and total slots. */
 int sgpu_fiber_totals(sgpu_ctx *ctx, int64_t *fibers, int64_t *slots);
 
+/* ------------------------------------------------------------------
+ * Transient nonlinear analysis of 2D forceBeamColumn fiber frames.
+ * One device workgroup runs one (frame, ground motion) instance's whole
+ * Newmark / Newton loop; many instances run in one launch.
+ * ------------------------------------------------------------------ */
+
+/* A 2D frame after loading (solver/run_case/loader.mojo:758-2460): nodes,
+ * plain DOF numbering (3 per node), constraint flags, lumped masses
+ * (M_total), forceBeamColumn2d elements with a fiber-section definition
+ * and integration rule each (10 slots per element in elem_xi / elem_w,
+ * points on [0, 1], weights halved: beam_integration.mojo:224-241), and the
+ * transient_nonlinear analysis record: Newmark gamma / beta, NormUnbalance
+ * tolerance, Newton iteration limit, Rayleigh alphaM / betaK / betaKInit and
+ * the UniformExcitation direction (input_types.mojo:1821-2290).  Recorded
+ * DOFs are global DOF indices. */
+typedef struct {
+    int32_t n_nodes;
+    const double *x;
+    const double *y;
+    const int32_t *constrained;
+    const double *mass;
+    int32_t n_elem;
+    const int32_t *elem_nodes;
+    const int32_t *elem_def;
+    const int32_t *elem_nip;
+    const double *elem_xi;
+    const double *elem_w;
+    double dt, gamma, beta, tol;
+    int32_t max_iters;
+    double alpha_m, beta_k, beta_k_init;
+    int32_t exc_dir;
+    int32_t n_record;
+    const int32_t *record_dofs;
+} sgpu_frame2d_desc;
+
+/* Path time series with a constant dt (time_series.mojo:71-96). */
+typedef struct {
+    int32_t n;
+    const double *values;
+    double dt, factor, start_time;
+    int32_t use_last;
+} sgpu_path_series;
+
+/* Register a frame.  Replaces the set-up half of run_transient_nonlinear
+ * (transient_nonlinear.mojo:830-1700): Newmark constants, the BandGeneral
+ * structure from the element free maps (linear_solver_backend.mojo:
+ * 1481-1506, :1664-1676) and the element-to-band gather in element order
+ * (stiffness_internal_frame2d.mojo:594-613). */
+int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *desc, int32_t *frame_id);
+
+/* Add one instance of a frame driven by one ground-acceleration record for
+ * n_steps steps; allocates its fiber-section instances.  The acceleration is
+ * evaluated at t = (step + 1) dt as transient_nonlinear.mojo:1800-1852 does. */
+int sgpu_frame2d_add_instance(sgpu_ctx *ctx, int32_t frame_id, int32_t n_steps, const sgpu_path_series *accel,
+                              int32_t *instance_id);
+
+/* Freeze frames and instances (finalizes the sections too) and upload. */
+int sgpu_frame2d_finalize(sgpu_ctx *ctx);
+
+/* Advance every unfinished instance by up to steps_per_launch steps
+ * (asynchronous), or run all of them to the end (synchronous).  Replaces the
+ * step loop of run_transient_nonlinear (transient_nonlinear.mojo:1698-3110)
+ * with the Newton loop of :1959-2846. */
+int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t steps_per_launch);
+int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t steps_per_launch);
+
+/* Per-step recorded displacements (n_steps x n_record), per-step Newton
+ * iteration counts (assemblies, including the converged one), status
+ * (0 = ran to the end; k > 0 = step k did not converge, where the reference
+ * aborts; -10 = BandGeneral needed a row swap, where the reference switches
+ * to its dense LU; -11 = singular band pivot; -12 = element tangent
+ * recovery failed) and steps completed.  Any pointer may be NULL. */
+int sgpu_frame2d_result(sgpu_ctx *ctx, int32_t instance, double *u_hist, int32_t *iters, int32_t *status,
+                        int32_t *steps_done);
+
+/* out4 = {free DOFs, band half-width, recorded DOFs, total DOFs}. */
+int sgpu_frame2d_info(sgpu_ctx *ctx, int32_t frame_id, int32_t *out4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,84 +17,10 @@
 
 #include "../../include/strut_gpu.h"
 #include "material_laws.h"
+#include "section_device.h"
 #include "strut_gpu_types.h"
 
 namespace sgpu {
-
-constexpr int kBatch = 64;
-
-__device__ __forceinline__ double tree8(double v) {
-    double w = v + __shfl_xor(v, 4, 64);
-    double x = w + __shfl_xor(w, 2, 64);
-    return x + __shfl_xor(x, 1, 64);
-}
-
-__device__ __forceinline__ void load_hist(const StateArrays &s, int64_t gi, int64_t ai, int type,
-                                          FiberHist &h) {
-    h.a0 = h.a1 = h.a2 = h.a3 = h.a4 = 0.0;
-    h.kon = 0;
-    h.eps = s.eps[gi];
-    h.sig = s.sig[gi];
-    h.h0 = s.h0[gi];
-    h.h1 = s.h1[gi];
-    if (type == kSteel02) {
-        h.tan = 0.0;  // Steel02 never reads its committed tangent
-        h.a0 = s.a0[ai];
-        h.a1 = s.a1[ai];
-        h.a2 = s.a2[ai];
-        h.a3 = s.a3[ai];
-        h.a4 = s.a4[ai];
-        h.kon = s.kon[ai];
-    } else {
-        h.tan = s.tan[gi];
-        if (type == kConcrete01) h.a0 = s.a0[ai];
-    }
-}
-
-__device__ __forceinline__ void store_hist(const StateArrays &s, int64_t gi, int64_t ai, int type,
-                                           const FiberHist &h) {
-    s.eps[gi] = h.eps;
-    s.sig[gi] = h.sig;
-    s.tan[gi] = h.tan;
-    s.h0[gi] = h.h0;
-    s.h1[gi] = h.h1;
-    if (type == kSteel02) {
-        s.a0[ai] = h.a0;
-        s.a1[ai] = h.a1;
-        s.a2[ai] = h.a2;
-        s.a3[ai] = h.a3;
-        s.a4[ai] = h.a4;
-        s.kon[ai] = h.kon;
-    } else if (type == kConcrete01) {
-        s.a0[ai] = h.a0;
-    }
-}
-
-// Evaluate one slot: returns (fs, ks) = (sigma * A, E_t * A) for material
-// fibers and (E A strain, E A) for elastic fibers (fiber2d.mojo:3480-3484).
-__device__ __forceinline__ void eval_slot(const MatDev *__restrict__ mats, int mi, double strain,
-                                          double area, const StateArrays &c, const StateArrays &t,
-                                          int64_t gi, int64_t ai, double &fs, double &ks) {
-    const MatDev &m = mats[mi];
-    const int type = m.type;
-    if (type == kElastic) {
-        ks = m.p[0] * area;
-        fs = ks * strain;
-        return;
-    }
-    FiberHist hc, ht;
-    load_hist(c, gi, ai, type, hc);
-    ht = hc;
-    switch (type) {
-    case kConcrete02: concrete02_trial(m, hc, strain, ht); break;
-    case kSteel02: steel02_trial(m, hc, strain, ht); break;
-    case kSteel01: steel01_trial(m, hc, strain, ht); break;
-    default: concrete01_trial(m, hc, strain, ht); break;
-    }
-    store_hist(t, gi, ai, type, ht);
-    fs = ht.sig * area;
-    ks = ht.tan * area;
-}
 
 template <int DIM>
 __device__ void section_point(const SecDefDev &d, const SlotTables &tb, const MatDev *__restrict__ mats,

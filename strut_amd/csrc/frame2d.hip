@@ -1,0 +1,813 @@
+// Transient nonlinear analysis of 2D forceBeamColumn fiber frames on gfx950.
+//
+// One workgroup (4 wavefronts) owns one (frame, ground motion) instance and
+// runs its Newmark / Newton loop on the device for a chunk of steps
+// (transient_nonlinear.mojo:1698-3110): element state determination with one
+// wavefront per element (the forceBeamColumn2d local iteration of
+// force_beam_column2d.mojo:804-1919, its fiber sections evaluated wave-wide),
+// an atomic-free gather of the element matrices into the band in element
+// order (the per-entry summation order of the reference's serial scatter,
+// stiffness_internal_frame2d.mojo:594-613), the Rayleigh damping and Newmark
+// residual, the NormUnbalance test, the BandGeneral LU without pivoting
+// (linear_solver_backend.mojo:631-679) and its solve (:735-769) in LDS, then
+// commit.  Every floating-point expression keeps the reference's operation
+// order, so an instance reproduces the CPU restatement's iterates.
+#include <hip/hip_runtime.h>
+
+#include "frame2d_types.h"
+#include "section_device.h"
+#include "strut_gpu_types.h"
+
+namespace sgpu {
+
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// force_beam_column2d.mojo:223-272
+__device__ __forceinline__ bool invert3(double a, double b, double c, double d, double e, double f, double g,
+                                        double h, double i, double *o) {
+    const double co00 = e * i - f * h;
+    const double co01 = -(d * i - f * g);
+    const double co02 = d * h - e * g;
+    const double co10 = -(b * i - c * h);
+    const double co11 = a * i - c * g;
+    const double co12 = -(a * h - b * g);
+    const double co20 = b * f - c * e;
+    const double co21 = -(a * f - c * d);
+    const double co22 = a * e - b * d;
+    const double det = a * co00 + b * co01 + c * co02;
+    if (fabs(det) <= 1.0e-40) return false;
+    const double inv_det = 1.0 / det;
+    o[0] = co00 * inv_det;
+    o[1] = co10 * inv_det;
+    o[2] = co20 * inv_det;
+    o[3] = co01 * inv_det;
+    o[4] = co11 * inv_det;
+    o[5] = co21 * inv_det;
+    o[6] = co02 * inv_det;
+    o[7] = co12 * inv_det;
+    o[8] = co22 * inv_det;
+    return true;
+}
+
+struct ElemCtx {
+    const Frame2dArgs *A;
+    const ElemDev *el;
+    int inst0;  // section instance of integration point 0
+};
+
+// fiber_section2d_batch_eval_same_def (fiber2d.mojo:3042-3086): points in
+// order, stop at the first singular flexibility.
+__device__ bool eval_sections(const ElemCtx &E, const double *pe, const double *pk, double *ax, double *mo,
+                              double *f00, double *f01, double *f11) {
+    const SecDefDev &d = E.A->defs[E.el->def];
+#pragma unroll
+    for (int ip = 0; ip < kMaxNipDev; ++ip) {
+        if (ip >= E.el->nip) break;
+        const Sec2dOut r =
+            wave_section_2d(d, E.A->tb, E.A->mats, E.A->c, E.A->t, E.inst0 + ip, pe[ip], pk[ip]);
+        ax[ip] = r.N;
+        mo[ip] = r.Mz;
+        const double det = r.k11 * r.k22 - r.k12 * r.k12;
+        if (fabs(det) <= 1.0e-40) return false;
+        const double inv_det = 1.0 / det;
+        f00[ip] = r.k22 * inv_det;
+        f01[ip] = -r.k12 * inv_det;
+        f11[ip] = r.k11 * inv_det;
+    }
+    return true;
+}
+
+struct FbcScratch {
+    double vs_e[kMaxNipDev], vs_k[kMaxNipDev];
+    double sub_e[kMaxNipDev], sub_k[kMaxNipDev];
+    double sub_ax[kMaxNipDev], sub_mo[kMaxNipDev];
+    double sub_f00[kMaxNipDev], sub_f01[kMaxNipDev], sub_f11[kMaxNipDev];
+};
+
+// force_beam_column2d.mojo:804-1189 (try_increment); best = {q0,q1,q2,k00,k01,k02,k10,k11,k12,k22}
+__device__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, double bv0, double bv1, double bv2,
+                              double dv0, double dv1, double dv2, int scheme, double *best) {
+    const ElemDev &el = *E.el;
+    const int nip = el.nip;
+    const double L = el.L;
+    double q0 = q[0], q1 = q[1], q2 = q[2];
+    const double tv0 = bv0 + dv0, tv1 = bv1 + dv1, tv2 = bv2 + dv2;
+    if (!el.if_valid) return false;
+    double pe[kMaxNipDev], pk[kMaxNipDev], ax[kMaxNipDev], mo[kMaxNipDev], f00[kMaxNipDev], f01[kMaxNipDev],
+        f11[kMaxNipDev];
+#pragma unroll
+    for (int ip = 0; ip < kMaxNipDev; ++ip) {
+        if (ip < nip) {
+            pe[ip] = q[3 + ip];
+            pk[ip] = q[3 + nip + ip];
+            sc.vs_e[ip] = pe[ip];
+            sc.vs_k[ip] = pk[ip];
+        }
+    }
+    if (!eval_sections(E, pe, pk, ax, mo, f00, f01, f11)) return false;
+    double pf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ip = 0; ip < kMaxNipDev; ++ip) {
+        if (ip < nip) {
+            sc.sub_e[ip] = sc.vs_e[ip];
+            sc.sub_k[ip] = sc.vs_k[ip];
+            sc.sub_ax[ip] = ax[ip];
+            sc.sub_mo[ip] = mo[ip];
+            sc.sub_f00[ip] = f00[ip];
+            sc.sub_f01[ip] = f01[ip];
+            sc.sub_f11[ip] = f11[ip];
+            const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+            const double a00 = f00[ip], a01 = f01[ip], a10 = a01, a11 = f11[ip];
+            pf[0] += wL * a00;
+            pf[1] += wL * a01 * b_mi;
+            pf[2] += wL * a01 * b_mj;
+            pf[3] += wL * b_mi * a10;
+            pf[4] += wL * b_mi * a11 * b_mi;
+            pf[5] += wL * b_mi * a11 * b_mj;
+            pf[6] += wL * b_mj * a10;
+            pf[7] += wL * b_mj * a11 * b_mi;
+            pf[8] += wL * b_mj * a11 * b_mj;
+        }
+    }
+    double pk3[9];
+    if (invert3(pf[0], pf[1], pf[2], pf[3], pf[4], pf[5], pf[6], pf[7], pf[8], pk3)) {
+        q0 += pk3[0] * dv0 + pk3[1] * dv1 + pk3[2] * dv2;
+        q1 += pk3[3] * dv0 + pk3[4] * dv1 + pk3[5] * dv2;
+        q2 += pk3[6] * dv0 + pk3[7] * dv1 + pk3[8] * dv2;
+    }
+    const int n_iters = scheme == 1 ? 100 : 10;
+    for (int it = 0; it < n_iters; ++it) {
+        const bool use_init = scheme == 1 || (scheme == 2 && it == 0);
+#pragma unroll
+        for (int ip = 0; ip < kMaxNipDev; ++ip) {
+            if (ip < nip) {
+                const double xi = el.xi[ip], b_mi = xi - 1.0, b_mj = xi;
+                const double ss0 = q0 + 0.0;
+                const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
+                const double dss0 = ss0 - sc.sub_ax[ip];
+                const double dss1 = ss1 - sc.sub_mo[ip];
+                const double s00 = use_init ? el.if00 : sc.sub_f00[ip];
+                const double s01 = use_init ? el.if01 : sc.sub_f01[ip];
+                const double s11 = use_init ? el.if11 : sc.sub_f11[ip];
+                const double dvs0 = s00 * dss0 + s01 * dss1;
+                const double dvs1 = s01 * dss0 + s11 * dss1;
+                pe[ip] = sc.sub_e[ip] + dvs0;
+                pk[ip] = sc.sub_k[ip] + dvs1;
+            }
+        }
+        if (!eval_sections(E, pe, pk, ax, mo, f00, f01, f11)) return false;
+        double f[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double vr0 = 0.0, vr1 = 0.0, vr2 = 0.0;
+#pragma unroll
+        for (int ip = 0; ip < kMaxNipDev; ++ip) {
+            if (ip < nip) {
+                const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+                const double ss0 = q0 + 0.0;
+                const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
+                sc.sub_e[ip] = pe[ip];
+                sc.sub_k[ip] = pk[ip];
+                sc.sub_ax[ip] = ax[ip];
+                sc.sub_mo[ip] = mo[ip];
+                const double a00 = f00[ip], a01 = f01[ip], a11 = f11[ip], a10 = a01;
+                sc.sub_f00[ip] = a00;
+                sc.sub_f01[ip] = a01;
+                sc.sub_f11[ip] = a11;
+                const double dr0 = ss0 - ax[ip];
+                const double dr1 = ss1 - mo[ip];
+                const double dv_0 = a00 * dr0 + a01 * dr1;
+                const double dv_1 = a01 * dr0 + a11 * dr1;
+                f[0] += wL * a00;
+                f[1] += wL * a01 * b_mi;
+                f[2] += wL * a01 * b_mj;
+                f[3] += wL * b_mi * a10;
+                f[4] += wL * b_mi * a11 * b_mi;
+                f[5] += wL * b_mi * a11 * b_mj;
+                f[6] += wL * b_mj * a10;
+                f[7] += wL * b_mj * a11 * b_mi;
+                f[8] += wL * b_mj * a11 * b_mj;
+                vr0 += wL * (pe[ip] + dv_0);
+                vr1 += wL * b_mi * (pk[ip] + dv_1);
+                vr2 += wL * b_mj * (pk[ip] + dv_1);
+            }
+        }
+        double k[9];
+        if (!invert3(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], k)) return false;
+        const double r0 = tv0 - vr0, r1 = tv1 - vr1, r2 = tv2 - vr2;
+        const double dq0 = k[0] * r0 + k[1] * r1 + k[2] * r2;
+        const double dq1 = k[3] * r0 + k[4] * r1 + k[5] * r2;
+        const double dq2 = k[6] * r0 + k[7] * r1 + k[8] * r2;
+        const double work = fabs(r0 * dq0 + r1 * dq1 + r2 * dq2);
+        q0 += dq0;
+        q1 += dq1;
+        q2 += dq2;
+        if (work < 1.0e-12) {
+            q[0] = q0;
+            q[1] = q1;
+            q[2] = q2;
+#pragma unroll
+            for (int ip = 0; ip < kMaxNipDev; ++ip) {
+                if (ip < nip) {
+                    q[3 + ip] = sc.sub_e[ip];
+                    q[3 + nip + ip] = sc.sub_k[ip];
+                    sc.vs_e[ip] = sc.sub_e[ip];
+                    sc.vs_k[ip] = sc.sub_k[ip];
+                }
+            }
+            best[0] = q0;
+            best[1] = q1;
+            best[2] = q2;
+            best[3] = k[0];
+            best[4] = k[1];
+            best[5] = k[2];
+            best[6] = k[3];
+            best[7] = k[4];
+            best[8] = k[5];
+            best[9] = k[8];
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ double max_abs3(double a, double b, double c) {
+    double m = fabs(a);
+    if (fabs(b) > m) m = fabs(b);
+    if (fabs(c) > m) m = fabs(c);
+    return m;
+}
+
+__device__ __forceinline__ void restore_predictor(double *q, int nip, double q0, double q1, double q2,
+                                                  const FbcScratch &sc) {
+    q[0] = q0;
+    q[1] = q1;
+    q[2] = q2;
+#pragma unroll
+    for (int ip = 0; ip < kMaxNipDev; ++ip) {
+        if (ip < nip) {
+            q[3 + ip] = sc.vs_e[ip];
+            q[3 + nip + ip] = sc.vs_k[ip];
+        }
+    }
+}
+
+// Local stiffness entry before the transformation (force_beam_column2d.mojo:1882-1888).
+__device__ __forceinline__ void basic_col(int a, double inv_L, double &a0, double &a1, double &a2) {
+    a0 = a == 0 ? -1.0 : (a == 3 ? 1.0 : 0.0);
+    a1 = (a == 1) ? inv_L : (a == 4 ? -inv_L : (a == 2 ? 1.0 : 0.0));
+    a2 = (a == 1) ? inv_L : (a == 4 ? -inv_L : (a == 5 ? 1.0 : 0.0));
+}
+
+__device__ __forceinline__ double klocal(const double *kb, double inv_L, int a, int b) {
+    double a0, a1, a2, b0, b1, b2;
+    basic_col(a, inv_L, a0, a1, a2);
+    basic_col(b, inv_L, b0, b1, b2);
+    const double k00 = kb[0], k01 = kb[1], k02 = kb[2], k10 = kb[3], k11 = kb[4], k12 = kb[5];
+    const double k20 = k02, k21 = k12, k22 = kb[6];
+    return a0 * (k00 * b0 + k01 * b1 + k02 * b2) + a1 * (k10 * b0 + k11 * b1 + k12 * b2) +
+           a2 * (k20 * b0 + k21 * b1 + k22 * b2);
+}
+
+// utils.mojo:172-238: column pass (rows keep their index) then row pass.
+__device__ __forceinline__ double kcol(const double *kb, double inv_L, double c, double s, int a, int b) {
+    switch (b) {
+    case 0: return c * klocal(kb, inv_L, a, 0) - s * klocal(kb, inv_L, a, 1);
+    case 1: return s * klocal(kb, inv_L, a, 0) + c * klocal(kb, inv_L, a, 1);
+    case 3: return c * klocal(kb, inv_L, a, 3) - s * klocal(kb, inv_L, a, 4);
+    case 4: return s * klocal(kb, inv_L, a, 3) + c * klocal(kb, inv_L, a, 4);
+    default: return klocal(kb, inv_L, a, b);
+    }
+}
+
+__device__ __forceinline__ double kglobal(const double *kb, double inv_L, double c, double s, int a, int b) {
+    switch (a) {
+    case 0: return c * kcol(kb, inv_L, c, s, 0, b) - s * kcol(kb, inv_L, c, s, 1, b);
+    case 1: return s * kcol(kb, inv_L, c, s, 0, b) + c * kcol(kb, inv_L, c, s, 1, b);
+    case 3: return c * kcol(kb, inv_L, c, s, 3, b) - s * kcol(kb, inv_L, c, s, 4, b);
+    case 4: return s * kcol(kb, inv_L, c, s, 3, b) + c * kcol(kb, inv_L, c, s, 4, b);
+    default: return kcol(kb, inv_L, c, s, a, b);
+    }
+}
+
+// force_beam_column2d.mojo:1418-1919 for one element, executed by one whole
+// wavefront; lanes 0..35 store k_global, lanes 36..41 f_global.  Returns
+// false when the reference would abort (no recoverable tangent).
+__device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q,
+                              const double *uf, double *out42, const FrameDev &F) {
+    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e]};
+    const int nip = el.nip;
+    const double L = el.L, inv_L = el.inv_L, c = el.c, s = el.s;
+    double ug[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ug[k] = el.fmap[k] >= 0 ? uf[el.fmap[k]] : 0.0;
+    double ul[6];
+    ul[0] = c * ug[0] + s * ug[1];
+    ul[1] = -s * ug[0] + c * ug[1];
+    ul[2] = ug[2];
+    ul[3] = c * ug[3] + s * ug[4];
+    ul[4] = -s * ug[3] + c * ug[4];
+    ul[5] = ug[5];
+    const int boff = 3 + 2 * nip;
+    const double chord = (ul[4] - ul[1]) / L;
+    const double v0 = ul[3] - ul[0], v1 = ul[2] - chord, v2 = ul[5] - chord;
+    double ab0 = q[boff], ab1 = q[boff + 1], ab2 = q[boff + 2];
+    double aq0 = q[0], aq1 = q[1], aq2 = q[2];
+    FbcScratch sc;
+#pragma unroll
+    for (int ip = 0; ip < kMaxNipDev; ++ip) {
+        sc.vs_e[ip] = ip < nip ? q[3 + ip] : 0.0;
+        sc.vs_k[ip] = ip < nip ? q[3 + nip + ip] : 0.0;
+    }
+    double rem0 = v0 - ab0, rem1 = v1 - ab1, rem2 = v2 - ab2;
+    double at0 = rem0, at1 = rem1, at2 = rem2;
+    bool converged = false, have_best = false;
+    double best[10];
+    int num_sub = 1;
+    if (max_abs3(rem0, rem1, rem2) <= 1.0e-16) {
+        for (int scheme = 0; scheme < 3; ++scheme) {
+            restore_predictor(q, nip, aq0, aq1, aq2, sc);
+            q[boff] = ab0;
+            q[boff + 1] = ab1;
+            q[boff + 2] = ab2;
+            if (!try_increment(E, q, sc, ab0, ab1, ab2, 0.0, 0.0, 0.0, scheme, best)) continue;
+            have_best = true;
+            converged = true;
+            break;
+        }
+    } else {
+        while (true) {
+            if (num_sub > 4) break;
+            const double tv0 = ab0 + at0, tv1 = ab1 + at1, tv2 = ab2 + at2;
+            bool scheme_success = false;
+            for (int scheme = 0; scheme < 3; ++scheme) {
+                restore_predictor(q, nip, aq0, aq1, aq2, sc);
+                q[boff] = ab0;
+                q[boff + 1] = ab1;
+                q[boff + 2] = ab2;
+                double solved[10];
+                if (!try_increment(E, q, sc, ab0, ab1, ab2, at0, at1, at2, scheme, solved)) continue;
+                ab0 = tv0;
+                ab1 = tv1;
+                ab2 = tv2;
+                q[boff] = ab0;
+                q[boff + 1] = ab1;
+                q[boff + 2] = ab2;
+                aq0 = solved[0];
+                aq1 = solved[1];
+                aq2 = solved[2];
+#pragma unroll
+                for (int ip = 0; ip < kMaxNipDev; ++ip) {
+                    if (ip < nip) {
+                        sc.vs_e[ip] = q[3 + ip];
+                        sc.vs_k[ip] = q[3 + nip + ip];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 10; ++k) best[k] = solved[k];
+                have_best = true;
+                rem0 = v0 - ab0;
+                rem1 = v1 - ab1;
+                rem2 = v2 - ab2;
+                if (max_abs3(rem0, rem1, rem2) <= 1.0e-12) {
+                    converged = true;
+                } else {
+                    at0 = rem0;
+                    at1 = rem1;
+                    at2 = rem2;
+                    num_sub = 1;
+                }
+                scheme_success = true;
+                break;
+            }
+            if (converged) break;
+            if (scheme_success) continue;
+            at0 /= 10.0;
+            at1 /= 10.0;
+            at2 /= 10.0;
+            num_sub += 1;
+        }
+    }
+    if (!converged) {
+        restore_predictor(q, nip, aq0, aq1, aq2, sc);
+        q[boff] = ab0;
+        q[boff + 1] = ab1;
+        q[boff + 2] = ab2;
+        const SecDefDev &d = A.defs[el.def];
+        for (int ip = 0; ip < nip; ++ip)
+            (void)wave_section_2d(d, A.tb, A.mats, A.c, A.t, E.inst0 + ip, sc.vs_e[ip], sc.vs_k[ip]);
+    }
+    if (!have_best) {
+        // force_beam_column2d.mojo:735-801 (initial basic tangent)
+        double f[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int ip = 0; ip < nip; ++ip) {
+            const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+            const double a00 = el.if00, a01 = el.if01, a10 = a01, a11 = el.if11;
+            f[0] += wL * a00;
+            f[1] += wL * a01 * b_mi;
+            f[2] += wL * a01 * b_mj;
+            f[3] += wL * b_mi * a10;
+            f[4] += wL * b_mi * a11 * b_mi;
+            f[5] += wL * b_mi * a11 * b_mj;
+            f[6] += wL * b_mj * a10;
+            f[7] += wL * b_mj * a11 * b_mi;
+            f[8] += wL * b_mj * a11 * b_mj;
+        }
+        double kt[9];
+        if (!el.if_valid || !invert3(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], kt)) return false;
+        best[0] = q[0];
+        best[1] = q[1];
+        best[2] = q[2];
+        best[3] = kt[0];
+        best[4] = kt[1];
+        best[5] = kt[2];
+        best[6] = kt[3];
+        best[7] = kt[4];
+        best[8] = kt[5];
+        best[9] = kt[8];
+    }
+    const double kb[7] = {best[3], best[4], best[5], best[6], best[7], best[8], best[9]};
+    const int lane = threadIdx.x & 63;
+    if (lane < 36) {
+        out42[lane] = kglobal(kb, inv_L, c, s, lane / 6, lane % 6);
+    } else if (lane < 42) {
+        const double q0 = best[0], q1 = best[1], q2 = best[2];
+        double fl[6];
+        fl[0] = -q0;
+        fl[1] = (q1 + q2) * inv_L;
+        fl[2] = q1;
+        fl[3] = q0;
+        fl[4] = -(q1 + q2) * inv_L;
+        fl[5] = q2;
+        fl[0] += 0.0;
+        fl[1] += 0.0;
+        fl[4] += 0.0;
+        const int a = lane - 36;
+        double v;
+        if (a == 0) v = c * fl[0] - s * fl[1];
+        else if (a == 1) v = s * fl[0] + c * fl[1];
+        else if (a == 3) v = c * fl[3] - s * fl[4];
+        else if (a == 4) v = s * fl[3] + c * fl[4];
+        else v = fl[a];
+        out42[lane] = v;
+    }
+    return true;
+}
+
+__device__ __forceinline__ double tree8_seq(const double *v) {
+    const double w0 = v[0] + v[4], w1 = v[1] + v[5], w2 = v[2] + v[6], w3 = v[3] + v[7];
+    const double x0 = w0 + w2, x1 = w1 + w3;
+    return x0 + x1;
+}
+
+// Copy the instance's committed <- trial (commit) or trial <- committed.
+__device__ void copy_instance_state(const Frame2dArgs &A, const InstDev &I, bool commit) {
+    const StateArrays &dst = commit ? A.c : A.t;
+    const StateArrays &src = commit ? A.t : A.c;
+    for (int r = 0; r < I.n_ranges; ++r) {
+        for (int64_t k = threadIdx.x; k < I.slot_count[r]; k += blockDim.x) {
+            const int64_t g = I.slot_begin[r] + k;
+            dst.eps[g] = src.eps[g];
+            dst.sig[g] = src.sig[g];
+            dst.tan[g] = src.tan[g];
+            dst.h0[g] = src.h0[g];
+            dst.h1[g] = src.h1[g];
+        }
+        for (int64_t k = threadIdx.x; k < I.aux_count[r]; k += blockDim.x) {
+            const int64_t g = I.aux_begin[r] + k;
+            dst.a0[g] = src.a0[g];
+            dst.a1[g] = src.a1[g];
+            dst.a2[g] = src.a2[g];
+            dst.a3[g] = src.a3[g];
+            dst.a4[g] = src.a4[g];
+            dst.kon[g] = src.kon[g];
+        }
+    }
+}
+
+// All element states of the instance, then the raw K_ff band (gather in
+// element order) into `band` and F_int at the free DOFs into `fint`.
+// Returns false (block-uniform) when an element failed.
+__device__ bool assemble(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, const double *uf, double *band,
+                         double *fint, int *flag) {
+    const int wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    double *out = A.scratch + I.scratch_base;
+    for (int e = wave; e < F.n_elem; e += n_waves) {
+        const ElemDev &el = A.elems[F.elem_base + e];
+        const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, uf, out + 42 * e, F);
+        if (!ok && (threadIdx.x & 63) == 0) *flag = 1;
+    }
+    __syncthreads();
+    if (*flag) return false;
+    const int n = F.n_free, bw = F.bw, hbw = F.hbw;
+    const int32_t *off = A.kg_off + F.kg_off_base;
+    for (int sl = threadIdx.x; sl < n * bw; sl += blockDim.x) {
+        double v = 0.0;
+        for (int k = off[sl]; k < off[sl + 1]; ++k) v += out[A.kg[k]];
+        band[sl] = v;
+    }
+    const int32_t *foff = A.fg_off + F.fg_off_base;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double v = 0.0;
+        for (int k = foff[i]; k < foff[i + 1]; ++k) v += out[A.fg[k]];
+        fint[i] = v;
+    }
+    (void)hbw;
+    __syncthreads();
+    return true;
+}
+
+__global__ __launch_bounds__(256) void frame2d_kernel(Frame2dArgs A, int steps_per_launch) {
+    extern __shared__ double lds[];
+    __shared__ int flag;
+    __shared__ double s_norm, s_diag;
+    __shared__ int s_stat;
+    const int bid = blockIdx.x;
+    const InstDev I = A.insts[bid];
+    const FrameDev F = A.frames[I.frame];
+    InstState st = A.state[bid];
+    if (st.status != 0 || st.step >= I.n_steps) return;
+    const int n = F.n_free, bw = F.bw, hbw = F.hbw;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double *uf = lds, *vf = uf + n, *af = vf + n, *un = af + n, *vn = un + n, *an = vn + n;
+    double *P = an + n, *R = P + n, *du = R + n, *vt = du + n, *at = vt + n, *fint = at + n, *work = fint + n;
+    double *band = work + n;  // n * bw
+    const double *mass = A.mass_ff + F.free_base;
+    double *vec = A.vec + I.vec_base;
+    for (int i = tid; i < n; i += nt) {
+        uf[i] = vec[i];
+        vf[i] = vec[n + i];
+        af[i] = vec[2 * n + i];
+    }
+    if (tid == 0) flag = 0;
+    __syncthreads();
+    if (!st.setup_done) {
+        // initial assembly at u (transient_nonlinear.mojo:1207), K_init (:1329-1337), revert (:1452-1453)
+        if (!assemble(A, I, F, uf, band, fint, &flag)) {
+            if (tid == 0) A.state[bid].status = kInstElementFail;
+            return;
+        }
+        double *ki = A.kinit + I.kinit_base;
+        for (int sl = tid; sl < n * bw; sl += nt) ki[sl] = band[sl];
+        copy_instance_state(A, I, false);
+        __syncthreads();
+        st.setup_done = 1;
+    }
+    const double *kinit = A.kinit + I.kinit_base;
+    int steps_run = 0;
+    while (steps_run < steps_per_launch && st.step < I.n_steps) {
+        const int step = st.step;
+        for (int i = tid; i < n; i += nt) {  // :1757-1766, :553-590
+            un[i] = uf[i];
+            vn[i] = vf[i];
+            an[i] = af[i];
+            uf[i] = un[i] + F.dt * vn[i] + F.coeff * an[i];
+        }
+        const double ag = A.ag[I.ag_base + step];
+        for (int i = tid; i < n; i += nt) {  // :1823-1852
+            double load = 0.0;
+            if (A.exc_flag[F.free_base + i]) load += -A.exc_ff[F.free_base + i] * ag;
+            double fe = 0.0;
+            fe += load;
+            P[i] = fe;
+        }
+        __syncthreads();
+        bool converged = false;
+        int iters = 0;
+        for (int it = 0; it < F.max_iters; ++it) {
+            ++iters;
+            if (!assemble(A, I, F, uf, band, fint, &flag)) {
+                if (tid == 0) A.state[bid].status = kInstElementFail;
+                return;
+            }
+            for (int i = tid; i < n; i += nt) {  // :496-531
+                const double anext = F.a0 * (uf[i] - un[i]) - F.a2 * vn[i] - F.a3 * an[i];
+                at[i] = anext;
+                vt[i] = vn[i] + F.dt * (F.omg * an[i] + F.gamma * anext);
+            }
+            __syncthreads();
+            // residual (:2337-2356): damping row dot in W-chunks over the band, inertia M_ii a_i
+            for (int i = tid; i < n; i += nt) {
+                double damping = 0.0;
+                if (F.has_damping) {
+                    const int full = (n / 8) * 8;
+                    int j0 = i - hbw < 0 ? 0 : i - hbw;
+                    int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+                    double sum = 0.0;
+                    for (int cb = (j0 / 8) * 8; cb < full && cb <= j1; cb += 8) {
+                        double v[8];
+#pragma unroll
+                        for (int l = 0; l < 8; ++l) {
+                            const int j = cb + l;
+                            double cij = 0.0;
+                            if (j >= j0 && j <= j1) {
+                                const int sl = i * bw + (j - i + hbw);
+                                const double mij = i == j ? mass[i] : 0.0;
+                                double cc = 0.0;
+                                if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
+                                if (F.beta_k != 0.0) cc += F.beta_k * band[sl];
+                                if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
+                                cij = cc;
+                            }
+                            v[l] = (cij * vt[j]) * 1.0;
+                        }
+                        sum += tree8_seq(v);
+                    }
+                    for (int j = full > j0 ? full : j0; j <= j1; ++j) {
+                        const int sl = i * bw + (j - i + hbw);
+                        const double mij = i == j ? mass[i] : 0.0;
+                        double cc = 0.0;
+                        if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
+                        if (F.beta_k != 0.0) cc += F.beta_k * band[sl];
+                        if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
+                        sum += (cc * vt[j]) * 1.0;
+                    }
+                    damping = sum;
+                }
+                const double inertia = (mass[i] * at[i]) * 1.0;
+                R[i] = P[i] - fint[i] - 0.0 - damping - inertia;
+            }
+            __syncthreads();
+            if (tid == 0) {  // :2357-2371
+                const int full = (n / 8) * 8;
+                double sum = 0.0;
+                for (int cb = 0; cb < full; cb += 8) {
+                    double v[8];
+                    for (int l = 0; l < 8; ++l) v[l] = (R[cb + l] * R[cb + l]) * 1.0;
+                    sum += tree8_seq(v);
+                }
+                for (int j = full; j < n; ++j) sum += (R[j] * R[j]) * 1.0;
+                s_norm = sqrt(sum);
+            }
+            __syncthreads();
+            if (s_norm <= F.tol) {
+                converged = true;
+                break;
+            }
+            // K_eff = K_ff + a1 C + a0 M on the band (:2400-2411, lsb.mojo:1877-1981)
+            for (int sl = tid; sl < n * bw; sl += nt) {
+                const int i = sl / bw, j = i + (sl % bw) - hbw;
+                if (j < 0 || j >= n) {
+                    band[sl] = 0.0;
+                    continue;
+                }
+                const double kb = band[sl];
+                const double mij = i == j ? mass[i] : 0.0;
+                double v = kb;
+                if (F.has_damping) {
+                    double cc = 0.0;
+                    if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
+                    if (F.beta_k != 0.0) cc += F.beta_k * kb;
+                    if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
+                    v += F.a1 * cc;
+                }
+                v += F.a0 * mij;
+                band[sl] = v;
+            }
+            __syncthreads();
+            // BandGeneral LU without pivoting (lsb.mojo:631-679)
+            for (int k = 0; k < n; ++k) {
+                const int i1 = k + hbw > n - 1 ? n - 1 : k + hbw;
+                if (k > 0) {  // store the multipliers of column k - 1
+                    const int kp = k - 1;
+                    const int ip1 = kp + hbw > n - 1 ? n - 1 : kp + hbw;
+                    for (int i = kp + 1 + tid; i <= ip1; i += nt) {
+                        const int sl = i * bw + (kp - i + hbw);
+                        band[sl] = band[sl] / s_diag;
+                    }
+                }
+                if (tid == 0) {
+                    int prow = k;
+                    double pabs = 0.0;
+                    for (int row = k; row <= i1; ++row) {
+                        const double cand = fabs(band[row * bw + (k - row + hbw)]);
+                        if (cand > pabs) {
+                            pabs = cand;
+                            prow = row;
+                        }
+                    }
+                    s_stat = 0;
+                    if (pabs <= 1.0e-14) s_stat = kInstSingular;
+                    else if (prow != k) s_stat = kInstBandPivot;
+                    else {
+                        const double diag = band[k * bw + hbw];
+                        if (fabs(diag) <= 1.0e-14) s_stat = kInstSingular;
+                    }
+                }
+                __syncthreads();
+                if (s_stat != 0) {
+                    if (tid == 0) A.state[bid].status = s_stat;
+                    return;
+                }
+                const double diag = band[k * bw + hbw];
+                const int j1 = k + hbw > n - 1 ? n - 1 : k + hbw;
+                const int ni = i1 - k, nj = j1 - k;
+                for (int p = tid; p < ni * nj; p += nt) {
+                    const int i = k + 1 + p / nj, j = k + 1 + p % nj;
+                    const double factor = band[i * bw + (k - i + hbw)] / diag;
+                    if (factor == 0.0) continue;
+                    band[i * bw + (j - i + hbw)] -= factor * band[k * bw + (j - k + hbw)];
+                }
+                if (tid == 0) s_diag = diag;
+                __syncthreads();
+            }
+            {
+                const int kp = n - 1;
+                for (int i = kp + 1 + tid; i < n; i += nt) band[i * bw + (kp - i + hbw)] /= s_diag;
+            }
+            __syncthreads();
+            // forward (column order, same per-row subtraction order as lsb.mojo:746-757), then backward
+            if (tid < 64) {
+                const int lane = tid;
+                for (int i = lane; i < n; i += 64) work[i] = R[i];
+                wave_sync();
+                for (int j = 0; j < n; ++j) {
+                    const double yj = work[j];
+                    const int i = j + 1 + lane;
+                    if (lane < hbw && i < n) work[i] -= band[i * bw + (j - i + hbw)] * yj;
+                    wave_sync();
+                }
+                if (lane == 0) {
+                    for (int i = n - 1; i >= 0; --i) {
+                        double sum = work[i];
+                        const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+                        for (int j = i + 1; j <= j1; ++j) sum -= band[i * bw + (j - i + hbw)] * du[j];
+                        du[i] = sum / band[i * bw + hbw];
+                    }
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += nt) uf[i] += du[i];
+            __syncthreads();
+        }
+        if (tid == 0) A.iters[I.iters_base + step] = iters;
+        if (!converged) {
+            if (tid == 0) A.state[bid].status = step + 1;
+            break;
+        }
+        // commit (:3081-3101), committed basic sync (helpers.mojo:1832-1883), Newmark update (:594-647)
+        copy_instance_state(A, I, true);
+        for (int e = tid; e < F.n_elem; e += nt) {
+            const ElemDev &el = A.elems[F.elem_base + e];
+            double ug[6];
+            for (int k = 0; k < 6; ++k) ug[k] = el.fmap[k] >= 0 ? uf[el.fmap[k]] : 0.0;
+            const double cs = el.c2, sn = el.s2;
+            double ul[6];
+            ul[0] = cs * ug[0] + sn * ug[1];
+            ul[1] = -sn * ug[0] + cs * ug[1];
+            ul[2] = ug[2];
+            ul[3] = cs * ug[3] + sn * ug[4];
+            ul[4] = -sn * ug[3] + cs * ug[4];
+            ul[5] = ug[5];
+            const double chord = (ul[4] - ul[1]) / el.L2;
+            double *b = A.q + I.q_base + el.q_off + 3 + 2 * el.nip;
+            b[0] = ul[3] - ul[0];
+            b[1] = ul[2] - chord;
+            b[2] = ul[5] - chord;
+        }
+        for (int i = tid; i < n; i += nt) {
+            const double anext = F.a0 * (uf[i] - un[i]) - F.a2 * vn[i] - F.a3 * an[i];
+            af[i] = anext;
+            vf[i] = vn[i] + F.dt * (F.omg * an[i] + F.gamma * anext);
+        }
+        double *h = A.hist + I.hist_base + (int64_t)step * F.n_record;
+        for (int r = tid; r < F.n_record; r += nt) {
+            const int fi = A.rec_fidx[F.rec_base + r];
+            h[r] = fi >= 0 ? uf[fi] : 0.0;
+        }
+        st.total_iters += iters;
+        st.step = step + 1;
+        ++steps_run;
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += nt) {
+        vec[i] = uf[i];
+        vec[n + i] = vf[i];
+        vec[2 * n + i] = af[i];
+    }
+    if (tid == 0) {
+        InstState o = st;
+        o.status = A.state[bid].status;
+        A.state[bid] = o;
+    }
+}
+
+size_t frame2d_lds_bytes(int n_free, int bw) { return sizeof(double) * ((size_t)13 * n_free + (size_t)n_free * bw); }
+
+hipError_t launch_frame2d(hipStream_t stream, const Frame2dArgs &args, int n_inst, int steps_per_launch,
+                          size_t lds_bytes) {
+    static size_t configured = 0;
+    if (lds_bytes > configured) {
+        hipError_t e = hipFuncSetAttribute((const void *)frame2d_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds_bytes);
+        if (e != hipSuccess) return e;
+        configured = lds_bytes;
+    }
+    hipLaunchKernelGGL(frame2d_kernel, dim3(n_inst), dim3(256), lds_bytes, stream, args, steps_per_launch);
+    return hipGetLastError();
+}
+
+}  // namespace sgpu

@@ -1,0 +1,458 @@
+// Host side of the 2D fiber-frame transient: loads a frame (plain DOF
+// numbering, band half-width from the element free maps, gather lists),
+// adds (frame, ground motion) instances and launches frame2d.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "ctx.h"
+#include "frame2d_types.h"
+
+namespace sgpu {
+
+
+size_t frame2d_lds_bytes(int n_free, int bw);
+hipError_t launch_frame2d(hipStream_t stream, const Frame2dArgs &args, int n_inst, int steps_per_launch,
+                          size_t lds_bytes);
+
+struct FrameDef {
+    FrameDev dev{};
+    int n_nodes = 0, total_dofs = 0;
+    std::vector<ElemDev> elems;
+    std::vector<int32_t> elem_def;
+    std::vector<double> mass_ff, exc_ff;
+    std::vector<int32_t> exc_flag, rec_fidx, kg_off, kg, fg_off, fg;
+};
+
+struct InstHost {
+    int frame = 0;
+    int n_steps = 0;
+    std::vector<double> ag;
+    std::vector<int32_t> elem_inst;
+    std::vector<int32_t> range_def, range_first, range_count;  // section instances per definition
+};
+
+struct FrameHost {
+    std::vector<FrameDef> frames;
+    std::vector<InstHost> insts;
+    bool finalized = false;
+    Frame2dArgs args{};
+    size_t lds = 0;
+    int max_steps = 0;
+    InstState *d_state = nullptr;
+    double *d_hist = nullptr;
+    int32_t *d_iters = nullptr;
+    std::vector<int64_t> hist_base, iters_base;
+};
+
+void frame2d_host_free(FrameHost *fh) { delete fh; }
+
+}  // namespace sgpu
+
+using namespace sgpu;
+
+namespace {
+
+FrameHost &frames_of(sgpu_ctx *ctx) {
+    if (!ctx->frames) ctx->frames = new FrameHost();
+    return *ctx->frames;
+}
+
+// time_series.mojo:71-96 (Path with dt)
+double path_eval(const sgpu_path_series &ts, double t) {
+    const int n = ts.n;
+    if (n == 0) return 0.0;
+    if (t < ts.start_time) return 0.0;
+    const double incr = (t - ts.start_time) / ts.dt;
+    const long incr1 = (long)std::floor(incr);
+    const long incr2 = incr1 + 1;
+    if (incr2 >= n) {
+        if (ts.use_last) return ts.factor * ts.values[n - 1];
+        return 0.0;
+    }
+    const double v1 = ts.values[incr1], v2 = ts.values[incr2];
+    return ts.factor * (v1 + (v2 - v1) * (incr - (double)incr1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *frame_id) {
+    if (!ctx || !d || !frame_id) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (fh.finalized) return fail(ctx, SGPU_ERR_STATE, "frames must be created before sgpu_frame2d_finalize");
+    if (d->n_nodes <= 0 || d->n_elem <= 0 || !d->x || !d->y || !d->constrained || !d->mass || !d->elem_nodes ||
+        !d->elem_def || !d->elem_nip || !d->elem_xi || !d->elem_w)
+        return fail(ctx, SGPU_ERR_ARG, "incomplete frame description");
+    if (!(d->dt > 0.0) || !(d->beta > 0.0) || d->max_iters <= 0)
+        return fail(ctx, SGPU_ERR_ARG, "transient needs dt > 0, beta > 0, max_iters > 0");
+    FrameDef f;
+    f.n_nodes = d->n_nodes;
+    f.total_dofs = 3 * d->n_nodes;
+    std::vector<int> free_idx(f.total_dofs, -1);
+    int n = 0;
+    for (int i = 0; i < f.total_dofs; ++i)
+        if (!d->constrained[i]) free_idx[i] = n++;
+    if (n == 0) return fail(ctx, SGPU_ERR_ARG, "no free dofs");
+    int hbw = 0;
+    for (int e = 0; e < d->n_elem; ++e) {
+        ElemDev el{};
+        const int ni = d->elem_nodes[2 * e], nj = d->elem_nodes[2 * e + 1];
+        if (ni < 0 || nj < 0 || ni >= d->n_nodes || nj >= d->n_nodes)
+            return fail(ctx, SGPU_ERR_ARG, "element %d: node index out of range", e);
+        const int def = d->elem_def[e];
+        if (def < 0 || def >= (int)ctx->defs.size() || ctx->defs[def].dim != 2)
+            return fail(ctx, SGPU_ERR_ARG, "element %d: needs a 2D fiber-section definition", e);
+        const SecDefHost &sd = ctx->defs[def];
+        if (!sd.init_flex_valid)
+            return fail(ctx, SGPU_ERR_ARG, "element %d: section initial flexibility is singular", e);
+        bool any_nonlinear = false;
+        for (size_t s = 0; s < sd.mat.size(); ++s)
+            if (sd.mat[s] >= 0 && ctx->mats[sd.mat[s]].type != kElastic) any_nonlinear = true;
+        if (!any_nonlinear)
+            return fail(ctx, SGPU_ERR_ARG, "element %d: all-elastic sections are not on the device path", e);
+        const int nip = d->elem_nip[e];
+        if (nip < 1 || nip > kMaxNipDev)
+            return fail(ctx, SGPU_ERR_ARG, "element %d: 1..%d integration points on the device path", e, kMaxNipDev);
+        el.nip = nip;
+        el.def = def;
+        const double dx = d->x[nj] - d->x[ni], dy = d->y[nj] - d->y[ni];
+        el.L = std::hypot(dx, dy);
+        if (el.L == 0.0) return fail(ctx, SGPU_ERR_ARG, "element %d: zero length", e);
+        el.inv_L = 1.0 / el.L;
+        el.c = dx * el.inv_L;
+        el.s = dy * el.inv_L;
+        el.L2 = std::sqrt(dx * dx + dy * dy);
+        el.c2 = dx / el.L2;
+        el.s2 = dy / el.L2;
+        for (int ip = 0; ip < nip; ++ip) {
+            el.xi[ip] = d->elem_xi[10 * e + ip];
+            el.w[ip] = d->elem_w[10 * e + ip];
+        }
+        el.if00 = sd.init_flex[0];
+        el.if01 = sd.init_flex[1];
+        el.if11 = sd.init_flex[2];
+        el.if_valid = 1;
+        for (int k = 0; k < 3; ++k) {
+            el.fmap[k] = free_idx[3 * ni + k];
+            el.fmap[3 + k] = free_idx[3 * nj + k];
+        }
+        for (int a = 0; a < 6; ++a)  // lsb.mojo:1481-1506
+            for (int b = a + 1; b < 6; ++b)
+                if (el.fmap[a] >= 0 && el.fmap[b] >= 0) hbw = std::max(hbw, std::abs(el.fmap[a] - el.fmap[b]));
+        el.q_off = 0;
+        f.elems.push_back(el);
+        f.elem_def.push_back(def);
+    }
+    int q_off = 0;
+    for (ElemDev &el : f.elems) {
+        el.q_off = q_off;
+        q_off += 3 + 2 * el.nip + 3;
+    }
+    const int bw = 2 * hbw + 1;
+    // gather lists in element order
+    std::vector<std::vector<int32_t>> kl((size_t)n * bw), fl(n);
+    for (int e = 0; e < d->n_elem; ++e) {
+        const ElemDev &el = f.elems[e];
+        for (int a = 0; a < 6; ++a) {
+            if (el.fmap[a] < 0) continue;
+            for (int b = 0; b < 6; ++b) {
+                if (el.fmap[b] < 0) continue;
+                const int i = el.fmap[a], j = el.fmap[b];
+                kl[(size_t)i * bw + (j - i + hbw)].push_back(42 * e + 6 * a + b);
+            }
+            fl[el.fmap[a]].push_back(42 * e + 36 + a);
+        }
+    }
+    f.kg_off.push_back(0);
+    for (auto &v : kl) {
+        f.kg.insert(f.kg.end(), v.begin(), v.end());
+        f.kg_off.push_back((int32_t)f.kg.size());
+    }
+    f.fg_off.push_back(0);
+    for (auto &v : fl) {
+        f.fg.insert(f.fg.end(), v.begin(), v.end());
+        f.fg_off.push_back((int32_t)f.fg.size());
+    }
+    for (int i = 0; i < f.total_dofs; ++i) {
+        if (free_idx[i] < 0) continue;
+        f.mass_ff.push_back(d->mass[i]);
+        const bool exc = d->exc_dir > 0 && (i % 3) + 1 == d->exc_dir;
+        f.exc_ff.push_back(exc ? d->mass[i] : 0.0);
+        f.exc_flag.push_back(exc ? 1 : 0);
+    }
+    bool has_mass = false;
+    for (double m : f.mass_ff) has_mass = has_mass || m != 0.0;
+    if (!has_mass) return fail(ctx, SGPU_ERR_ARG, "transient_nonlinear requires masses on free dofs");
+    for (int r = 0; r < d->n_record; ++r) {
+        const int dof = d->record_dofs[r];
+        if (dof < 0 || dof >= f.total_dofs) return fail(ctx, SGPU_ERR_ARG, "record dof %d out of range", dof);
+        f.rec_fidx.push_back(free_idx[dof]);
+    }
+    FrameDev &fd = f.dev;
+    fd.n_elem = d->n_elem;
+    fd.n_free = n;
+    fd.hbw = hbw;
+    fd.bw = bw;
+    fd.n_record = d->n_record;
+    fd.max_iters = d->max_iters;
+    fd.exc_dir = d->exc_dir;
+    fd.has_damping = (d->alpha_m != 0.0 || d->beta_k != 0.0 || d->beta_k_init != 0.0) ? 1 : 0;
+    fd.q_len = q_off;
+    fd.dt = d->dt;
+    fd.gamma = d->gamma;
+    fd.beta = d->beta;
+    fd.tol = d->tol;
+    // transient_nonlinear.mojo:1052-1055, :559, :510
+    fd.a0 = 1.0 / (d->beta * d->dt * d->dt);
+    fd.a1 = d->gamma / (d->beta * d->dt);
+    fd.a2 = 1.0 / (d->beta * d->dt);
+    fd.a3 = 1.0 / (2.0 * d->beta) - 1.0;
+    fd.coeff = d->dt * d->dt * (0.5 - d->beta);
+    fd.omg = 1.0 - d->gamma;
+    fd.alpha_m = d->alpha_m;
+    fd.beta_k = d->beta_k;
+    fd.beta_k_init = d->beta_k_init;
+    const size_t lds = frame2d_lds_bytes(n, bw);
+    if (lds > 150 * 1024)
+        return fail(ctx, SGPU_ERR_ARG, "frame band of %d x %d does not fit the LDS of one workgroup", n, bw);
+    *frame_id = (int32_t)fh.frames.size();
+    fh.frames.push_back(std::move(f));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_add_instance(sgpu_ctx *ctx, int32_t frame_id, int32_t n_steps, const sgpu_path_series *accel,
+                              int32_t *instance_id) {
+    if (!ctx || !accel || !instance_id) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (fh.finalized || ctx->finalized)
+        return fail(ctx, SGPU_ERR_STATE, "instances must be added before sgpu_frame2d_finalize");
+    if (frame_id < 0 || frame_id >= (int)fh.frames.size() || n_steps <= 0)
+        return fail(ctx, SGPU_ERR_ARG, "bad frame id or step count");
+    if (accel->n < 0 || (accel->n > 0 && (!accel->values || !(accel->dt > 0.0))))
+        return fail(ctx, SGPU_ERR_ARG, "bad acceleration series");
+    const FrameDef &f = fh.frames[frame_id];
+    InstHost in;
+    in.frame = frame_id;
+    in.n_steps = n_steps;
+    in.ag.resize(n_steps);
+    for (int s = 0; s < n_steps; ++s) in.ag[s] = path_eval(*accel, (double)(s + 1) * f.dev.dt);
+    for (int e = 0; e < f.dev.n_elem; ++e) {
+        int32_t first = 0;
+        int rc = sgpu_section_instances_alloc(ctx, f.elem_def[e], f.elems[e].nip, &first);
+        if (rc) return rc;
+        in.elem_inst.push_back(first);
+        bool merged = false;
+        for (size_t r = 0; r < in.range_def.size(); ++r)
+            if (in.range_def[r] == f.elem_def[e] && in.range_first[r] + in.range_count[r] == first) {
+                in.range_count[r] += f.elems[e].nip;
+                merged = true;
+            }
+        if (!merged) {
+            if (in.range_def.size() >= 4) return fail(ctx, SGPU_ERR_ARG, "at most 4 section definitions per frame");
+            in.range_def.push_back(f.elem_def[e]);
+            in.range_first.push_back(first);
+            in.range_count.push_back(f.elems[e].nip);
+        }
+    }
+    *instance_id = (int32_t)fh.insts.size();
+    fh.insts.push_back(std::move(in));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (fh.finalized) return fail(ctx, SGPU_ERR_STATE, "frames already finalized");
+    if (fh.insts.empty()) return fail(ctx, SGPU_ERR_STATE, "no frame instances");
+    int rc;
+    if (!ctx->finalized && (rc = sgpu_sections_finalize(ctx))) return rc;
+    std::vector<FrameDev> fdev;
+    std::vector<ElemDev> elems;
+    std::vector<double> mass_ff, exc_ff, ag;
+    std::vector<int32_t> exc_flag, rec_fidx, kg_off, kg, fg_off, fg, elem_inst;
+    size_t lds = 0;
+    for (FrameDef &f : fh.frames) {
+        FrameDev d = f.dev;
+        d.elem_base = (int64_t)elems.size();
+        d.free_base = (int64_t)mass_ff.size();
+        d.rec_base = (int64_t)rec_fidx.size();
+        // offsets of the gather lists are stored relative to the frame's list start
+        d.kg_off_base = (int64_t)kg_off.size();
+        d.fg_off_base = (int64_t)fg_off.size();
+        const int32_t kg0 = (int32_t)kg.size(), fg0 = (int32_t)fg.size();
+        for (int32_t o : f.kg_off) kg_off.push_back(o + kg0);
+        for (int32_t o : f.fg_off) fg_off.push_back(o + fg0);
+        kg.insert(kg.end(), f.kg.begin(), f.kg.end());
+        fg.insert(fg.end(), f.fg.begin(), f.fg.end());
+        elems.insert(elems.end(), f.elems.begin(), f.elems.end());
+        mass_ff.insert(mass_ff.end(), f.mass_ff.begin(), f.mass_ff.end());
+        exc_ff.insert(exc_ff.end(), f.exc_ff.begin(), f.exc_ff.end());
+        exc_flag.insert(exc_flag.end(), f.exc_flag.begin(), f.exc_flag.end());
+        rec_fidx.insert(rec_fidx.end(), f.rec_fidx.begin(), f.rec_fidx.end());
+        fdev.push_back(d);
+        lds = std::max(lds, frame2d_lds_bytes(d.n_free, d.bw));
+    }
+    std::vector<InstDev> idev;
+    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0;
+    fh.hist_base.clear();
+    fh.iters_base.clear();
+    fh.max_steps = 0;
+    for (InstHost &in : fh.insts) {
+        const FrameDev &d = fdev[in.frame];
+        InstDev I{};
+        I.frame = in.frame;
+        I.n_steps = in.n_steps;
+        I.ag_base = (int64_t)ag.size();
+        ag.insert(ag.end(), in.ag.begin(), in.ag.end());
+        I.elem_inst_base = (int64_t)elem_inst.size();
+        elem_inst.insert(elem_inst.end(), in.elem_inst.begin(), in.elem_inst.end());
+        I.q_base = q;
+        q += d.q_len;
+        I.vec_base = vec;
+        vec += 3 * d.n_free;
+        I.kinit_base = kinit;
+        kinit += (int64_t)d.n_free * d.bw;
+        I.scratch_base = scratch;
+        scratch += 42 * d.n_elem;
+        I.hist_base = hist;
+        hist += (int64_t)in.n_steps * d.n_record;
+        I.iters_base = iters;
+        iters += in.n_steps;
+        fh.hist_base.push_back(I.hist_base);
+        fh.iters_base.push_back(I.iters_base);
+        I.n_ranges = (int32_t)in.range_def.size();
+        for (int r = 0; r < I.n_ranges; ++r) {
+            const SecDefHost &sd = ctx->defs[in.range_def[r]];
+            const int64_t ns = (int64_t)sd.y.size();
+            I.slot_begin[r] = sd.state_base + (int64_t)in.range_first[r] * ns;
+            I.slot_count[r] = (int64_t)in.range_count[r] * ns;
+            I.aux_begin[r] = sd.aux_base + (int64_t)in.range_first[r] * sd.n_aux;
+            I.aux_count[r] = (int64_t)in.range_count[r] * sd.n_aux;
+        }
+        fh.max_steps = std::max(fh.max_steps, in.n_steps);
+        idev.push_back(I);
+    }
+    Frame2dArgs &A = fh.args;
+    FrameDev *d_frames;
+    InstDev *d_insts;
+    ElemDev *d_elems;
+    double *d_mass, *d_exc, *d_ag;
+    int32_t *d_excf, *d_rec, *d_kgo, *d_kg, *d_fgo, *d_fg, *d_ei;
+    if ((rc = upload(ctx, &d_frames, fdev))) return rc;
+    if ((rc = upload(ctx, &d_insts, idev))) return rc;
+    if ((rc = upload(ctx, &d_elems, elems))) return rc;
+    if ((rc = upload(ctx, &d_mass, mass_ff))) return rc;
+    if ((rc = upload(ctx, &d_exc, exc_ff))) return rc;
+    if ((rc = upload(ctx, &d_excf, exc_flag))) return rc;
+    if ((rc = upload(ctx, &d_rec, rec_fidx))) return rc;
+    if ((rc = upload(ctx, &d_kgo, kg_off))) return rc;
+    if ((rc = upload(ctx, &d_kg, kg))) return rc;
+    if ((rc = upload(ctx, &d_fgo, fg_off))) return rc;
+    if ((rc = upload(ctx, &d_fg, fg))) return rc;
+    if ((rc = upload(ctx, &d_ei, elem_inst))) return rc;
+    if ((rc = upload(ctx, &d_ag, ag))) return rc;
+    std::vector<InstState> st(fh.insts.size(), InstState{0, 0, 0, 0});
+    if ((rc = upload(ctx, &fh.d_state, st))) return rc;
+    std::vector<double> zq(q, 0.0), zv(vec, 0.0);
+    double *d_q, *d_vec, *d_kinit, *d_scratch;
+    if ((rc = upload(ctx, &d_q, zq))) return rc;
+    if ((rc = upload(ctx, &d_vec, zv))) return rc;
+    if ((rc = dalloc(ctx, &d_kinit, (size_t)kinit))) return rc;
+    if ((rc = dalloc(ctx, &d_scratch, (size_t)scratch))) return rc;
+    if ((rc = dalloc(ctx, &fh.d_hist, (size_t)hist))) return rc;
+    if ((rc = dalloc(ctx, &fh.d_iters, (size_t)iters))) return rc;
+    HIP_OK(ctx, hipMemset(fh.d_hist, 0, sizeof(double) * (size_t)(hist > 0 ? hist : 1)));
+    HIP_OK(ctx, hipMemset(fh.d_iters, 0, sizeof(int32_t) * (size_t)(iters > 0 ? iters : 1)));
+    A.frames = d_frames;
+    A.insts = d_insts;
+    A.state = fh.d_state;
+    A.elems = d_elems;
+    A.mass_ff = d_mass;
+    A.exc_ff = d_exc;
+    A.exc_flag = d_excf;
+    A.rec_fidx = d_rec;
+    A.kg_off = d_kgo;
+    A.kg = d_kg;
+    A.fg_off = d_fgo;
+    A.fg = d_fg;
+    A.elem_inst = d_ei;
+    A.ag = d_ag;
+    A.q = d_q;
+    A.vec = d_vec;
+    A.kinit = d_kinit;
+    A.scratch = d_scratch;
+    A.hist = fh.d_hist;
+    A.iters = fh.d_iters;
+    A.defs = ctx->d_defs;
+    A.tb = ctx->tables();
+    A.mats = ctx->d_mats;
+    A.c = ctx->c;
+    A.t = ctx->t;
+    fh.lds = lds;
+    fh.finalized = true;
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t steps_per_launch) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (steps_per_launch <= 0) return fail(ctx, SGPU_ERR_ARG, "steps_per_launch must be > 0");
+    HIP_OK(ctx, launch_frame2d(ctx->stream, fh.args, (int)fh.insts.size(), steps_per_launch, fh.lds));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t steps_per_launch) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (steps_per_launch <= 0) return fail(ctx, SGPU_ERR_ARG, "steps_per_launch must be > 0");
+    const int launches = (fh.max_steps + steps_per_launch - 1) / steps_per_launch;
+    for (int k = 0; k < launches; ++k) {
+        int rc = sgpu_frame2d_launch(ctx, steps_per_launch);
+        if (rc) return rc;
+    }
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_result(sgpu_ctx *ctx, int32_t instance, double *u_hist, int32_t *iters, int32_t *status,
+                        int32_t *steps_done) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (instance < 0 || instance >= (int)fh.insts.size()) return fail(ctx, SGPU_ERR_ARG, "instance out of range");
+    const InstHost &in = fh.insts[instance];
+    const FrameDef &f = fh.frames[in.frame];
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    InstState st;
+    HIP_OK(ctx, hipMemcpy(&st, fh.d_state + instance, sizeof(st), hipMemcpyDeviceToHost));
+    if (status) *status = st.status;
+    if (steps_done) *steps_done = st.step;
+    if (u_hist && f.dev.n_record > 0)
+        HIP_OK(ctx, hipMemcpy(u_hist, fh.d_hist + fh.hist_base[instance],
+                              sizeof(double) * (size_t)in.n_steps * f.dev.n_record, hipMemcpyDeviceToHost));
+    if (iters)
+        HIP_OK(ctx, hipMemcpy(iters, fh.d_iters + fh.iters_base[instance], sizeof(int32_t) * (size_t)in.n_steps,
+                              hipMemcpyDeviceToHost));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_info(sgpu_ctx *ctx, int32_t frame_id, int32_t *out4) {
+    if (!ctx || !out4) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (frame_id < 0 || frame_id >= (int)fh.frames.size()) return fail(ctx, SGPU_ERR_ARG, "frame out of range");
+    const FrameDef &f = fh.frames[frame_id];
+    out4[0] = f.dev.n_free;
+    out4[1] = f.dev.hbw;
+    out4[2] = f.dev.n_record;
+    out4[3] = f.total_dofs;
+    return SGPU_OK;
+}
+
+}  // extern "C"

@@ -28,101 +28,11 @@ void launch_copy_instances(hipStream_t stream, const SecDefDev *defs, const int3
 
 using namespace sgpu;
 
-namespace {
+#include "ctx.h"
 
-constexpr int kW = 8;       // SIMD width of the reference's family loops
-constexpr int kBatchSlots = 64;
-
-struct SecDefHost {
-    int dim = 2;
-    int n_fibers = 0;
-    double y_bar = 0.0, z_bar = 0.0;
-    // slot layout
-    std::vector<double> y, z, area;
-    std::vector<int32_t> mat, aux, chunk;
-    std::vector<int32_t> batch_chunk_begin;  // n_batches + 1
-    std::vector<int32_t> fiber_slot;         // input fiber -> slot
-    std::vector<uint8_t> slot_real;          // 1 for real material fibers
-    int n_aux = 0;
-    int n_elastic_chunks = 0;
-    int zero_k12 = 0;
-    int init_flex_valid = 0;
-    double init_flex[3] = {0, 0, 0};
-    int n_inst = 0;
-    int64_t slot_base = 0, batch_base = 0, state_base = 0, aux_base = 0;
-};
-
-}  // namespace
-
-struct sgpu_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::vector<MatDev> mats;
-    std::vector<SecDefHost> defs;
-    bool finalized = false;
-    int64_t total_slots = 0, total_aux = 0, total_fibers = 0, total_table_slots = 0;
-    // device buffers
-    MatDev *d_mats = nullptr;
-    SecDefDev *d_defs = nullptr;
-    double *d_tab_y = nullptr, *d_tab_z = nullptr, *d_tab_area = nullptr;
-    int32_t *d_tab_mat = nullptr, *d_tab_aux = nullptr, *d_tab_chunk = nullptr, *d_tab_bcb = nullptr;
-    StateArrays c{}, t{};
-    std::vector<void *> allocations;
-    // scratch for the host-buffer entry points
-    sgpu_section_point *d_pts = nullptr;
-    sgpu_section_result *d_res = nullptr;
-    int scratch_cap = 0;
-    int32_t *d_list = nullptr;
-    int list_cap = 0;
-    SlotTables tables() const {
-        SlotTables tb;
-        tb.y = d_tab_y;
-        tb.z = d_tab_z;
-        tb.area = d_tab_area;
-        tb.mat = d_tab_mat;
-        tb.aux = d_tab_aux;
-        tb.chunk = d_tab_chunk;
-        tb.batch_chunk_begin = d_tab_bcb;
-        return tb;
-    }
-};
+using namespace sgpu;
 
 namespace {
-
-int fail(sgpu_ctx *ctx, int code, const char *fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    if (ctx) ctx->err = buf;
-    return code;
-}
-
-#define HIP_OK(ctx, expr)                                                                         \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess) return fail(ctx, SGPU_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-template <typename T>
-int dalloc(sgpu_ctx *ctx, T **p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess) return fail(ctx, SGPU_ERR_HIP, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-    ctx->allocations.push_back((void *)*p);
-    return SGPU_OK;
-}
-
-template <typename T>
-int upload(sgpu_ctx *ctx, T **p, const std::vector<T> &v) {
-    int rc = dalloc(ctx, p, v.size());
-    if (rc) return rc;
-    if (!v.empty()) HIP_OK(ctx, hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return SGPU_OK;
-}
 
 double initial_tangent(const MatDev &m) {  // core.mojo:270-282
     switch (m.type) {
@@ -299,6 +209,7 @@ int build_def(sgpu_ctx *ctx, SecDefHost &d, const std::vector<Cell> &cells) {
         const int local = ci - d.batch_chunk_begin[b];
         if (local < 0 || local >= kBatchSlots) return fail(ctx, SGPU_ERR_STATE, "internal: chunk index");
         d.chunk[s] = local * 2 + (w == kW ? 1 : 0);
+        d.chunk_slot.push_back(s);
     }
     return SGPU_OK;
 }
@@ -395,6 +306,7 @@ int sgpu_ctx_destroy(sgpu_ctx *ctx) {
     if (ctx->d_res) (void)hipFree(ctx->d_res);
     if (ctx->d_list) (void)hipFree(ctx->d_list);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    sgpu::frame2d_host_free(ctx->frames);
     delete ctx;
     return SGPU_OK;
 }
@@ -482,11 +394,13 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
     HIP_OK(ctx, hipSetDevice(ctx->device));
     std::vector<SecDefDev> ddefs;
     std::vector<double> ty, tz, ta;
-    std::vector<int32_t> tm, tx, tc, tb;
+    std::vector<int32_t> tm, tx, tc, tb, tcs;
     int64_t state = 0, aux = 0, fibers = 0;
     for (SecDefHost &d : ctx->defs) {
         d.slot_base = (int64_t)ty.size();
         d.batch_base = (int64_t)tb.size();
+        d.chunk_base = (int64_t)tcs.size();
+        tcs.insert(tcs.end(), d.chunk_slot.begin(), d.chunk_slot.end());
         d.state_base = state;
         d.aux_base = aux;
         ty.insert(ty.end(), d.y.begin(), d.y.end());
@@ -510,6 +424,7 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
         dd.n_inst = d.n_inst;
         dd.slot_base = d.slot_base;
         dd.batch_base = d.batch_base;
+        dd.chunk_base = d.chunk_base;
         dd.state_base = d.state_base;
         dd.aux_base = d.aux_base;
         for (int k = 0; k < 3; ++k) dd.init_flex[k] = d.init_flex[k];
@@ -530,6 +445,7 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &ctx->d_tab_aux, tx))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_chunk, tc))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_bcb, tb))) return rc;
+    if ((rc = upload(ctx, &ctx->d_tab_cslot, tcs))) return rc;
     // initial states, built per definition and replicated over its instances
     std::vector<double> e(state), s(state), tg(state), h0(state), h1(state), a0(aux, 0.0);
     std::vector<int32_t> kon(aux, 0);

@@ -45,6 +45,7 @@ struct SecDefDev {
     int32_t n_inst;
     int64_t slot_base;         // into the per-slot definition tables
     int64_t batch_base;        // into batch_chunk_begin (n_batches + 1 entries)
+    int64_t chunk_base;        // into chunk_slot (one entry per chunk)
     int64_t state_base;        // first slot-state index of instance 0
     int64_t aux_base;          // first aux-state index of instance 0
     double init_flex[6];       // 2D: {f00, f01, f11}; valid flag in init_flex_valid
@@ -63,6 +64,7 @@ struct SlotTables {
     const int32_t *aux;     // aux index within the instance, or -1
     const int32_t *chunk;   // chunk code (see above)
     const int32_t *batch_chunk_begin;
+    const int32_t *chunk_slot;  // first slot of each chunk
 };
 
 // Fiber state, structure of arrays over all instances of all definitions.
