@@ -47,6 +47,13 @@ EXPORTED = (
     "sgpu_sections_revert",
     "sgpu_fiber_state_read",
     "sgpu_fiber_totals",
+    "sgpu_frame2d_create",
+    "sgpu_frame2d_add_instance",
+    "sgpu_frame2d_finalize",
+    "sgpu_frame2d_launch",
+    "sgpu_frame2d_run",
+    "sgpu_frame2d_result",
+    "sgpu_frame2d_info",
 )
 
 

@@ -90,7 +90,7 @@ struct FbcScratch {
 };
 
 // force_beam_column2d.mojo:804-1189 (try_increment); best = {q0,q1,q2,k00,k01,k02,k10,k11,k12,k22}
-__device__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, double bv0, double bv1, double bv2,
+__device__ __noinline__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, double bv0, double bv1, double bv2,
                               double dv0, double dv1, double dv2, int scheme, double *best) {
     const ElemDev &el = *E.el;
     const int nip = el.nip;
@@ -296,7 +296,7 @@ __device__ __forceinline__ double kglobal(const double *kb, double inv_L, double
 // force_beam_column2d.mojo:1418-1919 for one element, executed by one whole
 // wavefront; lanes 0..35 store k_global, lanes 36..41 f_global.  Returns
 // false when the reference would abort (no recoverable tangent).
-__device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q,
+__device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q,
                               const double *uf, double *out42, const FrameDev &F) {
     ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e]};
     const int nip = el.nip;
@@ -327,6 +327,7 @@ __device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const Elem
     bool converged = false, have_best = false;
     double best[10];
     int num_sub = 1;
+    int guard = 0;  // bounds the subdivision loop (the reference's has no bound)
     if (max_abs3(rem0, rem1, rem2) <= 1.0e-16) {
         for (int scheme = 0; scheme < 3; ++scheme) {
             restore_predictor(q, nip, aq0, aq1, aq2, sc);
@@ -340,7 +341,7 @@ __device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const Elem
         }
     } else {
         while (true) {
-            if (num_sub > 4) break;
+            if (num_sub > 4 || ++guard > 4096) break;
             const double tv0 = ab0 + at0, tv1 = ab1 + at1, tv2 = ab2 + at2;
             bool scheme_success = false;
             for (int scheme = 0; scheme < 3; ++scheme) {
@@ -668,7 +669,13 @@ __global__ __launch_bounds__(256) void frame2d_kernel(Frame2dArgs A, int steps_p
                 band[sl] = v;
             }
             __syncthreads();
-            // BandGeneral LU without pivoting (lsb.mojo:631-679)
+            // BandGeneral LU without pivoting (lsb.mojo:631-679); the band as
+            // loaded is kept for the dense fallback of :652-657.
+            double *dense = A.dense + I.dense_base;
+            double *orig = dense + (int64_t)n * n;
+            for (int sl = tid; sl < n * bw; sl += nt) orig[sl] = band[sl];
+            int32_t *piv = A.piv + I.piv_base;
+            bool dense_mode = false;
             for (int k = 0; k < n; ++k) {
                 const int i1 = k + hbw > n - 1 ? n - 1 : k + hbw;
                 if (k > 0) {  // store the multipliers of column k - 1
@@ -698,6 +705,10 @@ __global__ __launch_bounds__(256) void frame2d_kernel(Frame2dArgs A, int steps_p
                     }
                 }
                 __syncthreads();
+                if (s_stat == kInstBandPivot) {
+                    dense_mode = true;
+                    break;
+                }
                 if (s_stat != 0) {
                     if (tid == 0) A.state[bid].status = s_stat;
                     return;
@@ -714,32 +725,119 @@ __global__ __launch_bounds__(256) void frame2d_kernel(Frame2dArgs A, int steps_p
                 if (tid == 0) s_diag = diag;
                 __syncthreads();
             }
-            {
-                const int kp = n - 1;
-                for (int i = kp + 1 + tid; i < n; i += nt) band[i * bw + (kp - i + hbw)] /= s_diag;
-            }
-            __syncthreads();
-            // forward (column order, same per-row subtraction order as lsb.mojo:746-757), then backward
-            if (tid < 64) {
-                const int lane = tid;
-                for (int i = lane; i < n; i += 64) work[i] = R[i];
-                wave_sync();
-                for (int j = 0; j < n; ++j) {
-                    const double yj = work[j];
-                    const int i = j + 1 + lane;
-                    if (lane < hbw && i < n) work[i] -= band[i * bw + (j - i + hbw)] * yj;
-                    wave_sync();
+            if (dense_mode) {
+                // lsb.mojo:608-628 then :175-214: dense LU with partial pivoting
+                for (int p = tid; p < n * n; p += nt) {
+                    const int row = p / n, col = p % n;
+                    const int off = col - row + hbw;
+                    dense[p] = (off >= 0 && off < bw) ? orig[row * bw + off] : 0.0;
                 }
-                if (lane == 0) {
-                    for (int i = n - 1; i >= 0; --i) {
-                        double sum = work[i];
-                        const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
-                        for (int j = i + 1; j <= j1; ++j) sum -= band[i * bw + (j - i + hbw)] * du[j];
-                        du[i] = sum / band[i * bw + hbw];
+                for (int i = tid; i < n; i += nt) piv[i] = i;
+                __syncthreads();
+                for (int col = 0; col < n; ++col) {
+                    if (col > 0) {  // store the multipliers of column col - 1
+                        const int cp = col - 1;
+                        for (int row = cp + 1 + tid; row < n; row += nt)
+                            dense[row * n + cp] = dense[row * n + cp] / s_diag;
+                    }
+                    __syncthreads();
+                    if (tid == 0) {
+                        int prow = col;
+                        double pabs = fabs(dense[col * n + col]);
+                        for (int row = col + 1; row < n; ++row) {
+                            const double cand = fabs(dense[row * n + col]);
+                            if (cand > pabs) {
+                                pabs = cand;
+                                prow = row;
+                            }
+                        }
+                        s_stat = pabs <= 1.0e-14 ? kInstSingular : 0;
+                        s_norm = (double)prow;  // reuse as the broadcast pivot row
+                    }
+                    __syncthreads();
+                    if (s_stat != 0) {
+                        if (tid == 0) A.state[bid].status = s_stat;
+                        return;
+                    }
+                    const int prow = (int)s_norm;
+                    if (prow != col) {
+                        for (int j = tid; j < n; j += nt) {
+                            const double tmp = dense[col * n + j];
+                            dense[col * n + j] = dense[prow * n + j];
+                            dense[prow * n + j] = tmp;
+                        }
+                        if (tid == 0) {
+                            const int tp = piv[col];
+                            piv[col] = piv[prow];
+                            piv[prow] = tp;
+                        }
+                    }
+                    __syncthreads();
+                    const double diag = dense[col * n + col];
+                    if (fabs(diag) <= 1.0e-14) {
+                        if (tid == 0) A.state[bid].status = kInstSingular;
+                        return;
+                    }
+                    const int m = n - col - 1;
+                    for (int p = tid; p < m * m; p += nt) {
+                        const int row = col + 1 + p / m, j = col + 1 + p % m;
+                        const double factor = dense[row * n + col] / diag;
+                        if (factor == 0.0) continue;
+                        dense[row * n + j] -= factor * dense[col * n + j];
+                    }
+                    if (tid == 0) s_diag = diag;
+                    __syncthreads();
+                }
+                // lsb.mojo:216-232
+                if (tid < 64) {
+                    const int lane = tid;
+                    for (int i = lane; i < n; i += 64) work[i] = R[piv[i]];
+                    wave_sync();
+                    for (int j = 0; j < n; ++j) {
+                        const double yj = work[j];
+                        for (int i = j + 1 + lane; i < n; i += 64) work[i] -= dense[i * n + j] * yj;
+                        wave_sync();
+                    }
+                    if (lane == 0) {
+                        for (int i = n - 1; i >= 0; --i) {
+                            double sum = work[i];
+                            for (int j = i + 1; j < n; ++j) sum -= dense[i * n + j] * du[j];
+                            du[i] = sum / dense[i * n + i];
+                        }
+                        st.dense_fallbacks += 1;
                     }
                 }
+                __syncthreads();
+            } else {
+                {
+                    const int kp = n - 1;
+                    for (int i = kp + 1 + tid; i < n; i += nt) band[i * bw + (kp - i + hbw)] /= s_diag;
+                }
+                __syncthreads();
+                // forward (column order, same per-row subtraction order as lsb.mojo:746-757), then backward
+                if (tid < 64) {
+                    const int lane = tid;
+                    for (int i = lane; i < n; i += 64) work[i] = R[i];
+                    wave_sync();
+                    for (int j = 0; j < n; ++j) {
+                        const double yj = work[j];
+                        for (int t = lane; t < hbw; t += 64) {
+                            const int i = j + 1 + t;
+                            if (i < n) work[i] -= band[i * bw + (j - i + hbw)] * yj;
+                        }
+                        wave_sync();
+                    }
+                    if (lane == 0) {
+                        for (int i = n - 1; i >= 0; --i) {
+                            double sum = work[i];
+                            const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+                            for (int j = i + 1; j <= j1; ++j) sum -= band[i * bw + (j - i + hbw)] * du[j];
+                            du[i] = sum / band[i * bw + hbw];
+                        }
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
             for (int i = tid; i < n; i += nt) uf[i] += du[i];
             __syncthreads();
         }

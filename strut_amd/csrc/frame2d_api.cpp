@@ -298,7 +298,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         lds = std::max(lds, frame2d_lds_bytes(d.n_free, d.bw));
     }
     std::vector<InstDev> idev;
-    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0;
+    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0, dense = 0, piv = 0;
     fh.hist_base.clear();
     fh.iters_base.clear();
     fh.max_steps = 0;
@@ -319,6 +319,10 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         kinit += (int64_t)d.n_free * d.bw;
         I.scratch_base = scratch;
         scratch += 42 * d.n_elem;
+        I.dense_base = dense;
+        dense += (int64_t)d.n_free * d.n_free + (int64_t)d.n_free * d.bw;
+        I.piv_base = piv;
+        piv += d.n_free;
         I.hist_base = hist;
         hist += (int64_t)in.n_steps * d.n_record;
         I.iters_base = iters;
@@ -356,14 +360,17 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &d_fg, fg))) return rc;
     if ((rc = upload(ctx, &d_ei, elem_inst))) return rc;
     if ((rc = upload(ctx, &d_ag, ag))) return rc;
-    std::vector<InstState> st(fh.insts.size(), InstState{0, 0, 0, 0});
+    std::vector<InstState> st(fh.insts.size(), InstState{0, 0, 0, 0, 0, 0});
     if ((rc = upload(ctx, &fh.d_state, st))) return rc;
     std::vector<double> zq(q, 0.0), zv(vec, 0.0);
-    double *d_q, *d_vec, *d_kinit, *d_scratch;
+    double *d_q, *d_vec, *d_kinit, *d_scratch, *d_dense;
+    int32_t *d_piv;
     if ((rc = upload(ctx, &d_q, zq))) return rc;
     if ((rc = upload(ctx, &d_vec, zv))) return rc;
     if ((rc = dalloc(ctx, &d_kinit, (size_t)kinit))) return rc;
     if ((rc = dalloc(ctx, &d_scratch, (size_t)scratch))) return rc;
+    if ((rc = dalloc(ctx, &d_dense, (size_t)dense))) return rc;
+    if ((rc = dalloc(ctx, &d_piv, (size_t)piv))) return rc;
     if ((rc = dalloc(ctx, &fh.d_hist, (size_t)hist))) return rc;
     if ((rc = dalloc(ctx, &fh.d_iters, (size_t)iters))) return rc;
     HIP_OK(ctx, hipMemset(fh.d_hist, 0, sizeof(double) * (size_t)(hist > 0 ? hist : 1)));
@@ -386,6 +393,8 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     A.vec = d_vec;
     A.kinit = d_kinit;
     A.scratch = d_scratch;
+    A.dense = d_dense;
+    A.piv = d_piv;
     A.hist = fh.d_hist;
     A.iters = fh.d_iters;
     A.defs = ctx->d_defs;

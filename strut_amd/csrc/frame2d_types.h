@@ -44,6 +44,8 @@ struct InstDev {
     int64_t ag_base;        // ground acceleration at t = (step + 1) dt
     int64_t elem_inst_base; // section instance of integration point 0, per element
     int64_t q_base, vec_base, kinit_base, scratch_base, hist_base, iters_base;
+    int64_t dense_base;     // n*n dense fallback matrix followed by the n*bw band copy
+    int64_t piv_base;       // n row permutation of the dense fallback
     int64_t slot_begin[4], slot_count[4], aux_begin[4], aux_count[4];  // fiber state ranges
 };
 
@@ -52,6 +54,8 @@ struct InstState {
     int32_t status;     // 0 running / done; > 0: step that did not converge; < 0: failure code
     int32_t setup_done;
     int32_t total_iters;
+    int32_t dense_fallbacks;  // BandGeneral factorizations that switched to the dense LU
+    int32_t pad_;
 };
 
 enum InstStatus : int32_t {
@@ -75,8 +79,8 @@ struct Frame2dArgs {
     const int32_t *fg_off, *fg;
     const int32_t *elem_inst;
     const double *ag;
-    double *q, *vec, *kinit, *scratch, *hist;
-    int32_t *iters;
+    double *q, *vec, *kinit, *scratch, *hist, *dense;
+    int32_t *iters, *piv;
     const SecDefDev *defs;
     SlotTables tb;
     const MatDev *mats;

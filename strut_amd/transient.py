@@ -1,0 +1,154 @@
+"""Host-side driver of the device transient for 2D fiber frames.
+
+``FrameBatch`` registers frames (strut_amd.frame2d.Frame2dCase) and any
+number of ground-motion instances per frame, then runs them all on one GPU
+through the C ABI (sgpu_frame2d_*).  It mirrors what Strut's
+run_transient_nonlinear (transient_nonlinear.mojo:830) does for one case,
+for a whole ensemble at once.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int32, c_void_p
+from typing import List, Optional
+
+import numpy as np
+
+from . import _native
+from ._native import StrutGpuError, dptr, iptr
+from .frame2d import MAX_NIP, Frame2dCase
+from .sections import SectionRuntime
+
+
+class FrameDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", c_int32), ("x", POINTER(c_double)), ("y", POINTER(c_double)),
+        ("constrained", POINTER(c_int32)), ("mass", POINTER(c_double)),
+        ("n_elem", c_int32), ("elem_nodes", POINTER(c_int32)), ("elem_def", POINTER(c_int32)),
+        ("elem_nip", POINTER(c_int32)), ("elem_xi", POINTER(c_double)), ("elem_w", POINTER(c_double)),
+        ("dt", c_double), ("gamma", c_double), ("beta", c_double), ("tol", c_double),
+        ("max_iters", c_int32),
+        ("alpha_m", c_double), ("beta_k", c_double), ("beta_k_init", c_double),
+        ("exc_dir", c_int32),
+        ("n_record", c_int32), ("record_dofs", POINTER(c_int32)),
+    ]
+
+
+class PathSeries(ctypes.Structure):
+    _fields_ = [("n", c_int32), ("values", POINTER(c_double)), ("dt", c_double), ("factor", c_double),
+                ("start_time", c_double), ("use_last", c_int32)]
+
+
+def _declare(lib):
+    if getattr(lib, "_frame2d_declared", False):
+        return
+    sig = {
+        "sgpu_frame2d_create": (ctypes.c_int, [c_void_p, POINTER(FrameDesc), POINTER(c_int32)]),
+        "sgpu_frame2d_add_instance": (ctypes.c_int, [c_void_p, c_int32, c_int32, POINTER(PathSeries),
+                                                     POINTER(c_int32)]),
+        "sgpu_frame2d_finalize": (ctypes.c_int, [c_void_p]),
+        "sgpu_frame2d_launch": (ctypes.c_int, [c_void_p, c_int32]),
+        "sgpu_frame2d_run": (ctypes.c_int, [c_void_p, c_int32]),
+        "sgpu_frame2d_result": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_double), POINTER(c_int32),
+                                               POINTER(c_int32), POINTER(c_int32)]),
+        "sgpu_frame2d_info": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_int32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    lib._frame2d_declared = True
+
+
+class FrameBatch:
+    """Frames x ground motions on one device context."""
+
+    def __init__(self, materials, device: int = 0):
+        self.rt = SectionRuntime(materials, device=device)
+        self.lib = self.rt.lib
+        _declare(self.lib)
+        self.frames: List[Frame2dCase] = []
+        self.frame_defs: List[dict] = []
+        self.n_record: List[int] = []
+        self.insts: List[tuple] = []
+        self._keep = []
+        self.finalized = False
+
+    def _check(self, rc: int) -> int:
+        return self.rt._check(rc)
+
+    def add_frame(self, fc: Frame2dCase, record_all: bool = False) -> int:
+        defs = {sid: self.rt.add_definition(2, fc.section_cells[sid]) for sid in fc.section_ids}
+        rec = np.arange(fc.total_dofs, dtype=np.int32) if record_all else fc.record_dofs.astype(np.int32)
+        k = dict(x=np.ascontiguousarray(fc.x), y=np.ascontiguousarray(fc.y),
+                 con=np.ascontiguousarray(fc.constrained, dtype=np.int32), mass=np.ascontiguousarray(fc.mass),
+                 en=np.ascontiguousarray(fc.elem_nodes.reshape(-1), dtype=np.int32),
+                 ed=np.array([defs[int(s)] for s in fc.elem_section], dtype=np.int32),
+                 nip=np.ascontiguousarray(fc.elem_nip, dtype=np.int32),
+                 xi=np.ascontiguousarray(fc.elem_xi.reshape(-1)), w=np.ascontiguousarray(fc.elem_w.reshape(-1)),
+                 rec=np.ascontiguousarray(rec))
+        assert fc.elem_xi.shape[1] == MAX_NIP
+        d = FrameDesc()
+        d.n_nodes = fc.n_nodes
+        d.x, d.y = dptr(k["x"]), dptr(k["y"])
+        d.constrained = iptr(k["con"])
+        d.mass = dptr(k["mass"])
+        d.n_elem = len(fc.elem_nip)
+        d.elem_nodes, d.elem_def, d.elem_nip = iptr(k["en"]), iptr(k["ed"]), iptr(k["nip"])
+        d.elem_xi, d.elem_w = dptr(k["xi"]), dptr(k["w"])
+        d.dt, d.gamma, d.beta, d.tol, d.max_iters = fc.dt, fc.gamma, fc.beta, fc.tol, fc.max_iters
+        d.alpha_m, d.beta_k, d.beta_k_init, d.exc_dir = fc.alpha_m, fc.beta_k, fc.beta_k_init, fc.exc_dir
+        d.n_record = len(rec)
+        d.record_dofs = iptr(k["rec"])
+        fid = c_int32()
+        self._check(self.lib.sgpu_frame2d_create(self.rt.ctx, ctypes.byref(d), ctypes.byref(fid)))
+        self.frames.append(fc)
+        self.n_record.append(len(rec))
+        self._keep.append(k)
+        return fid.value
+
+    def add_motion(self, frame_id: int, values, dt: float, steps: int, factor: float = 1.0,
+                   start_time: float = 0.0, use_last: bool = False) -> int:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        ps = PathSeries(len(v), dptr(v), dt, factor, start_time, 1 if use_last else 0)
+        iid = c_int32()
+        self._check(self.lib.sgpu_frame2d_add_instance(self.rt.ctx, frame_id, steps, ctypes.byref(ps),
+                                                       ctypes.byref(iid)))
+        self.insts.append((frame_id, steps))
+        return iid.value
+
+    def add_case_motion(self, frame_id: int, fc: Frame2dCase, steps: Optional[int] = None) -> int:
+        return self.add_motion(frame_id, fc.ts_values, fc.ts_dt, fc.steps if steps is None else steps,
+                               fc.ts_factor, fc.ts_start, bool(fc.ts_use_last))
+
+    def finalize(self) -> None:
+        self._check(self.lib.sgpu_frame2d_finalize(self.rt.ctx))
+        self.finalized = True
+
+    def launch(self, steps_per_launch: int) -> None:
+        self._check(self.lib.sgpu_frame2d_launch(self.rt.ctx, steps_per_launch))
+
+    def run(self, steps_per_launch: int = 100) -> None:
+        self._check(self.lib.sgpu_frame2d_run(self.rt.ctx, steps_per_launch))
+
+    def synchronize(self) -> None:
+        self.rt.synchronize()
+
+    def result(self, inst: int) -> dict:
+        frame_id, steps = self.insts[inst]
+        nrec = self.n_record[frame_id]
+        u = np.zeros((steps, nrec))
+        it = np.zeros(steps, dtype=np.int32)
+        st, done = c_int32(), c_int32()
+        self._check(self.lib.sgpu_frame2d_result(self.rt.ctx, inst, dptr(u), iptr(it), ctypes.byref(st),
+                                                 ctypes.byref(done)))
+        return {"u": u, "iters": it, "status": st.value, "steps_done": done.value}
+
+    def info(self, frame_id: int) -> dict:
+        out = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.sgpu_frame2d_info(self.rt.ctx, frame_id, iptr(out)))
+        return {"free": int(out[0]), "hbw": int(out[1]), "record": int(out[2]), "total_dofs": int(out[3])}
+
+    def close(self) -> None:
+        self.rt.close()
