@@ -31,6 +31,7 @@ struct SecDefHost {
     std::vector<int32_t> mat, aux, chunk;
     std::vector<int32_t> batch_chunk_begin;  // n_batches + 1
     std::vector<int32_t> chunk_slot;         // first slot of each chunk
+    std::vector<int32_t> aux_slot;           // slot of each aux index
     std::vector<int32_t> fiber_slot;         // input fiber -> slot
     std::vector<uint8_t> slot_real;          // 1 for real material fibers
     int n_aux = 0;
@@ -39,7 +40,7 @@ struct SecDefHost {
     int init_flex_valid = 0;
     double init_flex[3] = {0, 0, 0};
     int n_inst = 0;
-    int64_t slot_base = 0, batch_base = 0, chunk_base = 0, state_base = 0, aux_base = 0;
+    int64_t slot_base = 0, batch_base = 0, chunk_base = 0, state_base = 0, aux_base = 0, aux_slot_base = 0;
 };
 
 }  // namespace sgpu
@@ -59,7 +60,8 @@ struct sgpu_ctx {
     SecDefDev *d_defs = nullptr;
     double *d_tab_y = nullptr, *d_tab_z = nullptr, *d_tab_area = nullptr;
     int32_t *d_tab_mat = nullptr, *d_tab_aux = nullptr, *d_tab_chunk = nullptr, *d_tab_bcb = nullptr,
-            *d_tab_cslot = nullptr;
+            *d_tab_cslot = nullptr, *d_tab_auxslot = nullptr;
+    int max_aux = 0;  // most aux slots of any definition
     StateArrays c{}, t{};
     std::vector<void *> allocations;
     // scratch for the host-buffer entry points
@@ -67,6 +69,8 @@ struct sgpu_ctx {
     sgpu_section_result *d_res = nullptr;
     int scratch_cap = 0;
     int32_t *d_list = nullptr;
+    double *d_fsks = nullptr;  // (sigma A, E_t A) of aux slots, per point: the aux pre-pass output
+    int64_t fsks_cap = 0;
     int list_cap = 0;
     // 2D frame transient instances (frame2d_api.cpp)
     sgpu::FrameHost *frames = nullptr;
@@ -80,6 +84,7 @@ struct sgpu_ctx {
         tb.chunk = d_tab_chunk;
         tb.batch_chunk_begin = d_tab_bcb;
         tb.chunk_slot = d_tab_cslot;
+        tb.aux_slot = d_tab_auxslot;
         return tb;
     }
 };

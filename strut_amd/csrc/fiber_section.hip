@@ -22,170 +22,303 @@
 
 namespace sgpu {
 
+constexpr int kPointsPerBlock = 4;         // one wavefront per section point
+constexpr int kRedStride = kBatch + 1;     // LDS row stride (odd, so the NQ rows sit in different banks)
+constexpr bool kPrefetch = true;           // load batch b + 1 while batch b is reduced
+
+// Orders the LDS traffic of one wavefront: writes before the barrier are
+// seen by reads after it (LDS requests of one wave complete in order; this
+// only keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Section term q of one slot from its fiber force fs = sigma A and stiffness
+// ks = E_t A (fiber2d.mojo:2065-2082, fiber3d.mojo:804-822).
 template <int DIM>
-__device__ void section_point(const SecDefDev &d, const SlotTables &tb, const MatDev *__restrict__ mats,
-                              const StateArrays &c, const StateArrays &t, int inst, double eps0,
-                              double kz, double ky, sgpu_section_result *out) {
-    constexpr int NQ = DIM == 2 ? 5 : 9;
-    __shared__ double red[9][kBatch + 1];
-    __shared__ double fin[9];
-    const int lane = threadIdx.x;
-    const int64_t inst_state = d.state_base + (int64_t)inst * d.n_slots;
-    const int64_t inst_aux = d.aux_base + (int64_t)inst * d.n_aux;
-    const int32_t *bcb = tb.batch_chunk_begin + d.batch_base;
-    const int n_chunks_total = bcb[d.n_batches];
-    double acc = 0.0;
-    for (int b = 0; b < d.n_batches; ++b) {
-        const int s = b * kBatch + lane;
-        double term[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) term[q] = 0.0;
-        int code = -1;
-        if (s < d.n_slots) {
-            const int64_t gs = d.slot_base + s;
-            const int mi = tb.mat[gs];
-            code = tb.chunk[gs];
-            if (mi >= 0) {
-                const double y = tb.y[gs];
-                const double area = tb.area[gs];
-                double strain, z = 0.0;
-                if (DIM == 2) {
-                    strain = eps0 - y * kz;
-                } else {
-                    z = tb.z[gs];
-                    strain = eps0 + z * ky - y * kz;
-                }
-                const int32_t aux = tb.aux[gs];
-                double fs, ks;
-                eval_slot(mats, mi, strain, area, c, t, inst_state + s, inst_aux + (aux < 0 ? 0 : aux), fs,
-                          ks);
-                if (DIM == 2) {
-                    term[0] = fs;
-                    term[1] = -fs * y;
-                    term[2] = ks;
-                    term[3] = -ks * y;
-                    term[4] = ks * y * y;
-                } else {
-                    term[0] = fs;
-                    term[1] = fs * z;
-                    term[2] = -fs * y;
-                    term[3] = ks;
-                    term[4] = ks * z;
-                    term[5] = -ks * y;
-                    term[6] = ks * z * z;
-                    term[7] = -ks * z * y;
-                    term[8] = ks * y * y;
-                }
-            }
+__device__ __forceinline__ double slot_term(int q, double fs, double ks, double y, double z) {
+    if (DIM == 2) {
+        switch (q) {
+        case 0: return fs;
+        case 1: return -fs * y;
+        case 2: return ks;
+        case 3: return -ks * y;
+        default: return ks * y * y;
         }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const double tr = tree8(term[q]);
-            if (code >= 0) red[q][code >> 1] = (code & 1) ? tr : term[q];
-        }
-        __syncthreads();
-        if (lane < NQ) {
-            const int c0 = bcb[b], c1 = bcb[b + 1];
-            const bool zero_lane = d.zero_k12 && (DIM == 2 ? lane == 3 : (lane == 4 || lane == 5));
-            for (int cc = c0; cc < c1; ++cc) {
-                if (zero_lane && cc == d.n_elastic_chunks) acc = 0.0;
-                acc += red[lane][cc - c0];
-            }
-        }
-        __syncthreads();
     }
-    if (lane < NQ) {
-        const bool zero_lane = d.zero_k12 && (DIM == 2 ? lane == 3 : (lane == 4 || lane == 5));
-        if (zero_lane && n_chunks_total <= d.n_elastic_chunks) acc = 0.0;
-        fin[lane] = acc;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        sgpu_section_result r;
-        for (int q = 0; q < 3; ++q) r.s[q] = 0.0;
-        for (int q = 0; q < 6; ++q) {
-            r.k[q] = 0.0;
-            r.f[q] = 0.0;
-        }
-        r.pad_ = 0;
-        if (DIM == 2) {
-            // fiber2d.mojo:3074-3085
-            const double k11 = fin[2], k12 = fin[3], k22 = fin[4];
-            r.s[0] = fin[0];
-            r.s[1] = fin[1];
-            r.k[0] = k11;
-            r.k[1] = k12;
-            r.k[2] = k22;
-            const double det = k11 * k22 - k12 * k12;
-            if (fabs(det) <= 1.0e-40) {
-                r.valid = 0;
-            } else {
-                const double inv_det = 1.0 / det;
-                r.f[0] = k22 * inv_det;
-                r.f[1] = -k12 * inv_det;
-                r.f[2] = k11 * inv_det;
-                r.valid = 1;
-            }
-        } else {
-            // force_beam_column3d.mojo:679-727 applied as :1093-1117
-            r.s[0] = fin[0];
-            r.s[1] = fin[1];
-            r.s[2] = fin[2];
-            const double a = fin[3], bb = fin[4], cq = fin[5], e = fin[6], f = fin[7], i = fin[8];
-            const double dd = bb, g = cq, h = f;
-            r.k[0] = a;
-            r.k[1] = bb;
-            r.k[2] = cq;
-            r.k[3] = e;
-            r.k[4] = f;
-            r.k[5] = i;
-            const double co00 = e * i - f * h;
-            const double co01 = -(dd * i - f * g);
-            const double co02 = dd * h - e * g;
-            const double co10 = -(bb * i - cq * h);
-            const double co11 = a * i - cq * g;
-            const double co20 = bb * f - cq * e;
-            const double co21 = -(a * f - cq * dd);
-            const double co22 = a * e - bb * dd;
-            const double det = a * co00 + bb * co01 + cq * co02;
-            if (fabs(det) <= 1.0e-40) {
-                r.valid = 0;
-            } else {
-                const double inv_det = 1.0 / det;
-                r.f[0] = co00 * inv_det;
-                r.f[1] = co10 * inv_det;
-                r.f[2] = co20 * inv_det;
-                r.f[3] = co11 * inv_det;
-                r.f[4] = co21 * inv_det;
-                r.f[5] = co22 * inv_det;
-                r.valid = 1;
-            }
-        }
-        *out = r;
+    switch (q) {
+    case 0: return fs;
+    case 1: return fs * z;
+    case 2: return -fs * y;
+    case 3: return ks;
+    case 4: return ks * z;
+    case 5: return -ks * y;
+    case 6: return ks * z * z;
+    case 7: return -ks * z * y;
+    default: return ks * y * y;
     }
 }
 
-__global__ __launch_bounds__(64) void section_eval_kernel(const SecDefDev *__restrict__ defs, int n_defs,
-                                                          SlotTables tb, const MatDev *__restrict__ mats,
-                                                          StateArrays c, StateArrays t,
-                                                          const sgpu_section_point *__restrict__ pts,
-                                                          sgpu_section_result *__restrict__ res, int n) {
-    const int p = blockIdx.x;
+// Everything one lane reads for its slot of a batch before the law runs:
+// the slot's table entry and the committed history of the common fields.
+struct SlotIn {
+    int mi, code, aux;
+    double y, z, area;
+    double eps, sig, tan, h0, h1;
+};
+
+__device__ __forceinline__ void load_slot_in(const SecDefDev &d, const SlotTables &tb, const StateArrays &c,
+                                             int64_t inst_state, int s, bool dim3, SlotIn &in) {
+    in.mi = -1;
+    in.code = -1;
+    in.aux = -1;
+    in.y = in.z = in.area = 0.0;
+    in.eps = in.sig = in.tan = in.h0 = in.h1 = 0.0;
+    if (s >= d.n_slots) return;
+    const int64_t gs = d.slot_base + s;
+    const int64_t gi = inst_state + s;
+    in.mi = tb.mat[gs];
+    in.code = tb.chunk[gs];
+    in.aux = tb.aux[gs];
+    in.y = tb.y[gs];
+    in.area = tb.area[gs];
+    if (dim3) in.z = tb.z[gs];
+    // The committed history is read for every slot (the address does not
+    // depend on the table), so the loads of a batch issue together.
+    in.eps = c.eps[gi];
+    in.sig = c.sig[gi];
+    in.tan = c.tan[gi];
+    in.h0 = c.h0[gi];
+    in.h1 = c.h1[gi];
+}
+
+// One section point, evaluated by one wavefront.  Lanes walk the slots 64 at
+// a time; each batch leaves its chunk partials in the wave's LDS rows, and
+// lanes 0..NQ-1 add them one after another onto their running sums, so the
+// sums see the chunks in exactly the sequential order of the reference.
+template <int DIM>
+__device__ void section_point(const SecDefDev &d, const SlotTables &tb, const MatDev *__restrict__ mats,
+                              const StateArrays &t, int inst, double eps0, double kz, double ky,
+                              const double *__restrict__ fsks, double *red, const StateArrays &c,
+                              sgpu_section_result *out) {
+    constexpr int NQ = DIM == 2 ? 5 : 9;
+    const int lane = threadIdx.x & (kBatch - 1);
+    const int64_t inst_state = d.state_base + (int64_t)inst * d.n_slots;
+    const int32_t *bcb = tb.batch_chunk_begin + d.batch_base;
+    const bool zero_lane = d.zero_k12 && (DIM == 2 ? lane == 3 : (lane == 4 || lane == 5));
+    double acc = 0.0;
+    SlotIn cur;
+    load_slot_in(d, tb, c, inst_state, lane, DIM == 3, cur);
+    for (int b = 0; b < d.n_batches; ++b) {
+        SlotIn nxt;
+        if (kPrefetch && b + 1 < d.n_batches) load_slot_in(d, tb, c, inst_state, (b + 1) * kBatch + lane, DIM == 3, nxt);
+        double fs = 0.0, ks = 0.0;
+        if (cur.mi >= 0) {
+            if (cur.aux >= 0) {
+                fs = fsks[2 * cur.aux];
+                ks = fsks[2 * cur.aux + 1];
+            } else {
+                const MatDev &m = mats[cur.mi];
+                const double strain = DIM == 2 ? eps0 - cur.y * kz : eps0 + cur.z * ky - cur.y * kz;
+                if (m.type == kElastic) {
+                    ks = m.p[0] * cur.area;
+                    fs = ks * strain;
+                } else {
+                    FiberHist hc, ht;
+                    hc.eps = cur.eps;
+                    hc.sig = cur.sig;
+                    hc.tan = cur.tan;
+                    hc.h0 = cur.h0;
+                    hc.h1 = cur.h1;
+                    hc.a0 = hc.a1 = hc.a2 = hc.a3 = hc.a4 = 0.0;
+                    hc.kon = 0;
+                    ht = hc;
+                    if (m.type == kConcrete02)
+                        concrete02_trial(m, hc, strain, ht);
+                    else
+                        steel01_trial(m, hc, strain, ht);
+                    const int64_t gi = inst_state + b * kBatch + lane;
+                    t.eps[gi] = ht.eps;
+                    t.sig[gi] = ht.sig;
+                    t.tan[gi] = ht.tan;
+                    t.h0[gi] = ht.h0;
+                    t.h1[gi] = ht.h1;
+                    fs = ht.sig * cur.area;
+                    ks = ht.tan * cur.area;
+                }
+            }
+        }
+        const int cbase = bcb[b];
+        const int dst = cur.code >= 0 ? (cur.code >> 1) : -1;
+        const bool w8 = cur.code >= 0 && (cur.code & 1);
+        if (__builtin_amdgcn_ballot_w64(w8) != 0) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const double term = slot_term<DIM>(q, fs, ks, cur.y, cur.z);
+                const double tr = tree8_dpp(term);
+                if (dst >= 0) red[q * kRedStride + dst] = w8 ? tr : term;
+            }
+        } else {  // every chunk of the batch is a single slot
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                if (dst >= 0) red[q * kRedStride + dst] = slot_term<DIM>(q, fs, ks, cur.y, cur.z);
+        }
+        wave_sync();
+        if (lane < NQ) {
+            const int c1 = bcb[b + 1];
+            const double *row = red + lane * kRedStride - cbase;
+            int cc = cbase;
+            if (zero_lane && d.n_elastic_chunks >= cbase && d.n_elastic_chunks < c1) {
+                for (; cc < d.n_elastic_chunks; ++cc) acc += row[cc];
+                acc = 0.0;  // uniform elastic modulus: k12 (k13) restarts with the nonlinear families
+            }
+#pragma unroll 8
+            for (; cc < c1; ++cc) acc += row[cc];
+        }
+        wave_sync();
+        if (kPrefetch) {
+            cur = nxt;
+        } else if (b + 1 < d.n_batches) {
+            load_slot_in(d, tb, c, inst_state, (b + 1) * kBatch + lane, DIM == 3, cur);
+        }
+    }
+    if (zero_lane && bcb[d.n_batches] <= d.n_elastic_chunks) acc = 0.0;
+    double fin[9];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) fin[q] = readlane_d(acc, q);
+    if (lane != 0) return;
+    sgpu_section_result r;
+    for (int q = 0; q < 3; ++q) r.s[q] = 0.0;
+    for (int q = 0; q < 6; ++q) {
+        r.k[q] = 0.0;
+        r.f[q] = 0.0;
+    }
+    r.pad_ = 0;
+    if (DIM == 2) {
+        // fiber2d.mojo:3074-3085
+        const double k11 = fin[2], k12 = fin[3], k22 = fin[4];
+        r.s[0] = fin[0];
+        r.s[1] = fin[1];
+        r.k[0] = k11;
+        r.k[1] = k12;
+        r.k[2] = k22;
+        const double det = k11 * k22 - k12 * k12;
+        if (fabs(det) <= 1.0e-40) {
+            r.valid = 0;
+        } else {
+            const double inv_det = 1.0 / det;
+            r.f[0] = k22 * inv_det;
+            r.f[1] = -k12 * inv_det;
+            r.f[2] = k11 * inv_det;
+            r.valid = 1;
+        }
+    } else {
+        // force_beam_column3d.mojo:679-727 applied as :1093-1117
+        r.s[0] = fin[0];
+        r.s[1] = fin[1];
+        r.s[2] = fin[2];
+        const double a = fin[3], bb = fin[4], cq = fin[5], e = fin[6], f = fin[7], i = fin[8];
+        const double dd = bb, g = cq, h = f;
+        r.k[0] = a;
+        r.k[1] = bb;
+        r.k[2] = cq;
+        r.k[3] = e;
+        r.k[4] = f;
+        r.k[5] = i;
+        const double co00 = e * i - f * h;
+        const double co01 = -(dd * i - f * g);
+        const double co02 = dd * h - e * g;
+        const double co10 = -(bb * i - cq * h);
+        const double co11 = a * i - cq * g;
+        const double co20 = bb * f - cq * e;
+        const double co21 = -(a * f - cq * dd);
+        const double co22 = a * e - bb * dd;
+        const double det = a * co00 + bb * co01 + cq * co02;
+        if (fabs(det) <= 1.0e-40) {
+            r.valid = 0;
+        } else {
+            const double inv_det = 1.0 / det;
+            r.f[0] = co00 * inv_det;
+            r.f[1] = co10 * inv_det;
+            r.f[2] = co20 * inv_det;
+            r.f[3] = co11 * inv_det;
+            r.f[4] = co21 * inv_det;
+            r.f[5] = co22 * inv_det;
+            r.valid = 1;
+        }
+    }
+    *out = r;
+}
+
+__device__ __forceinline__ bool point_ok(const SecDefDev *__restrict__ defs, int n_defs,
+                                         const sgpu_section_point &pt) {
+    return pt.def_id >= 0 && pt.def_id < n_defs && pt.instance >= 0 && pt.instance < defs[pt.def_id].n_inst;
+}
+
+// Aux pre-pass: the Steel02 and Concrete01 slots of every point (few per
+// section) evaluated from their committed history; the trial history is
+// stored and (sigma A, E_t A) left in fsks[point][aux][2] for the main pass.
+// Lanes are packed densely: lane l of a block serves point
+// blockIdx.x * (blockDim.x / per) + l / per and aux index l % per (plus
+// multiples of per below the definition's n_aux), per = min(max_aux, 256).
+__global__ __launch_bounds__(256) void aux_eval_kernel(const SecDefDev *__restrict__ defs, int n_defs,
+                                                       SlotTables tb, const MatDev *__restrict__ mats,
+                                                       StateArrays c, StateArrays t,
+                                                       const sgpu_section_point *__restrict__ pts, int n,
+                                                       double *__restrict__ fsks, int max_aux, int per) {
+    const int ppb = blockDim.x / per;
+    const int k = threadIdx.x / per;
+    if (k >= ppb) return;
+    const int p = blockIdx.x * ppb + k;
     if (p >= n) return;
     const sgpu_section_point pt = pts[p];
-    if (pt.def_id < 0 || pt.def_id >= n_defs || pt.instance < 0 || pt.instance >= defs[pt.def_id].n_inst) {
-        if (threadIdx.x == 0) {
+    if (!point_ok(defs, n_defs, pt)) return;
+    const SecDefDev &d = defs[pt.def_id];
+    const int64_t inst_state = d.state_base + (int64_t)pt.instance * d.n_slots;
+    const int64_t inst_aux = d.aux_base + (int64_t)pt.instance * d.n_aux;
+    for (int j = threadIdx.x - k * per; j < d.n_aux; j += per) {
+        const int s = tb.aux_slot[d.aux_slot_base + j];
+        const int64_t gs = d.slot_base + s;
+        const int mi = tb.mat[gs];
+        const double y = tb.y[gs];
+        const double area = tb.area[gs];
+        const double strain = d.dim == 2 ? pt.eps0 - y * pt.kappa_z : pt.eps0 + tb.z[gs] * pt.kappa_y - y * pt.kappa_z;
+        double fs, ks;
+        eval_slot(mats, mi, strain, area, c, t, inst_state + s, inst_aux + j, fs, ks);
+        double *o = fsks + ((int64_t)p * max_aux + j) * 2;
+        o[0] = fs;
+        o[1] = ks;
+    }
+}
+
+// Main pass: every point's slots without aux state, and the section
+// reduction.  Wave w of a block evaluates point blockIdx.x * kPointsPerBlock + w.
+__global__ __launch_bounds__(kPointsPerBlock * kBatch) void section_eval_kernel(
+    const SecDefDev *__restrict__ defs, int n_defs, SlotTables tb, const MatDev *__restrict__ mats, StateArrays c,
+    StateArrays t, const sgpu_section_point *__restrict__ pts, sgpu_section_result *__restrict__ res, int n,
+    const double *__restrict__ fsks, int max_aux) {
+    __shared__ double red[kPointsPerBlock][9 * kRedStride];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // wave-uniform point index: point and definition loads go through the scalar unit
+    const int p = blockIdx.x * kPointsPerBlock + wave;
+    if (p >= n) return;
+    const sgpu_section_point pt = pts[p];
+    if (!point_ok(defs, n_defs, pt)) {
+        if ((threadIdx.x & (kBatch - 1)) == 0) {
             sgpu_section_result r = {};
             r.valid = -1;
             res[p] = r;
         }
         return;
     }
-    const SecDefDev d = defs[pt.def_id];
+    const SecDefDev &d = defs[pt.def_id];
+    const double *pf = fsks + (int64_t)p * max_aux * 2;
     if (d.dim == 2)
-        section_point<2>(d, tb, mats, c, t, pt.instance, pt.eps0, pt.kappa_z, 0.0, res + p);
+        section_point<2>(d, tb, mats, t, pt.instance, pt.eps0, pt.kappa_z, 0.0, pf, red[wave], c, res + p);
     else
-        section_point<3>(d, tb, mats, c, t, pt.instance, pt.eps0, pt.kappa_z, pt.kappa_y, res + p);
+        section_point<3>(d, tb, mats, t, pt.instance, pt.eps0, pt.kappa_z, pt.kappa_y, pf, red[wave], c, res + p);
 }
 
 // Plain copies between the committed and trial arrays (commit: src = trial).
@@ -237,10 +370,18 @@ __global__ __launch_bounds__(256) void copy_instances_kernel(const SecDefDev *__
 
 void launch_section_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
                          const MatDev *mats, const StateArrays &c, const StateArrays &t,
-                         const sgpu_section_point *pts, sgpu_section_result *res, int n) {
+                         const sgpu_section_point *pts, sgpu_section_result *res, int n, double *fsks,
+                         int max_aux) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(section_eval_kernel, dim3(n), dim3(kBatch), 0, stream, defs, n_defs, tb, mats, c, t,
-                       pts, res, n);
+    const int blocks = (n + kPointsPerBlock - 1) / kPointsPerBlock;
+    if (max_aux > 0) {
+        const int per = max_aux < 256 ? max_aux : 256;
+        const int ppb = 256 / per;
+        hipLaunchKernelGGL(aux_eval_kernel, dim3((n + ppb - 1) / ppb), dim3(256), 0, stream, defs, n_defs, tb, mats,
+                           c, t, pts, n, fsks, max_aux, per);
+    }
+    hipLaunchKernelGGL(section_eval_kernel, dim3(blocks), dim3(kPointsPerBlock * kBatch), 0, stream, defs, n_defs,
+                       tb, mats, c, t, pts, res, n, fsks, max_aux);
 }
 
 void launch_copy_state(hipStream_t stream, const StateArrays &dst, const StateArrays &src, int64_t n_slots,

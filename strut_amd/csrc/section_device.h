@@ -18,6 +18,24 @@ __device__ __forceinline__ double tree8(double v) {
     return x + __shfl_xor(x, 1, 64);
 }
 
+// Value of lane l + N (N = 1, 2, 4) within the lane's 16-lane DPP row
+// (row_shl); lanes whose source falls outside the row read 0.
+template <int N>
+__device__ __forceinline__ double dpp_shl(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x100 | N, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x100 | N, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// The W = 8 halving tree ((v0+v4)+(v2+v6))+((v1+v5)+(v3+v7)) of the eight
+// lanes 8k..8k+7, left in lane 8k (other lanes hold partial garbage).  Same
+// pairs as tree8, formed with DPP row shifts instead of LDS permutes.
+__device__ __forceinline__ double tree8_dpp(double v) {
+    const double w = v + dpp_shl<4>(v);
+    const double x = w + dpp_shl<2>(w);
+    return x + dpp_shl<1>(x);
+}
+
 __device__ __forceinline__ void load_hist(const StateArrays &s, int64_t gi, int64_t ai, int type,
                                           FiberHist &h) {
     h.a0 = h.a1 = h.a2 = h.a3 = h.a4 = 0.0;
@@ -81,6 +99,32 @@ __device__ __forceinline__ void eval_slot(const MatDev *__restrict__ mats, int m
     default: concrete01_trial(m, hc, strain, ht); break;
     }
     store_hist(t, gi, ai, type, ht);
+    fs = ht.sig * area;
+    ks = ht.tan * area;
+}
+
+// Same for the laws without aux state (Elastic, Concrete02, Steel01); the
+// aux-state laws (Steel02, Concrete01) are evaluated by the aux pre-pass of
+// the section-point kernel, which keeps the heavy Steel02 code (and its pow)
+// out of the register budget of the main pass.
+__device__ __forceinline__ void eval_slot_common(const MatDev *__restrict__ mats, int mi, double strain,
+                                                 double area, const StateArrays &c, const StateArrays &t,
+                                                 int64_t gi, double &fs, double &ks) {
+    const MatDev &m = mats[mi];
+    const int type = m.type;
+    if (type == kElastic) {
+        ks = m.p[0] * area;
+        fs = ks * strain;
+        return;
+    }
+    FiberHist hc, ht;
+    load_hist(c, gi, 0, kConcrete02, hc);
+    ht = hc;
+    if (type == kConcrete02)
+        concrete02_trial(m, hc, strain, ht);
+    else
+        steel01_trial(m, hc, strain, ht);
+    store_hist(t, gi, 0, kConcrete02, ht);
     fs = ht.sig * area;
     ks = ht.tan * area;
 }

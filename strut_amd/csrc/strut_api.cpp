@@ -19,7 +19,8 @@
 namespace sgpu {
 void launch_section_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
                          const MatDev *mats, const StateArrays &c, const StateArrays &t,
-                         const sgpu_section_point *pts, sgpu_section_result *res, int n);
+                         const sgpu_section_point *pts, sgpu_section_result *res, int n, double *fsks,
+                         int max_aux);
 void launch_copy_state(hipStream_t stream, const StateArrays &dst, const StateArrays &src, int64_t n_slots,
                        int64_t n_aux);
 void launch_copy_instances(hipStream_t stream, const SecDefDev *defs, const int32_t *def_ids,
@@ -188,7 +189,10 @@ int build_def(sgpu_ctx *ctx, SecDefHost &d, const std::vector<Cell> &cells) {
         const int mi = d.mat[s];
         if (mi < 0) continue;
         const int ty = ctx->mats[mi].type;
-        if (ty == kSteel02 || ty == kConcrete01) d.aux[s] = d.n_aux++;
+        if (ty == kSteel02 || ty == kConcrete01) {
+            d.aux[s] = d.n_aux++;
+            d.aux_slot.push_back((int32_t)s);
+        }
     }
     // chunk codes and per-batch chunk ranges
     const int n_slots = (int)d.y.size();
@@ -253,6 +257,7 @@ int ensure_scratch(sgpu_ctx *ctx, int n) {
 int ensure_list(sgpu_ctx *ctx, int n) {
     if (2 * n <= ctx->list_cap) return SGPU_OK;
     if (ctx->d_list) (void)hipFree(ctx->d_list);
+    if (ctx->d_fsks) (void)hipFree(ctx->d_fsks);
     ctx->d_list = nullptr;
     int cap = 2 * n < 1024 ? 1024 : 2 * n;
     HIP_OK(ctx, hipMalloc((void **)&ctx->d_list, sizeof(int32_t) * cap));
@@ -394,13 +399,17 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
     HIP_OK(ctx, hipSetDevice(ctx->device));
     std::vector<SecDefDev> ddefs;
     std::vector<double> ty, tz, ta;
-    std::vector<int32_t> tm, tx, tc, tb, tcs;
+    std::vector<int32_t> tm, tx, tc, tb, tcs, tas;
+    ctx->max_aux = 0;
     int64_t state = 0, aux = 0, fibers = 0;
     for (SecDefHost &d : ctx->defs) {
         d.slot_base = (int64_t)ty.size();
         d.batch_base = (int64_t)tb.size();
         d.chunk_base = (int64_t)tcs.size();
         tcs.insert(tcs.end(), d.chunk_slot.begin(), d.chunk_slot.end());
+        d.aux_slot_base = (int64_t)tas.size();
+        tas.insert(tas.end(), d.aux_slot.begin(), d.aux_slot.end());
+        if (d.n_aux > ctx->max_aux) ctx->max_aux = d.n_aux;
         d.state_base = state;
         d.aux_base = aux;
         ty.insert(ty.end(), d.y.begin(), d.y.end());
@@ -427,6 +436,7 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
         dd.chunk_base = d.chunk_base;
         dd.state_base = d.state_base;
         dd.aux_base = d.aux_base;
+        dd.aux_slot_base = d.aux_slot_base;
         for (int k = 0; k < 3; ++k) dd.init_flex[k] = d.init_flex[k];
         dd.init_flex_valid = d.init_flex_valid;
         ddefs.push_back(dd);
@@ -443,6 +453,7 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &ctx->d_tab_area, ta))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_mat, tm))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_aux, tx))) return rc;
+    if ((rc = upload(ctx, &ctx->d_tab_auxslot, tas))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_chunk, tc))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_bcb, tb))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_cslot, tcs))) return rc;
@@ -504,8 +515,17 @@ int sgpu_section_eval_device(sgpu_ctx *ctx, int n, const sgpu_section_point *d_p
     if (!ctx) return SGPU_ERR_ARG;
     if (!ctx->finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_sections_finalize has not run");
     if (n < 0 || (n > 0 && (!d_points || !d_results))) return fail(ctx, SGPU_ERR_ARG, "bad point buffers");
+    if (n == 0) return SGPU_OK;
+    const int64_t need = (int64_t)n * ctx->max_aux * 2;
+    if (need > ctx->fsks_cap) {
+        if (ctx->d_fsks) HIP_OK(ctx, hipFree(ctx->d_fsks));
+        ctx->d_fsks = nullptr;
+        ctx->fsks_cap = 0;
+        HIP_OK(ctx, hipMalloc((void **)&ctx->d_fsks, sizeof(double) * need));
+        ctx->fsks_cap = need;
+    }
     launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c,
-                        ctx->t, d_points, d_results, n);
+                        ctx->t, d_points, d_results, n, ctx->d_fsks, ctx->max_aux);
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;
 }

@@ -48,6 +48,7 @@ struct SecDefDev {
     int64_t chunk_base;        // into chunk_slot (one entry per chunk)
     int64_t state_base;        // first slot-state index of instance 0
     int64_t aux_base;          // first aux-state index of instance 0
+    int64_t aux_slot_base;     // into aux_slot (n_aux entries)
     double init_flex[6];       // 2D: {f00, f01, f11}; valid flag in init_flex_valid
     int32_t init_flex_valid;
     int32_t pad_;
@@ -65,6 +66,7 @@ struct SlotTables {
     const int32_t *chunk;   // chunk code (see above)
     const int32_t *batch_chunk_begin;
     const int32_t *chunk_slot;  // first slot of each chunk
+    const int32_t *aux_slot;    // slot of each aux index
 };
 
 // Fiber state, structure of arrays over all instances of all definitions.
