@@ -1,16 +1,28 @@
-"""Benchmark: fiber updates/s of the fiber-section state determination path
-on one MI355X.
+"""Benchmark of the fiber-section state determination path on one MI355X.
 
-Workload: the section population of the 3D RC frame configuration (20
-stories x 5 x 5 bays; 720 forceBeamColumn columns and 1200 dispBeamColumn
-beams, 5 integration points each, 12 x 12 confined core + 2-fiber cover +
-12 Steel02 bars per section).  One step = every section evaluated once at a
-seeded synthetic deformation (trial fiber states, section resultants,
-tangents and flexibilities), then every fiber committed.  Inputs are
-resident in HBM before the timed region.
+Two legs, one JSON line.
 
-Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits:
-this build measures one GPU.
+1. Fiber updates/s (``value``).  Workload: the section population of the 3D
+   RC frame (20 stories x 5 x 5 bays; 720 forceBeamColumn columns and 1200
+   dispBeamColumn beams, 5 integration points each, 12 x 12 confined core +
+   2-fiber cover + 12 Steel02 bars per section = 268 fibers; 9600 sections,
+   2.57 M fibers).  As Strut's fiber micro-benchmark does
+   (benchmarks_micro.mojo:234-263), a preload history of committed steps
+   comes first (the warmup steps: evaluate + commit), then every timed step
+   evaluates every section once at a new seeded deformation from the
+   committed state (trial fiber states, section resultants, tangents and
+   flexibilities) without committing.  1 fiber update = one fiber's trial
+   evaluation (benchmarks_micro.mojo:250).  Inputs are resident in HBM
+   before the timed region.  The commit copy is timed separately.
+
+2. Ground-motion steps/s (``config.gm_steps``).  The 2D RC frame of 3
+   stories x 1 bay (forceBeamColumn, 5 Lobatto points, 204-fiber sections),
+   1000 steps of dt = 0.01 s under seeded band-limited ground motions at
+   PGA 0.4 g: one record alone (the plain path: records solved one by one)
+   and N records batched in one launch on the same GPU, plus the oracle's
+   single-core rate for one record.
+
+`--gpus N` with N > 1 exits: this build measures one GPU.
 """
 
 from __future__ import annotations
@@ -27,7 +39,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
-BYTES_PER_UPDATE = {4: 80, 3: 168}  # Concrete02, Steel02: committed read + trial write
+# Algorithmic bytes per fiber update: committed history read + trial history write.
+BYTES_PER_UPDATE = {4: 80, 3: 168, 1: 80, 2: 96}  # Concrete02, Steel02, Steel01, Concrete01
+AUX_TYPES = (3, 2)  # laws evaluated by aux_eval_kernel (Steel02, Concrete01)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "section_eval_kernel_traffic.json")
 
 
 def build_population():
@@ -42,19 +57,23 @@ def build_population():
     return mats, cells, counts, pop
 
 
-def algorithmic_bytes(mats, cells, counts) -> int:
+def algorithmic_bytes(mats, cells, counts, aux: bool) -> int:
+    """Per evaluation pass, over the fibers one kernel updates: the aux laws
+    (aux=True, aux_eval_kernel) or the others (section_eval_kernel)."""
     total = 0
     for c, n in zip(cells, counts):
-        per = sum(BYTES_PER_UPDATE.get(int(mats.defs[m].mat_type), 0) for m in c["mat"])
-        total += per * n
+        for m in c["mat"]:
+            ty = int(mats.defs[m].mat_type)
+            if (ty in AUX_TYPES) == aux:
+                total += BYTES_PER_UPDATE.get(ty, 0) * n
     return total
 
 
-def cpu_baseline(mats, cells, counts, budget_s: float = 12.0) -> dict:
-    """The oracle (single-threaded C restatement) on a bounded sample: the
-    first 48 section instances of the population (24 column, 24 beam), the
-    same deformation sequence, evaluated step after step until the time
-    budget is spent."""
+def cpu_baseline(mats, cells, budget_s: float = 12.0) -> dict:
+    """The oracle (single-threaded C restatement) on a bounded sample of the
+    same workload: 48 section instances (24 column, 24 beam), a 3-step
+    committed preload, then trial evaluations at new seeded deformations
+    without commit until the time budget is spent."""
     from oracle import oracle
     from strut_amd import cases
 
@@ -65,39 +84,33 @@ def cpu_baseline(mats, cells, counts, budget_s: float = 12.0) -> dict:
         r.alloc(24)
         refs.append((r, len(c)))
     n = 48
+    insts = np.arange(24)
+    for s in range(3):
+        e, kz, ky = cases.synthetic_section_deformation(n, s, seed=1234, dim=3)
+        for g, (r, _) in enumerate(refs):
+            sl = slice(24 * g, 24 * g + 24)
+            r.eval_many(insts, e[sl], ky[sl], kz[sl], commit=True)
     steps = 0
     updates = 0
     t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget_s:
-        e, kz, ky = cases.synthetic_section_deformation(n, steps, seed=1234, dim=3)
+        e, kz, ky = cases.synthetic_section_deformation(n, 3 + steps, seed=1234, dim=3)
         for g, (r, nf) in enumerate(refs):
             sl = slice(24 * g, 24 * g + 24)
-            r.eval_commit(np.arange(24), e[sl], ky[sl], kz[sl])
+            r.eval_many(insts, e[sl], ky[sl], kz[sl], commit=False)
             updates += nf * 24
         steps += 1
     dt = time.perf_counter() - t0
     return {"value": updates / dt, "unit": "fiber updates/s", "cores": 1, "kind": "port",
-            "sample": f"48 section instances (24 column, 24 beam, {sum(len(c) for c in cells) // 2 * 2} "
-                      f"fibers per pair) x {steps} steps of trial evaluation + commit, {dt:.1f} s, "
-                      f"oracle C restatement, 1 thread"}
+            "sample": f"48 section instances (24 column + 24 beam, {len(cells[0])} fibers each) x {steps} "
+                      f"trial evaluations after a 3-step committed preload, {dt:.1f} s, oracle C restatement, "
+                      "1 thread"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(ensembles and models shard across GPUs without collectives; see the design notes)",
-              file=sys.stderr)
-        return 2
-
+def section_leg(args) -> dict:
     import torch
 
-    from strut_amd import cases
+    from strut_amd import _native, cases
     from strut_amd._native import POINT_DTYPE, RESULT_DTYPE
     from strut_amd.sections import SectionRuntime
 
@@ -107,7 +120,7 @@ def main() -> int:
     for d, n in zip(defs, counts):
         rt.alloc_instances(d, n)
     rt.finalize()
-    fibers, slots = rt.totals()
+    fibers, _ = rt.totals()
     n_pts = sum(counts)
     did = np.concatenate([np.full(n, d, dtype=np.int32) for d, n in zip(defs, counts)])
     inst = np.concatenate([np.arange(n, dtype=np.int32) for n in counts])
@@ -122,55 +135,50 @@ def main() -> int:
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(rt.stream)
 
+    # preload history: committed steps
     for s in range(args.warmup):
         rt.eval_device(n_pts, d_pts[s].data_ptr(), d_res.data_ptr())
         rt.commit_all()
     rt.synchronize()
     torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+
+    rt.profile_enable(True)
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     t_start.record(stream)
     for k in range(args.steps):
-        s = args.warmup + k
-        ev[k][0].record(stream)
-        rt.eval_device(n_pts, d_pts[s].data_ptr(), d_res.data_ptr())
-        ev[k][1].record(stream)
-        rt.commit_all()
+        rt.eval_device(n_pts, d_pts[args.warmup + k].data_ptr(), d_res.data_ptr())
     t_end.record(stream)
     rt.synchronize()
     torch.cuda.synchronize()
     elapsed_ms = t_start.elapsed_time(t_end)
-    eval_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    main_ms, main_n = rt.profile_read(_native.KERNEL_SECTION_EVAL)
+    aux_ms, aux_n = rt.profile_read(_native.KERNEL_AUX_EVAL)
     res = d_res.cpu().numpy().view(RESULT_DTYPE)
     assert np.all(res["valid"] == 1), "a section flexibility was singular"
-    ms_per_step = elapsed_ms / args.steps
-    value = fibers * args.steps / (elapsed_ms / 1000.0)
-    alg_bytes = algorithmic_bytes(mats, cells, counts)
-    achieved = alg_bytes / (eval_ms / 1000.0) / 1e9
-    out = {
-        "metric": "fiber updates/s",
-        "value": value,
-        "unit": "fiber updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "section population of the 3D RC frame (20 stories x 5x5 bays): "
-                        f"{pop['columns']} columns + {pop['beams']} beams x 5 IPs = {n_pts} 3D fiber sections, "
-                        f"{fibers} fibers; step = trial evaluation of every section at a seeded cyclic "
-                        "deformation + commit",
-            "sections": n_pts,
-            "fibers": int(fibers),
-            "fibers_per_section": [len(c) for c in cells],
-            "gm_steps_per_s": None,
-        },
+
+    rt.profile_enable(True)
+    for _ in range(args.steps):
+        rt.commit_all()
+    commit_ms, commit_n = rt.profile_read(_native.KERNEL_COMMIT)
+    rt.profile_enable(False)
+    rt.close()
+
+    main_avg = main_ms / main_n
+    aux_avg = aux_ms / aux_n if aux_n else 0.0
+    main_bytes = algorithmic_bytes(mats, cells, counts, aux=False)
+    aux_bytes = algorithmic_bytes(mats, cells, counts, aux=True)
+    achieved = main_bytes / (main_avg / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            tr = json.load(fh)
+        if tr.get("kernel") == "section_eval_kernel" and tr.get("sections") == n_pts:
+            traffic = tr["bytes_per_launch"]
+    return {
+        "value": fibers * args.steps / (elapsed_ms / 1e3),
+        "ms_per_step": elapsed_ms / args.steps,
+        "n_pts": n_pts, "fibers": int(fibers), "pop": pop, "cells": cells,
         "roofline": {
             "bound": "hbm",
             "kernel": "section_eval_kernel",
@@ -178,14 +186,117 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes_per_launch": alg_bytes,
-            "avg_launch_ms": eval_ms,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": main_bytes,
+            "avg_launch_ms": main_avg,
         },
+        "aux": {"kernel": "aux_eval_kernel", "avg_launch_ms": aux_avg, "algorithmic_bytes_per_launch": aux_bytes,
+                "achieved_GBps": aux_bytes / (aux_avg / 1e3) / 1e9 if aux_avg else None},
+        "commit": {"kernel": "copy_state_kernel", "avg_launch_ms": commit_ms / commit_n,
+                   "bytes_per_launch": main_bytes + aux_bytes,
+                   "achieved_GBps": (main_bytes + aux_bytes) / (commit_ms / commit_n / 1e3) / 1e9},
+        "mats": mats,
+    }
+
+
+def gm_leg(n_batch: int, steps: int) -> dict:
+    """GM-steps/s of the 2D frame transient: one record alone, then n_batch
+    records in one launch, same GPU."""
+    from strut_amd import _native, cases
+    from strut_amd.frame2d import load_frame2d_case
+    from strut_amd.transient import FrameBatch
+
+    fc = load_frame2d_case(cases.config0_case(steps=steps, seed=0))
+    motions = [cases.ground_motion(s, fc.dt, steps, 0.4) for s in range(n_batch)]
+    out = {}
+    for label, ms in (("plain", motions[:1]), ("batched", motions)):
+        fb = FrameBatch(fc.materials)
+        fid = fb.add_frame(fc)
+        ids = [fb.add_motion(fid, m, fc.dt, steps) for m in ms]
+        fb.finalize()
+        fb.rt.profile_enable(True)
+        t0 = time.perf_counter()
+        fb.run(steps_per_launch=100)
+        dt = time.perf_counter() - t0
+        kms, kn = fb.rt.profile_read(_native.KERNEL_FRAME2D)
+        fb.rt.profile_enable(False)
+        res = [fb.result(i) for i in ids]
+        done = sum(r["steps_done"] for r in res)
+        iters = sum(int(r["iters"][:r["steps_done"]].sum()) for r in res)
+        out[label] = {"records": len(ms), "gm_steps": int(done), "seconds": dt, "gm_steps_per_s": done / dt,
+                      "kernel_ms": kms, "launches": kn, "newton_iters": iters,
+                      "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0))}
+        fb.close()
+    out["batched_vs_plain"] = out["batched"]["gm_steps_per_s"] / out["plain"]["gm_steps_per_s"]
+    return out
+
+
+def gm_cpu_baseline(steps: int) -> dict:
+    from oracle import oracle
+    from strut_amd import cases
+    from strut_amd.frame2d import load_frame2d_case
+
+    fc = load_frame2d_case(cases.config0_case(steps=steps, seed=0))
+    t0 = time.perf_counter()
+    r = oracle.run_transient2d(fc, record_history=False)
+    dt = time.perf_counter() - t0
+    return {"value": r["steps_done"] / dt, "unit": "GM-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one record (seed 0), {r['steps_done']} steps, {dt:.2f} s, oracle C restatement, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gm-batch", type=int, default=256, help="records in the batched GM-steps leg (0 = skip)")
+    ap.add_argument("--gm-steps", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(ensembles and models shard across GPUs without collectives; see the design notes)",
+              file=sys.stderr)
+        return 2
+
+    sec = section_leg(args)
+    config = {
+        "workload": "section population of the 3D RC frame (20 stories x 5x5 bays): "
+                    f"{sec['pop']['columns']} columns + {sec['pop']['beams']} beams x 5 IPs = {sec['n_pts']} "
+                    f"3D fiber sections, {sec['fibers']} fibers; step = trial evaluation of every section at a "
+                    "new seeded deformation from a committed preload history (Strut's fiber micro-benchmark "
+                    "unit, no commit in the timed steps)",
+        "sections": sec["n_pts"],
+        "fibers": sec["fibers"],
+        "fibers_per_section": [len(c) for c in sec["cells"]],
+        "aux_pass": sec["aux"],
+        "commit": sec["commit"],
+    }
+    if args.gm_batch > 0:
+        gm = gm_leg(args.gm_batch, args.gm_steps)
+        if not args.no_cpu_baseline:
+            gm["cpu_baseline"] = gm_cpu_baseline(args.gm_steps)
+        config["gm_steps"] = {"workload": f"2D RC frame 3 stories x 1 bay, forceBeamColumn 5 IPs, {args.gm_steps} "
+                                          "steps dt 0.01 s, PGA 0.4 g seeded ground motions (seeds 0..N-1)",
+                              **gm}
+    out = {
+        "metric": "fiber updates/s",
+        "value": sec["value"],
+        "unit": "fiber updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": config,
+        "roofline": sec["roofline"],
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(mats, cells, counts)
-    rt.close()
+        out["cpu_baseline"] = cpu_baseline(sec["mats"], sec["cells"])
     print(json.dumps(out))
     return 0
 

@@ -230,6 +230,23 @@ int sgpu_frame2d_result(sgpu_ctx *ctx, int32_t instance, double *u_hist, int32_t
 /* out4 = {free DOFs, band half-width, recorded DOFs, total DOFs}. */
 int sgpu_frame2d_info(sgpu_ctx *ctx, int32_t frame_id, int32_t *out4);
 
+/* ------------------------------------------------------------------
+ * Per-kernel timing.  Mirrors the phase timers of Strut's --profile output
+ * (phase_times_us.json, solver/run_case/helpers.mojo), per device kernel:
+ * while enabled, every launch on the context's stream is bracketed by HIP
+ * events; sgpu_profile_read synchronizes the stream and returns the summed
+ * milliseconds and the launch count of one kernel since the last enable.
+ * ------------------------------------------------------------------ */
+enum {
+    SGPU_KERNEL_AUX_EVAL = 0,     /* aux_eval_kernel: Steel02 / Concrete01 slots   */
+    SGPU_KERNEL_SECTION_EVAL = 1, /* section_eval_kernel: other slots + reduction  */
+    SGPU_KERNEL_COMMIT = 2,       /* commit / revert copies                        */
+    SGPU_KERNEL_FRAME2D = 3,      /* frame2d_kernel                                */
+    SGPU_KERNEL_COUNT = 4
+};
+int sgpu_profile_enable(sgpu_ctx *ctx, int on);
+int sgpu_profile_read(sgpu_ctx *ctx, int kernel, double *total_ms, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

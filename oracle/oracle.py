@@ -55,7 +55,7 @@ def load() -> ctypes.CDLL:
         "or_sec3d_fiber_trial": (c_int, [c_void_p, c_int, c_int, dp]),
         "or_sec3d_centroid": (None, [c_void_p, dp]),
         "or_sec3d_flexibility": (c_int, [dp, dp]),
-        "or_sec3d_eval_commit": (None, [c_void_p, c_int, ip, dp, dp, dp]),
+        "or_sec3d_eval_many": (None, [c_void_p, c_int, ip, dp, dp, dp, c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -188,11 +188,12 @@ class Section3d(_Section):
     def commit(self, inst: int) -> None:
         self.lib.or_sec3d_commit(self.h, inst)
 
-    def eval_commit(self, insts, eps0, ky, kz) -> None:
-        """Evaluate then commit a batch of instances in one C call (timing)."""
+    def eval_many(self, insts, eps0, ky, kz, commit: bool) -> None:
+        """Evaluate a batch of instances in one C call, then commit them if
+        asked (timing)."""
         i = np.ascontiguousarray(insts, dtype=np.int32)
         e, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (eps0, ky, kz))
-        self.lib.or_sec3d_eval_commit(self.h, len(i), _i(i), _d(e), _d(y), _d(z))
+        self.lib.or_sec3d_eval_many(self.h, len(i), _i(i), _d(e), _d(y), _d(z), 1 if commit else 0)
 
     def revert(self, inst: int) -> None:
         self.lib.or_sec3d_revert(self.h, inst)

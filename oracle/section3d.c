@@ -205,9 +205,11 @@ int or_sec3d_flexibility(const double *k6, double *f6) {
 
 /* Timing helper for the CPU baseline: evaluate n points (instances inst[i]
  * at the given deformations) in order, then commit those instances. */
-void or_sec3d_eval_commit(or_sec3d *s, int n, const int *inst, const double *eps0, const double *ky,
-                          const double *kz) {
+/* Timing helper: set_trial on n instances, then commit them when commit != 0. */
+void or_sec3d_eval_many(or_sec3d *s, int n, const int *inst, const double *eps0, const double *ky,
+                        const double *kz, int commit) {
     double out[9];
     for (int i = 0; i < n; ++i) or_sec3d_set_trial(s, inst[i], eps0[i], ky[i], kz[i], out);
-    for (int i = 0; i < n; ++i) or_sec3d_commit(s, inst[i]);
+    if (commit)
+        for (int i = 0; i < n; ++i) or_sec3d_commit(s, inst[i]);
 }

@@ -98,8 +98,9 @@ void or_sec3d_revert(or_sec3d *s, int inst);
 int or_sec3d_fiber_trial(const or_sec3d *s, int inst, int fiber, double *out3);
 void or_sec3d_centroid(const or_sec3d *s, double *yz);
 int or_sec3d_flexibility(const double *k6, double *f6);
-void or_sec3d_eval_commit(or_sec3d *s, int n, const int *inst, const double *eps0, const double *ky,
-                          const double *kz);
+/* timing helper: set_trial on n instances, then commit them if commit != 0 */
+void or_sec3d_eval_many(or_sec3d *s, int n, const int *inst, const double *eps0, const double *ky,
+                        const double *kz, int commit);
 
 #ifdef __cplusplus
 }

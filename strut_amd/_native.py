@@ -54,7 +54,15 @@ EXPORTED = (
     "sgpu_frame2d_run",
     "sgpu_frame2d_result",
     "sgpu_frame2d_info",
+    "sgpu_profile_enable",
+    "sgpu_profile_read",
 )
+
+# Kernel ids of sgpu_profile_read.
+KERNEL_AUX_EVAL = 0
+KERNEL_SECTION_EVAL = 1
+KERNEL_COMMIT = 2
+KERNEL_FRAME2D = 3
 
 
 class SectionPoint(ctypes.Structure):
@@ -118,6 +126,8 @@ def load() -> ctypes.CDLL:
         "sgpu_sections_revert": (c_int, [c_void_p, c_int, ip, ip]),
         "sgpu_fiber_state_read": (c_int, [c_void_p, c_int32, c_int32, c_int, dp]),
         "sgpu_fiber_totals": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int64)]),
+        "sgpu_profile_enable": (c_int, [c_void_p, c_int]),
+        "sgpu_profile_read": (c_int, [c_void_p, c_int, POINTER(c_double), POINTER(c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -72,6 +72,12 @@ struct sgpu_ctx {
     double *d_fsks = nullptr;  // (sigma A, E_t A) of aux slots, per point: the aux pre-pass output
     int64_t fsks_cap = 0;
     int list_cap = 0;
+    // per-kernel timing (sgpu_profile_*): event pairs recorded around each launch
+    bool prof_on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_pending;
+    std::vector<hipEvent_t> prof_free;
+    double prof_ms[SGPU_KERNEL_COUNT] = {};
+    int64_t prof_n[SGPU_KERNEL_COUNT] = {};
     // 2D frame transient instances (frame2d_api.cpp)
     sgpu::FrameHost *frames = nullptr;
     SlotTables tables() const {
@@ -116,6 +122,34 @@ inline int dalloc(sgpu_ctx *ctx, T **p, size_t count) {
     ctx->allocations.push_back((void *)*p);
     return SGPU_OK;
 }
+
+// Record the start (a) / end (b) event of one timed launch when profiling is on.
+inline hipEvent_t prof_event(sgpu_ctx *ctx) {
+    hipEvent_t e = nullptr;
+    if (!ctx->prof_free.empty()) {
+        e = ctx->prof_free.back();
+        ctx->prof_free.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    return e;
+}
+
+struct ProfScope {
+    sgpu_ctx *ctx;
+    int kernel;
+    hipEvent_t a = nullptr;
+    ProfScope(sgpu_ctx *c, int k) : ctx(c), kernel(k) {
+        if (ctx->prof_on && (a = prof_event(ctx)) != nullptr) (void)hipEventRecord(a, ctx->stream);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = prof_event(ctx);
+        if (!b) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->prof_pending.push_back({kernel, {a, b}});
+    }
+};
 
 template <typename T>
 inline int upload(sgpu_ctx *ctx, T **p, const std::vector<T> &v) {

@@ -368,18 +368,22 @@ __global__ __launch_bounds__(256) void copy_instances_kernel(const SecDefDev *__
     }
 }
 
+void launch_aux_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
+                     const MatDev *mats, const StateArrays &c, const StateArrays &t,
+                     const sgpu_section_point *pts, int n, double *fsks, int max_aux) {
+    if (n <= 0 || max_aux <= 0) return;
+    const int per = max_aux < 256 ? max_aux : 256;
+    const int ppb = 256 / per;
+    hipLaunchKernelGGL(aux_eval_kernel, dim3((n + ppb - 1) / ppb), dim3(256), 0, stream, defs, n_defs, tb, mats, c,
+                       t, pts, n, fsks, max_aux, per);
+}
+
 void launch_section_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
                          const MatDev *mats, const StateArrays &c, const StateArrays &t,
-                         const sgpu_section_point *pts, sgpu_section_result *res, int n, double *fsks,
+                         const sgpu_section_point *pts, sgpu_section_result *res, int n, const double *fsks,
                          int max_aux) {
     if (n <= 0) return;
     const int blocks = (n + kPointsPerBlock - 1) / kPointsPerBlock;
-    if (max_aux > 0) {
-        const int per = max_aux < 256 ? max_aux : 256;
-        const int ppb = 256 / per;
-        hipLaunchKernelGGL(aux_eval_kernel, dim3((n + ppb - 1) / ppb), dim3(256), 0, stream, defs, n_defs, tb, mats,
-                           c, t, pts, n, fsks, max_aux, per);
-    }
     hipLaunchKernelGGL(section_eval_kernel, dim3(blocks), dim3(kPointsPerBlock * kBatch), 0, stream, defs, n_defs,
                        tb, mats, c, t, pts, res, n, fsks, max_aux);
 }

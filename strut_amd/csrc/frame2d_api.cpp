@@ -412,7 +412,10 @@ int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t steps_per_launch) {
     FrameHost &fh = frames_of(ctx);
     if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
     if (steps_per_launch <= 0) return fail(ctx, SGPU_ERR_ARG, "steps_per_launch must be > 0");
-    HIP_OK(ctx, launch_frame2d(ctx->stream, fh.args, (int)fh.insts.size(), steps_per_launch, fh.lds));
+    {
+        ProfScope ps(ctx, SGPU_KERNEL_FRAME2D);
+        HIP_OK(ctx, launch_frame2d(ctx->stream, fh.args, (int)fh.insts.size(), steps_per_launch, fh.lds));
+    }
     return SGPU_OK;
 }
 

@@ -17,9 +17,12 @@
 #include "strut_gpu_types.h"
 
 namespace sgpu {
+void launch_aux_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
+                     const MatDev *mats, const StateArrays &c, const StateArrays &t,
+                     const sgpu_section_point *pts, int n, double *fsks, int max_aux);
 void launch_section_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
                          const MatDev *mats, const StateArrays &c, const StateArrays &t,
-                         const sgpu_section_point *pts, sgpu_section_result *res, int n, double *fsks,
+                         const sgpu_section_point *pts, sgpu_section_result *res, int n, const double *fsks,
                          int max_aux);
 void launch_copy_state(hipStream_t stream, const StateArrays &dst, const StateArrays &src, int64_t n_slots,
                        int64_t n_aux);
@@ -306,6 +309,11 @@ int sgpu_ctx_destroy(sgpu_ctx *ctx) {
     if (!ctx) return SGPU_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &pe : ctx->prof_pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    for (hipEvent_t e : ctx->prof_free) (void)hipEventDestroy(e);
     for (void *p : ctx->allocations) (void)hipFree(p);
     if (ctx->d_pts) (void)hipFree(ctx->d_pts);
     if (ctx->d_res) (void)hipFree(ctx->d_res);
@@ -524,8 +532,16 @@ int sgpu_section_eval_device(sgpu_ctx *ctx, int n, const sgpu_section_point *d_p
         HIP_OK(ctx, hipMalloc((void **)&ctx->d_fsks, sizeof(double) * need));
         ctx->fsks_cap = need;
     }
-    launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c,
-                        ctx->t, d_points, d_results, n, ctx->d_fsks, ctx->max_aux);
+    {
+        ProfScope ps(ctx, SGPU_KERNEL_AUX_EVAL);
+        launch_aux_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c, ctx->t,
+                        d_points, n, ctx->d_fsks, ctx->max_aux);
+    }
+    {
+        ProfScope ps(ctx, SGPU_KERNEL_SECTION_EVAL);
+        launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c,
+                            ctx->t, d_points, d_results, n, ctx->d_fsks, ctx->max_aux);
+    }
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;
 }
@@ -552,7 +568,10 @@ int sgpu_section_eval(sgpu_ctx *ctx, int n, const sgpu_section_point *points, sg
 int sgpu_sections_commit_all(sgpu_ctx *ctx) {
     if (!ctx) return SGPU_ERR_ARG;
     if (!ctx->finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_sections_finalize has not run");
-    launch_copy_state(ctx->stream, ctx->c, ctx->t, ctx->total_slots, ctx->total_aux);
+    {
+        ProfScope ps(ctx, SGPU_KERNEL_COMMIT);
+        launch_copy_state(ctx->stream, ctx->c, ctx->t, ctx->total_slots, ctx->total_aux);
+    }
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;
 }
@@ -560,7 +579,10 @@ int sgpu_sections_commit_all(sgpu_ctx *ctx) {
 int sgpu_sections_revert_all(sgpu_ctx *ctx) {
     if (!ctx) return SGPU_ERR_ARG;
     if (!ctx->finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_sections_finalize has not run");
-    launch_copy_state(ctx->stream, ctx->t, ctx->c, ctx->total_slots, ctx->total_aux);
+    {
+        ProfScope ps(ctx, SGPU_KERNEL_COMMIT);
+        launch_copy_state(ctx->stream, ctx->t, ctx->c, ctx->total_slots, ctx->total_aux);
+    }
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;
 }
@@ -579,6 +601,7 @@ static int copy_listed(sgpu_ctx *ctx, int n, const int32_t *def_ids, const int32
     if (rc) return rc;
     HIP_OK(ctx, hipMemcpyAsync(ctx->d_list, def_ids, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(ctx, hipMemcpyAsync(ctx->d_list + n, insts, sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream));
+    ProfScope ps(ctx, SGPU_KERNEL_COMMIT);
     if (commit) launch_copy_instances(ctx->stream, ctx->d_defs, ctx->d_list, ctx->d_list + n, n, ctx->c, ctx->t);
     else launch_copy_instances(ctx->stream, ctx->d_defs, ctx->d_list, ctx->d_list + n, n, ctx->t, ctx->c);
     HIP_OK(ctx, hipGetLastError());
@@ -627,3 +650,37 @@ int sgpu_fiber_totals(sgpu_ctx *ctx, int64_t *fibers, int64_t *slots) {
 }
 
 }  // extern "C"
+
+int sgpu_profile_enable(sgpu_ctx *ctx, int on) {
+    if (!ctx) return SGPU_ERR_ARG;
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &pe : ctx->prof_pending) {
+        ctx->prof_free.push_back(pe.second.first);
+        ctx->prof_free.push_back(pe.second.second);
+    }
+    ctx->prof_pending.clear();
+    for (int k = 0; k < SGPU_KERNEL_COUNT; ++k) {
+        ctx->prof_ms[k] = 0.0;
+        ctx->prof_n[k] = 0;
+    }
+    ctx->prof_on = on != 0;
+    return SGPU_OK;
+}
+
+int sgpu_profile_read(sgpu_ctx *ctx, int kernel, double *total_ms, int64_t *launches) {
+    if (!ctx) return SGPU_ERR_ARG;
+    if (kernel < 0 || kernel >= SGPU_KERNEL_COUNT) return fail(ctx, SGPU_ERR_ARG, "bad kernel id %d", kernel);
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &pe : ctx->prof_pending) {
+        float ms = 0.0f;
+        HIP_OK(ctx, hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+        ctx->prof_ms[pe.first] += ms;
+        ctx->prof_n[pe.first] += 1;
+        ctx->prof_free.push_back(pe.second.first);
+        ctx->prof_free.push_back(pe.second.second);
+    }
+    ctx->prof_pending.clear();
+    if (total_ms) *total_ms = ctx->prof_ms[kernel];
+    if (launches) *launches = ctx->prof_n[kernel];
+    return SGPU_OK;
+}

@@ -105,6 +105,18 @@ class SectionRuntime:
         return {"fibers": int(out[0]), "slots": int(out[1]), "y_bar": out[2], "z_bar": out[3],
                 "initial_flex_valid": bool(out[4]), "initial_flex": tuple(out[5:8])}
 
+    def profile_enable(self, on: bool = True) -> None:
+        """Start (and reset) or stop per-kernel event timing on this context."""
+        self._check(self.lib.sgpu_profile_enable(self.ctx, 1 if on else 0))
+
+    def profile_read(self, kernel: int) -> Tuple[float, int]:
+        """(total milliseconds, launches) of one kernel since profile_enable;
+        kernel ids are strut_amd._native.KERNEL_*."""
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self.lib.sgpu_profile_read(self.ctx, kernel, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def totals(self) -> Tuple[int, int]:
         f, s = c_int64(), c_int64()
         self._check(self.lib.sgpu_fiber_totals(self.ctx, ctypes.byref(f), ctypes.byref(s)))
