@@ -296,4 +296,29 @@ def run_transient2d(fc, record_history: bool = True) -> dict:
     return {"status": status, "u": u_hist, "iters": iters, "steps_done": o.steps_done, "hbw": o.hbw,
             "free_count": o.free_count, "section_evals": o.section_evals, "local_iters": o.local_iters,
             "subdivisions": o.subdivisions, "fallbacks": o.fallbacks, "dense_fallbacks": o.dense_fallbacks,
-            "q_state": q_state, "_sections": secs}
+            "q_state": q_state, "q_offset": q_off, "elem_first_inst": first, "_sections": secs}
+
+
+def band_solve(band: np.ndarray, hbw: int, rhs: np.ndarray):
+    """BandGeneral factor + solve of the oracle (band: n x (2 hbw + 1)).
+    Returns (ok, x, dense_fallback)."""
+    lib = load()
+    lib.or_band_solve.restype = c_int
+    lib.or_band_solve.argtypes = [c_int, c_int, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                  POINTER(c_int)]
+    b = np.ascontiguousarray(band, dtype=np.float64)
+    r = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros(len(r))
+    fb = c_int()
+    ok = lib.or_band_solve(len(r), hbw, _d_out(b), _d_out(r), _d_out(x), ctypes.byref(fb))
+    return bool(ok), x, bool(fb.value)
+
+
+def path_eval(values, dt: float, t: float, factor: float = 1.0, start_time: float = 0.0,
+              use_last: bool = False) -> float:
+    """The oracle's Path time series (time_series.mojo:71-96) at time t."""
+    lib = load()
+    lib.or_path_eval_raw.restype = c_double
+    lib.or_path_eval_raw.argtypes = [c_int, POINTER(c_double), c_double, c_double, c_double, c_int, c_double]
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    return lib.or_path_eval_raw(len(v), _d_out(v), dt, factor, start_time, 1 if use_last else 0, t)

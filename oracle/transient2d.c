@@ -210,6 +210,45 @@ static void band_solve(band_t *b, const double *rhs, double *x) {
     }
 }
 
+/* Test entry: factor and solve one BandGeneral system (n x (2 hbw + 1)
+ * values, row-major as lsb.mojo:377-381).  Returns 1 on success, 0 when a
+ * pivot is below the tolerance; *dense_fallback = 1 when the factorization
+ * switched to the dense partial-pivot LU. */
+int or_band_solve(int n, int hbw, const double *band, const double *rhs, double *x, int *dense_fallback) {
+    band_t b;
+    memset(&b, 0, sizeof(b));
+    b.n = n;
+    b.hbw = hbw;
+    b.w = 2 * hbw + 1;
+    b.band = malloc(sizeof(double) * (size_t)n * b.w);
+    b.dense = calloc((size_t)n * n, sizeof(double));
+    b.piv = calloc((size_t)n, sizeof(int));
+    b.work = calloc((size_t)n, sizeof(double));
+    memcpy(b.band, band, sizeof(double) * (size_t)n * b.w);
+    int ok = band_factor(&b);
+    if (ok) band_solve(&b, rhs, x);
+    if (dense_fallback) *dense_fallback = b.dense_fallback;
+    free(b.band);
+    free(b.dense);
+    free(b.piv);
+    free(b.work);
+    return ok;
+}
+
+/* Test entry: the Path series of a case record evaluated at t. */
+double or_path_eval_raw(int n, const double *values, double dt, double factor, double start_time, int use_last,
+                        double t) {
+    or_tr2d_case c;
+    memset(&c, 0, sizeof(c));
+    c.ts_n = n;
+    c.ts_values = values;
+    c.ts_dt = dt;
+    c.ts_factor = factor;
+    c.ts_start = start_time;
+    c.ts_use_last = use_last;
+    return or_path_eval(&c, t);
+}
+
 /* stiffness_internal_frame2d.mojo:336-634 for forceBeamColumn2d elements */
 static int assemble(const or_tr2d_case *c, or_elem2d *els, double *q, const double *u, double *K, double *F,
                     int total) {

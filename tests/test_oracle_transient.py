@@ -1,0 +1,216 @@
+"""Pins of the oracle's transient layer (CPU only).
+
+Each check fixes the oracle's 2D transient driver against something it does
+not restate:
+- the reference's own time-series test expectations;
+- library routines (numpy dense solves, a partial-pivot LU);
+- an independent Newmark integration of the linear frame;
+- the equilibrium and compatibility that a converged force-beam-column
+  state must satisfy.
+"""
+
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from strut_amd import cases
+from strut_amd.frame2d import load_frame2d_case
+
+
+# ---------------------------------------------------------------- Path series
+def test_path_series_reference_cases():
+    # tests/unit/test_time_series.py:137-143 evaluates Path {dt 0.5, values
+    # [0, 1, 2]} at t = 1.0 through its helper (:49-62): incr2 = 3 >= n, so the
+    # factor is 0 without use_last.
+    v = [0.0, 1.0, 2.0]
+    assert oracle.path_eval(v, 0.5, 1.0) == 0.0
+    assert oracle.path_eval(v, 0.5, 1.0, use_last=True) == 2.0
+    assert oracle.path_eval(v, 0.5, 0.75) == 1.5
+    assert oracle.path_eval(v, 0.5, 0.25) == 0.5
+    assert oracle.path_eval(v, 0.5, 0.0) == 0.0
+    assert oracle.path_eval(v, 0.5, 0.5, factor=2.5) == 2.5
+    # start_time shifts the series; before it the factor is 0
+    assert oracle.path_eval(v, 0.5, 0.9, start_time=1.0) == 0.0
+    assert oracle.path_eval(v, 0.5, 1.25, start_time=1.0) == 0.5
+    assert oracle.path_eval([], 0.5, 0.3) == 0.0
+
+
+# -------------------------------------------------------------- BandGeneral
+def _band_from_dense(A, hbw):
+    n = A.shape[0]
+    band = np.zeros((n, 2 * hbw + 1))
+    for i in range(n):
+        for j in range(max(0, i - hbw), min(n, i + hbw + 1)):
+            band[i, j - i + hbw] = A[i, j]
+    return band
+
+
+def _random_band(n, hbw, rng, dominant=True):
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in range(max(0, i - hbw), min(n, i + hbw + 1)):
+            A[i, j] = rng.standard_normal()
+        if dominant:
+            A[i, i] = np.abs(A[i]).sum() + 1.0
+    return A
+
+
+@pytest.mark.parametrize("n,hbw", [(1, 0), (5, 1), (18, 8), (40, 3), (60, 17)])
+def test_band_lu_without_pivoting_solves_the_system(n, hbw):
+    rng = np.random.default_rng(n * 31 + hbw)
+    A = _random_band(n, hbw, rng)
+    b = rng.standard_normal(n)
+    ok, x, fb = oracle.band_solve(_band_from_dense(A, hbw), hbw, b)
+    assert ok and not fb
+    ref = np.linalg.solve(A, b)
+    assert np.abs(x - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_band_lu_switches_to_dense_partial_pivoting():
+    # a small diagonal forces a row swap: the reference's band LU then
+    # refactors densely with partial pivoting (lsb.mojo:652-657, :175-214)
+    rng = np.random.default_rng(7)
+    n, hbw = 12, 2
+    A = _random_band(n, hbw, rng)
+    A[3, 3] = 1e-3
+    b = rng.standard_normal(n)
+    ok, x, fb = oracle.band_solve(_band_from_dense(A, hbw), hbw, b)
+    assert ok and fb
+    assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-11 * np.abs(x).max()
+
+
+def test_band_lu_reports_a_singular_matrix():
+    n, hbw = 6, 1
+    A = np.eye(n)
+    A[2, :] = 0.0
+    ok, _, _ = oracle.band_solve(_band_from_dense(A, hbw), hbw, np.ones(n))
+    assert not ok
+
+
+# ------------------------------------------------- elastic frame vs Newmark
+def _elastic_variant(case: dict, E: float) -> dict:
+    c = copy.deepcopy(case)
+    c["materials"] = [{"id": m["id"], "type": "Elastic", "params": {"E": E}} for m in case["materials"]]
+    return c
+
+
+def _frame_element_stiffness(EA, EI, x1, y1, x2, y2):
+    """Classic Euler-Bernoulli frame element (no shear, no P-Delta) in global
+    coordinates, written from the textbook local matrix and rotation."""
+    dx, dy = x2 - x1, y2 - y1
+    L = np.hypot(dx, dy)
+    c, s = dx / L, dy / L
+    a, b, d, e, f = EA / L, 12 * EI / L**3, 6 * EI / L**2, 4 * EI / L, 2 * EI / L
+    kl = np.array([[a, 0, 0, -a, 0, 0],
+                   [0, b, d, 0, -b, d],
+                   [0, d, e, 0, -d, f],
+                   [-a, 0, 0, a, 0, 0],
+                   [0, -b, -d, 0, b, -d],
+                   [0, d, f, 0, -d, e]])
+    T = np.zeros((6, 6))
+    for k in (0, 3):
+        T[k:k + 2, k:k + 2] = [[c, s], [-s, c]]
+        T[k + 2, k + 2] = 1.0
+    return T.T @ kl @ T
+
+
+def _newmark_linear(K, M, C, P, dt, gamma, beta):
+    """Average-acceleration Newmark of M a + C v + K u = P(t), from rest,
+    in the effective-stiffness form."""
+    n = K.shape[0]
+    a0 = 1.0 / (beta * dt * dt)
+    a1 = gamma / (beta * dt)
+    a2 = 1.0 / (beta * dt)
+    a3 = 1.0 / (2.0 * beta) - 1.0
+    a4 = gamma / beta - 1.0
+    a5 = dt / 2.0 * (gamma / beta - 2.0)
+    Keff = K + a0 * M + a1 * C
+    u, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+    hist = []
+    for k in range(P.shape[0]):
+        rhs = P[k] + M @ (a0 * u + a2 * v + a3 * a) + C @ (a1 * u + a4 * v + a5 * a)
+        un = np.linalg.solve(Keff, rhs)
+        an = a0 * (un - u) - a2 * v - a3 * a
+        v = v + dt * ((1 - gamma) * a + gamma * an)
+        u, a = un, an
+        hist.append(u.copy())
+    return np.array(hist)
+
+
+def test_elastic_frame_transient_matches_independent_newmark():
+    steps = 300
+    E = 3.0e7
+    case = _elastic_variant(cases.config0_case(steps=steps, seed=3), E)
+    fc = load_frame2d_case(case)
+    r = oracle.run_transient2d(fc)
+    assert r["status"] == 0
+    # Linear problem: one Newton correction; the second assembly passes the test.
+    assert np.all(r["iters"] == 2)
+    # independent assembly of K, M, C and the load history
+    nd = fc.total_dofs
+    K = np.zeros((nd, nd))
+    for e in range(len(fc.elem_nip)):
+        cells = fc.section_cells[int(fc.elem_section[e])]
+        A = cells["area"]
+        ybar = (A * cells["y"]).sum() / A.sum()
+        EA = E * A.sum()
+        EI = E * (A * (cells["y"] - ybar) ** 2).sum()
+        i, j = fc.elem_nodes[e]
+        ke = _frame_element_stiffness(EA, EI, fc.x[i], fc.y[i], fc.x[j], fc.y[j])
+        dofs = [3 * i, 3 * i + 1, 3 * i + 2, 3 * j, 3 * j + 1, 3 * j + 2]
+        K[np.ix_(dofs, dofs)] += ke
+    free = np.nonzero(fc.constrained == 0)[0]
+    Kf = K[np.ix_(free, free)]
+    Mf = np.diag(fc.mass[free])
+    Cf = fc.alpha_m * Mf + (fc.beta_k + fc.beta_k_init) * Kf
+    iota = np.array([1.0 if (d % 3) + 1 == fc.exc_dir else 0.0 for d in free])
+    t = (np.arange(steps) + 1) * fc.dt
+    ag = np.array([oracle.path_eval(fc.ts_values, fc.ts_dt, tk, fc.ts_factor, fc.ts_start, bool(fc.ts_use_last))
+                   for tk in t])
+    P = -(Mf @ iota)[None, :] * ag[:, None]
+    ref = _newmark_linear(Kf, Mf, Cf, P, fc.dt, fc.gamma, fc.beta)
+    got = r["u"][:, free]
+    assert np.abs(got).max() > 1e-4  # the frame actually moves
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+# ------------------------------ converged force-beam-column state invariants
+def test_force_beam_column_state_satisfies_equilibrium_and_compatibility():
+    steps = 150
+    fc = load_frame2d_case(cases.config0_case(steps=steps, seed=0))
+    r = oracle.run_transient2d(fc, record_history=False)
+    assert r["status"] == 0
+    q_state, q_off, first = r["q_state"], r["q_offset"], r["elem_first_inst"]
+    secs = r["_sections"]
+    checked = 0
+    for e in range(len(fc.elem_nip)):
+        nip = int(fc.elem_nip[e])
+        sid = int(fc.elem_section[e])
+        cells = fc.section_cells[sid]
+        A = cells["area"]
+        ybar = secs[sid].y_bar
+        yrel = cells["y"] - ybar
+        q = q_state[q_off[e]:q_off[e] + 3 + 2 * nip + 3]
+        q0, q1, q2 = q[:3]
+        eps0, kappa, vb = q[3:3 + nip], q[3 + nip:3 + 2 * nip], q[3 + 2 * nip:]
+        i, j = fc.elem_nodes[e]
+        L = np.hypot(fc.x[j] - fc.x[i], fc.y[j] - fc.y[i])
+        scale = max(abs(q0), abs(q1), abs(q2))
+        vr = np.zeros(3)
+        for ip in range(nip):
+            xi, w = fc.elem_xi[e, ip], fc.elem_w[e, ip]
+            st = secs[sid].fiber_trial(int(first[e]) + ip)      # (eps, sig, tan) per fiber
+            # fiber kinematics: eps = eps0 - y_rel * kappa
+            assert np.abs(st[:, 0] - (eps0[ip] - yrel * kappa[ip])).max() <= 1e-12 + 1e-9 * np.abs(st[:, 0]).max()
+            N = (st[:, 1] * A).sum()
+            M = -(st[:, 1] * A * yrel).sum()
+            # equilibrium: section forces = b(xi) q  (N = q0, M = (xi - 1) q1 + xi q2)
+            assert abs(N - q0) <= 1e-6 * scale
+            assert abs(M - ((xi - 1.0) * q1 + xi * q2)) <= 1e-6 * scale
+            vr += w * L * np.array([eps0[ip], (xi - 1.0) * kappa[ip], xi * kappa[ip]])
+            checked += 1
+        # compatibility: v = sum_i w_i L b_i^T e_i
+        assert np.abs(vr - vb).max() <= 1e-6 * max(np.abs(vb).max(), 1e-12)
+    assert checked == 5 * len(fc.elem_nip)
