@@ -216,7 +216,7 @@ def gm_leg(n_batch: int, steps: int) -> dict:
         fb.finalize()
         fb.rt.profile_enable(True)
         t0 = time.perf_counter()
-        fb.run(steps_per_launch=100)
+        fb.run(rounds=64)
         dt = time.perf_counter() - t0
         kms, kn = fb.rt.profile_read(_native.KERNEL_FRAME2D)
         fb.rt.profile_enable(False)

@@ -211,12 +211,16 @@ int sgpu_frame2d_add_instance(sgpu_ctx *ctx, int32_t frame_id, int32_t n_steps, 
 /* Freeze frames and instances (finalizes the sections too) and upload. */
 int sgpu_frame2d_finalize(sgpu_ctx *ctx);
 
-/* Advance every unfinished instance by up to steps_per_launch steps
- * (asynchronous), or run all of them to the end (synchronous).  Replaces the
- * step loop of run_transient_nonlinear (transient_nonlinear.mojo:1698-3110)
- * with the Newton loop of :1959-2846. */
-int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t steps_per_launch);
-int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t steps_per_launch);
+/* Advance every unfinished instance by `rounds` rounds (asynchronous): one
+ * round is one Newton iteration of every active instance, whatever step it is
+ * on (the first round of an instance is its set-up assembly); instances that
+ * have finished or stopped are masked out.  The rounds are captured once in a
+ * HIP graph and replayed.  sgpu_frame2d_run repeats this until every instance
+ * has finished (synchronous).  Replaces the step loop of
+ * run_transient_nonlinear (transient_nonlinear.mojo:1698-3110) with its
+ * Newton loop (:1959-2846). */
+int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds);
+int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds);
 
 /* Per-step recorded displacements (n_steps x n_record), per-step Newton
  * iteration counts (assemblies, including the converged one), status

@@ -26,15 +26,6 @@ constexpr int kPointsPerBlock = 4;         // one wavefront per section point
 constexpr int kRedStride = kBatch + 1;     // LDS row stride (odd, so the NQ rows sit in different banks)
 constexpr bool kPrefetch = true;           // load batch b + 1 while batch b is reduced
 
-// Orders the LDS traffic of one wavefront: writes before the barrier are
-// seen by reads after it (LDS requests of one wave complete in order; this
-// only keeps the compiler from moving them).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Section term q of one slot from its fiber force fs = sigma A and stiffness
 // ks = E_t A (fiber2d.mojo:2065-2082, fiber3d.mojo:804-822).
 template <int DIM>

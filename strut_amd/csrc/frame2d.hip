@@ -1,17 +1,24 @@
 // Transient nonlinear analysis of 2D forceBeamColumn fiber frames on gfx950.
 //
-// One workgroup (4 wavefronts) owns one (frame, ground motion) instance and
-// runs its Newmark / Newton loop on the device for a chunk of steps
-// (transient_nonlinear.mojo:1698-3110): element state determination with one
-// wavefront per element (the forceBeamColumn2d local iteration of
-// force_beam_column2d.mojo:804-1919, its fiber sections evaluated wave-wide),
-// an atomic-free gather of the element matrices into the band in element
-// order (the per-entry summation order of the reference's serial scatter,
-// stiffness_internal_frame2d.mojo:594-613), the Rayleigh damping and Newmark
-// residual, the NormUnbalance test, the BandGeneral LU without pivoting
-// (linear_solver_backend.mojo:631-679) and its solve (:735-769) in LDS, then
-// commit.  Every floating-point expression keeps the reference's operation
-// order, so an instance reproduces the CPU restatement's iterates.
+// Every (frame, ground motion) instance runs its own Newmark / Newton loop
+// (transient_nonlinear.mojo:1698-3110) as a state machine advanced by
+// rounds of two kernels; one round = one Newton iteration of every active
+// instance, whatever step each one is on (per-instance convergence masks):
+//  * frame2d_elem_kernel: one workgroup per (instance, element), one
+//    wavefront per integration point.  The forceBeamColumn2d state
+//    determination (force_beam_column2d.mojo:804-1919) runs as uniform
+//    control flow in every wavefront; the fiber sections of its integration
+//    points are evaluated concurrently, one per wavefront.
+//  * frame2d_sys_kernel: one workgroup per instance.  Atomic-free gather of
+//    the element matrices into the band in element order (the per-entry
+//    summation order of the reference's serial scatter,
+//    stiffness_internal_frame2d.mojo:594-613), Rayleigh damping and Newmark
+//    residual, the NormUnbalance test; then either commit and the next
+//    step's predictor, or the BandGeneral LU without pivoting
+//    (linear_solver_backend.mojo:631-679, dense partial-pivot fallback
+//    :175-240) and its solve (:735-769) in LDS.
+// Every floating-point expression keeps the reference's operation order, so
+// an instance reproduces the CPU restatement's iterates.
 #include <hip/hip_runtime.h>
 
 #include "frame2d_types.h"
@@ -20,12 +27,6 @@
 
 namespace sgpu {
 
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // force_beam_column2d.mojo:223-272
 __device__ __forceinline__ bool invert3(double a, double b, double c, double d, double e, double f, double g,
@@ -54,43 +55,73 @@ __device__ __forceinline__ bool invert3(double a, double b, double c, double d, 
     return true;
 }
 
+constexpr int kElemThreads = 64 * kMaxNipDev;  // one wavefront per integration point
+constexpr int kMatsInLds = 16;                   // material tables up to this size are staged in LDS
+
+// Section results of one block-wide evaluation, per integration point:
+// {N, Mz, f00, f01, f11, valid}.
+struct ElemShared {
+    double res[kMaxNipDev][6];
+};
+
 struct ElemCtx {
     const Frame2dArgs *A;
     const ElemDev *el;
     int inst0;  // section instance of integration point 0
+    ElemShared *sh;
+    const MatDev *mats;  // material table (LDS copy when it fits)
+    double *red;         // this wavefront's reduction rows
 };
 
-// fiber_section2d_batch_eval_same_def (fiber2d.mojo:3042-3086): points in
-// order, stop at the first singular flexibility.
-__device__ bool eval_sections(const ElemCtx &E, const double *pe, const double *pk, double *ax, double *mo,
-                              double *f00, double *f01, double *f11) {
+// fiber_section2d_batch_eval_same_def (fiber2d.mojo:3042-3086) for the
+// element's integration points: wavefront w evaluates point w, all points at
+// once.  The reference stops at the first singular point; every caller
+// treats a singular point as a failed attempt and re-evaluates every section
+// before the trial states are used again, so evaluating the later points as
+// well changes no outcome.  Called by every thread of the block.
+__device__ bool eval_sections(const ElemCtx &E, const double *pe, const double *pk) {
     const SecDefDev &d = E.A->defs[E.el->def];
-#pragma unroll
-    for (int ip = 0; ip < kMaxNipDev; ++ip) {
-        if (ip >= E.el->nip) break;
-        const Sec2dOut r =
-            wave_section_2d(d, E.A->tb, E.A->mats, E.A->c, E.A->t, E.inst0 + ip, pe[ip], pk[ip]);
-        ax[ip] = r.N;
-        mo[ip] = r.Mz;
-        const double det = r.k11 * r.k22 - r.k12 * r.k12;
-        if (fabs(det) <= 1.0e-40) return false;
-        const double inv_det = 1.0 / det;
-        f00[ip] = r.k22 * inv_det;
-        f01[ip] = -r.k12 * inv_det;
-        f11[ip] = r.k11 * inv_det;
+    const int nip = E.el->nip;
+    const int w = threadIdx.x >> 6;
+    __syncthreads();  // every wavefront has read the previous results
+    if (w < nip) {
+        const Sec2dOut r = wave_section_2d(d, E.A->tb, E.mats, E.A->c, E.A->t, E.inst0 + w, pe[w], pk[w], E.red);
+        if ((threadIdx.x & 63) == 0) {
+            const double det = r.k11 * r.k22 - r.k12 * r.k12;
+            double *o = E.sh->res[w];
+            o[0] = r.N;
+            o[1] = r.Mz;
+            if (fabs(det) <= 1.0e-40) {
+                o[5] = 0.0;
+            } else {
+                const double inv_det = 1.0 / det;
+                o[2] = r.k22 * inv_det;
+                o[3] = -r.k12 * inv_det;
+                o[4] = r.k11 * inv_det;
+                o[5] = 1.0;
+            }
+        }
     }
-    return true;
+    __syncthreads();
+    bool ok = true;
+    for (int ip = 0; ip < nip; ++ip)
+        if (E.sh->res[ip][5] == 0.0) ok = false;
+    return ok;
 }
 
+// Per-wavefront element scratch, kept in LDS so that only scalars stay in
+// registers across the section evaluations.
 struct FbcScratch {
     double vs_e[kMaxNipDev], vs_k[kMaxNipDev];
     double sub_e[kMaxNipDev], sub_k[kMaxNipDev];
     double sub_ax[kMaxNipDev], sub_mo[kMaxNipDev];
     double sub_f00[kMaxNipDev], sub_f01[kMaxNipDev], sub_f11[kMaxNipDev];
+    double pe[kMaxNipDev], pk[kMaxNipDev];  // deformations of the current evaluation
+    double best[10], solved[10];
 };
 
 // force_beam_column2d.mojo:804-1189 (try_increment); best = {q0,q1,q2,k00,k01,k02,k10,k11,k12,k22}
-__device__ __noinline__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, double bv0, double bv1, double bv2,
+__device__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, double bv0, double bv1, double bv2,
                               double dv0, double dv1, double dv2, int scheme, double *best) {
     const ElemDev &el = *E.el;
     const int nip = el.nip;
@@ -98,41 +129,34 @@ __device__ __noinline__ bool try_increment(const ElemCtx &E, double *q, FbcScrat
     double q0 = q[0], q1 = q[1], q2 = q[2];
     const double tv0 = bv0 + dv0, tv1 = bv1 + dv1, tv2 = bv2 + dv2;
     if (!el.if_valid) return false;
-    double pe[kMaxNipDev], pk[kMaxNipDev], ax[kMaxNipDev], mo[kMaxNipDev], f00[kMaxNipDev], f01[kMaxNipDev],
-        f11[kMaxNipDev];
-#pragma unroll
-    for (int ip = 0; ip < kMaxNipDev; ++ip) {
-        if (ip < nip) {
-            pe[ip] = q[3 + ip];
-            pk[ip] = q[3 + nip + ip];
-            sc.vs_e[ip] = pe[ip];
-            sc.vs_k[ip] = pk[ip];
-        }
+    for (int ip = 0; ip < nip; ++ip) {
+        sc.pe[ip] = q[3 + ip];
+        sc.pk[ip] = q[3 + nip + ip];
+        sc.vs_e[ip] = sc.pe[ip];
+        sc.vs_k[ip] = sc.pk[ip];
     }
-    if (!eval_sections(E, pe, pk, ax, mo, f00, f01, f11)) return false;
+    if (!eval_sections(E, sc.pe, sc.pk)) return false;
     double pf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ip = 0; ip < kMaxNipDev; ++ip) {
-        if (ip < nip) {
-            sc.sub_e[ip] = sc.vs_e[ip];
-            sc.sub_k[ip] = sc.vs_k[ip];
-            sc.sub_ax[ip] = ax[ip];
-            sc.sub_mo[ip] = mo[ip];
-            sc.sub_f00[ip] = f00[ip];
-            sc.sub_f01[ip] = f01[ip];
-            sc.sub_f11[ip] = f11[ip];
-            const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
-            const double a00 = f00[ip], a01 = f01[ip], a10 = a01, a11 = f11[ip];
-            pf[0] += wL * a00;
-            pf[1] += wL * a01 * b_mi;
-            pf[2] += wL * a01 * b_mj;
-            pf[3] += wL * b_mi * a10;
-            pf[4] += wL * b_mi * a11 * b_mi;
-            pf[5] += wL * b_mi * a11 * b_mj;
-            pf[6] += wL * b_mj * a10;
-            pf[7] += wL * b_mj * a11 * b_mi;
-            pf[8] += wL * b_mj * a11 * b_mj;
-        }
+    for (int ip = 0; ip < nip; ++ip) {
+        const double *r = E.sh->res[ip];
+        sc.sub_e[ip] = sc.vs_e[ip];
+        sc.sub_k[ip] = sc.vs_k[ip];
+        sc.sub_ax[ip] = r[0];
+        sc.sub_mo[ip] = r[1];
+        sc.sub_f00[ip] = r[2];
+        sc.sub_f01[ip] = r[3];
+        sc.sub_f11[ip] = r[4];
+        const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+        const double a00 = r[2], a01 = r[3], a10 = a01, a11 = r[4];
+        pf[0] += wL * a00;
+        pf[1] += wL * a01 * b_mi;
+        pf[2] += wL * a01 * b_mj;
+        pf[3] += wL * b_mi * a10;
+        pf[4] += wL * b_mi * a11 * b_mi;
+        pf[5] += wL * b_mi * a11 * b_mj;
+        pf[6] += wL * b_mj * a10;
+        pf[7] += wL * b_mj * a11 * b_mi;
+        pf[8] += wL * b_mj * a11 * b_mj;
     }
     double pk3[9];
     if (invert3(pf[0], pf[1], pf[2], pf[3], pf[4], pf[5], pf[6], pf[7], pf[8], pk3)) {
@@ -143,57 +167,53 @@ __device__ __noinline__ bool try_increment(const ElemCtx &E, double *q, FbcScrat
     const int n_iters = scheme == 1 ? 100 : 10;
     for (int it = 0; it < n_iters; ++it) {
         const bool use_init = scheme == 1 || (scheme == 2 && it == 0);
-#pragma unroll
-        for (int ip = 0; ip < kMaxNipDev; ++ip) {
-            if (ip < nip) {
-                const double xi = el.xi[ip], b_mi = xi - 1.0, b_mj = xi;
-                const double ss0 = q0 + 0.0;
-                const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
-                const double dss0 = ss0 - sc.sub_ax[ip];
-                const double dss1 = ss1 - sc.sub_mo[ip];
-                const double s00 = use_init ? el.if00 : sc.sub_f00[ip];
-                const double s01 = use_init ? el.if01 : sc.sub_f01[ip];
-                const double s11 = use_init ? el.if11 : sc.sub_f11[ip];
-                const double dvs0 = s00 * dss0 + s01 * dss1;
-                const double dvs1 = s01 * dss0 + s11 * dss1;
-                pe[ip] = sc.sub_e[ip] + dvs0;
-                pk[ip] = sc.sub_k[ip] + dvs1;
-            }
+        for (int ip = 0; ip < nip; ++ip) {
+            const double xi = el.xi[ip], b_mi = xi - 1.0, b_mj = xi;
+            const double ss0 = q0 + 0.0;
+            const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
+            const double dss0 = ss0 - sc.sub_ax[ip];
+            const double dss1 = ss1 - sc.sub_mo[ip];
+            const double s00 = use_init ? el.if00 : sc.sub_f00[ip];
+            const double s01 = use_init ? el.if01 : sc.sub_f01[ip];
+            const double s11 = use_init ? el.if11 : sc.sub_f11[ip];
+            const double dvs0 = s00 * dss0 + s01 * dss1;
+            const double dvs1 = s01 * dss0 + s11 * dss1;
+            sc.pe[ip] = sc.sub_e[ip] + dvs0;
+            sc.pk[ip] = sc.sub_k[ip] + dvs1;
         }
-        if (!eval_sections(E, pe, pk, ax, mo, f00, f01, f11)) return false;
+        if (!eval_sections(E, sc.pe, sc.pk)) return false;
         double f[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         double vr0 = 0.0, vr1 = 0.0, vr2 = 0.0;
-#pragma unroll
-        for (int ip = 0; ip < kMaxNipDev; ++ip) {
-            if (ip < nip) {
-                const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
-                const double ss0 = q0 + 0.0;
-                const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
-                sc.sub_e[ip] = pe[ip];
-                sc.sub_k[ip] = pk[ip];
-                sc.sub_ax[ip] = ax[ip];
-                sc.sub_mo[ip] = mo[ip];
-                const double a00 = f00[ip], a01 = f01[ip], a11 = f11[ip], a10 = a01;
-                sc.sub_f00[ip] = a00;
-                sc.sub_f01[ip] = a01;
-                sc.sub_f11[ip] = a11;
-                const double dr0 = ss0 - ax[ip];
-                const double dr1 = ss1 - mo[ip];
-                const double dv_0 = a00 * dr0 + a01 * dr1;
-                const double dv_1 = a01 * dr0 + a11 * dr1;
-                f[0] += wL * a00;
-                f[1] += wL * a01 * b_mi;
-                f[2] += wL * a01 * b_mj;
-                f[3] += wL * b_mi * a10;
-                f[4] += wL * b_mi * a11 * b_mi;
-                f[5] += wL * b_mi * a11 * b_mj;
-                f[6] += wL * b_mj * a10;
-                f[7] += wL * b_mj * a11 * b_mi;
-                f[8] += wL * b_mj * a11 * b_mj;
-                vr0 += wL * (pe[ip] + dv_0);
-                vr1 += wL * b_mi * (pk[ip] + dv_1);
-                vr2 += wL * b_mj * (pk[ip] + dv_1);
-            }
+        for (int ip = 0; ip < nip; ++ip) {
+            const double *r = E.sh->res[ip];
+            const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+            const double ss0 = q0 + 0.0;
+            const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
+            const double pe = sc.pe[ip], pk = sc.pk[ip];
+            sc.sub_e[ip] = pe;
+            sc.sub_k[ip] = pk;
+            sc.sub_ax[ip] = r[0];
+            sc.sub_mo[ip] = r[1];
+            const double a00 = r[2], a01 = r[3], a11 = r[4], a10 = a01;
+            sc.sub_f00[ip] = a00;
+            sc.sub_f01[ip] = a01;
+            sc.sub_f11[ip] = a11;
+            const double dr0 = ss0 - r[0];
+            const double dr1 = ss1 - r[1];
+            const double dv_0 = a00 * dr0 + a01 * dr1;
+            const double dv_1 = a01 * dr0 + a11 * dr1;
+            f[0] += wL * a00;
+            f[1] += wL * a01 * b_mi;
+            f[2] += wL * a01 * b_mj;
+            f[3] += wL * b_mi * a10;
+            f[4] += wL * b_mi * a11 * b_mi;
+            f[5] += wL * b_mi * a11 * b_mj;
+            f[6] += wL * b_mj * a10;
+            f[7] += wL * b_mj * a11 * b_mi;
+            f[8] += wL * b_mj * a11 * b_mj;
+            vr0 += wL * (pe + dv_0);
+            vr1 += wL * b_mi * (pk + dv_1);
+            vr2 += wL * b_mj * (pk + dv_1);
         }
         double k[9];
         if (!invert3(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], k)) return false;
@@ -209,14 +229,11 @@ __device__ __noinline__ bool try_increment(const ElemCtx &E, double *q, FbcScrat
             q[0] = q0;
             q[1] = q1;
             q[2] = q2;
-#pragma unroll
-            for (int ip = 0; ip < kMaxNipDev; ++ip) {
-                if (ip < nip) {
-                    q[3 + ip] = sc.sub_e[ip];
-                    q[3 + nip + ip] = sc.sub_k[ip];
-                    sc.vs_e[ip] = sc.sub_e[ip];
-                    sc.vs_k[ip] = sc.sub_k[ip];
-                }
+            for (int ip = 0; ip < nip; ++ip) {
+                q[3 + ip] = sc.sub_e[ip];
+                q[3 + nip + ip] = sc.sub_k[ip];
+                sc.vs_e[ip] = sc.sub_e[ip];
+                sc.vs_k[ip] = sc.sub_k[ip];
             }
             best[0] = q0;
             best[1] = q1;
@@ -296,10 +313,15 @@ __device__ __forceinline__ double kglobal(const double *kb, double inv_L, double
 // force_beam_column2d.mojo:1418-1919 for one element, executed by one whole
 // wavefront; lanes 0..35 store k_global, lanes 36..41 f_global.  Returns
 // false when the reference would abort (no recoverable tangent).
-__device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q,
-                              const double *uf, double *out42, const FrameDev &F) {
-    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e]};
+__device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q_global,
+                              const double *uf, double *out42, ElemShared *sh, double *q, FbcScratch &sc,
+                              const MatDev *mats, double *red) {
+    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e], sh, mats, red};
     const int nip = el.nip;
+    const int q_len = 3 + 2 * nip + 3;
+    // the wavefront's private copy of the element state (uniform in all wavefronts)
+    for (int k = threadIdx.x & 63; k < q_len; k += 64) q[k] = q_global[k];
+    wave_sync();
     const double L = el.L, inv_L = el.inv_L, c = el.c, s = el.s;
     double ug[6];
 #pragma unroll
@@ -316,7 +338,6 @@ __device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &
     const double v0 = ul[3] - ul[0], v1 = ul[2] - chord, v2 = ul[5] - chord;
     double ab0 = q[boff], ab1 = q[boff + 1], ab2 = q[boff + 2];
     double aq0 = q[0], aq1 = q[1], aq2 = q[2];
-    FbcScratch sc;
 #pragma unroll
     for (int ip = 0; ip < kMaxNipDev; ++ip) {
         sc.vs_e[ip] = ip < nip ? q[3 + ip] : 0.0;
@@ -325,7 +346,7 @@ __device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &
     double rem0 = v0 - ab0, rem1 = v1 - ab1, rem2 = v2 - ab2;
     double at0 = rem0, at1 = rem1, at2 = rem2;
     bool converged = false, have_best = false;
-    double best[10];
+    double *best = sc.best;
     int num_sub = 1;
     int guard = 0;  // bounds the subdivision loop (the reference's has no bound)
     if (max_abs3(rem0, rem1, rem2) <= 1.0e-16) {
@@ -349,7 +370,7 @@ __device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &
                 q[boff] = ab0;
                 q[boff + 1] = ab1;
                 q[boff + 2] = ab2;
-                double solved[10];
+                double *solved = sc.solved;
                 if (!try_increment(E, q, sc, ab0, ab1, ab2, at0, at1, at2, scheme, solved)) continue;
                 ab0 = tv0;
                 ab1 = tv1;
@@ -397,9 +418,7 @@ __device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &
         q[boff] = ab0;
         q[boff + 1] = ab1;
         q[boff + 2] = ab2;
-        const SecDefDev &d = A.defs[el.def];
-        for (int ip = 0; ip < nip; ++ip)
-            (void)wave_section_2d(d, A.tb, A.mats, A.c, A.t, E.inst0 + ip, sc.vs_e[ip], sc.vs_k[ip]);
+        (void)eval_sections(E, sc.vs_e, sc.vs_k);
     }
     if (!have_best) {
         // force_beam_column2d.mojo:735-801 (initial basic tangent)
@@ -432,6 +451,9 @@ __device__ __noinline__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &
     }
     const double kb[7] = {best[3], best[4], best[5], best[6], best[7], best[8], best[9]};
     const int lane = threadIdx.x & 63;
+    if (threadIdx.x >= 64) return true;  // wavefront 0 writes the outputs
+    wave_sync();
+    for (int k = lane; k < q_len; k += 64) q_global[k] = q[k];
     if (lane < 36) {
         out42[lane] = kglobal(kb, inv_L, c, s, lane / 6, lane % 6);
     } else if (lane < 42) {
@@ -489,21 +511,39 @@ __device__ void copy_instance_state(const Frame2dArgs &A, const InstDev &I, bool
     }
 }
 
-// All element states of the instance, then the raw K_ff band (gather in
-// element order) into `band` and F_int at the free DOFs into `fint`.
-// Returns false (block-uniform) when an element failed.
-__device__ bool assemble(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, const double *uf, double *band,
-                         double *fint, int *flag) {
-    const int wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
-    double *out = A.scratch + I.scratch_base;
-    for (int e = wave; e < F.n_elem; e += n_waves) {
-        const ElemDev &el = A.elems[F.elem_base + e];
-        const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, uf, out + 42 * e, F);
-        if (!ok && (threadIdx.x & 63) == 0) *flag = 1;
+// Element state determination of one (instance, element) pair per block.
+__global__ __launch_bounds__(kElemThreads) void frame2d_elem_kernel(Frame2dArgs A) {
+    __shared__ ElemShared sh;
+    __shared__ double qbuf[kMaxNipDev][3 + 2 * kMaxNipDev + 3];
+    __shared__ FbcScratch scr[kMaxNipDev];
+    __shared__ double wred[kMaxNipDev][5 * kWaveRedStride];
+    __shared__ MatDev smats[kMatsInLds];
+    const int p = blockIdx.x;
+    const int inst = A.pair_inst[p], e = A.pair_elem[p];
+    const InstState st = A.state[inst];
+    const InstDev &I = A.insts[inst];
+    if (st.status != 0 || st.step >= I.n_steps) return;  // finished or stopped (block-uniform)
+    const FrameDev &F = A.frames[I.frame];
+    const ElemDev &el = A.elems[F.elem_base + e];
+    const int w = threadIdx.x >> 6;
+    const MatDev *mats = A.mats;
+    if (A.n_mats <= kMatsInLds) {  // material parameters through LDS
+        const double *src = reinterpret_cast<const double *>(A.mats);
+        double *dst = reinterpret_cast<double *>(smats);
+        const int nd = A.n_mats * (int)(sizeof(MatDev) / sizeof(double));
+        for (int k = threadIdx.x; k < nd; k += blockDim.x) dst[k] = src[k];
+        __syncthreads();
+        mats = smats;
     }
-    __syncthreads();
-    if (*flag) return false;
-    const int n = F.n_free, bw = F.bw, hbw = F.hbw;
+    const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
+                                  A.scratch + I.scratch_base + 42 * e, &sh, qbuf[w], scr[w], mats, wred[w]);
+    if (!ok && threadIdx.x == 0) A.eflag[inst] = 1;
+}
+
+// K_ff band (gather in element order) into `band`, F_int into `fint`.
+__device__ void gather(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, double *band, double *fint) {
+    const double *out = A.scratch + I.scratch_base;
+    const int n = F.n_free, bw = F.bw;
     const int32_t *off = A.kg_off + F.kg_off_base;
     for (int sl = threadIdx.x; sl < n * bw; sl += blockDim.x) {
         double v = 0.0;
@@ -516,337 +556,136 @@ __device__ bool assemble(const Frame2dArgs &A, const InstDev &I, const FrameDev 
         for (int k = foff[i]; k < foff[i + 1]; ++k) v += out[A.fg[k]];
         fint[i] = v;
     }
-    (void)hbw;
-    __syncthreads();
-    return true;
 }
 
-__global__ __launch_bounds__(256) void frame2d_kernel(Frame2dArgs A, int steps_per_launch) {
+// Newmark predictor of a step (transient_nonlinear.mojo:1757-1766, :553-590)
+// and its external load (:1823-1852).
+__device__ void start_step(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, int step, double *uf,
+                           double *vf, double *af, double *un, double *vn, double *an, double *P) {
+    const int n = F.n_free;
+    const double ag = A.ag[I.ag_base + step];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        un[i] = uf[i];
+        vn[i] = vf[i];
+        an[i] = af[i];
+        uf[i] = un[i] + F.dt * vn[i] + F.coeff * an[i];
+        double load = 0.0;
+        if (A.exc_flag[F.free_base + i]) load += -A.exc_ff[F.free_base + i] * ag;
+        double fe = 0.0;
+        fe += load;
+        P[i] = fe;
+    }
+}
+
+// One Newton iteration of one instance per block (after the element kernel).
+__global__ __launch_bounds__(256) void frame2d_sys_kernel(Frame2dArgs A) {
     extern __shared__ double lds[];
-    __shared__ int flag;
     __shared__ double s_norm, s_diag;
     __shared__ int s_stat;
     const int bid = blockIdx.x;
     const InstDev I = A.insts[bid];
-    const FrameDev F = A.frames[I.frame];
     InstState st = A.state[bid];
     if (st.status != 0 || st.step >= I.n_steps) return;
+    const FrameDev F = A.frames[I.frame];
     const int n = F.n_free, bw = F.bw, hbw = F.hbw;
     const int tid = threadIdx.x, nt = blockDim.x;
-    double *uf = lds, *vf = uf + n, *af = vf + n, *un = af + n, *vn = un + n, *an = vn + n;
-    double *P = an + n, *R = P + n, *du = R + n, *vt = du + n, *at = vt + n, *fint = at + n, *work = fint + n;
+    double *gv = A.vec + I.vec_base;
+    double *uf = lds, *vf = uf + n, *af = vf + n, *un = af + n, *vn = un + n, *an = vn + n, *P = an + n;
+    double *R = P + n, *du = R + n, *vt = du + n, *at = vt + n, *fint = at + n, *work = fint + n;
     double *band = work + n;  // n * bw
     const double *mass = A.mass_ff + F.free_base;
-    double *vec = A.vec + I.vec_base;
-    for (int i = tid; i < n; i += nt) {
-        uf[i] = vec[i];
-        vf[i] = vec[n + i];
-        af[i] = vec[2 * n + i];
+    if (A.eflag[bid]) {  // an element state failed (the reference aborts)
+        if (tid == 0) {
+            st.status = kInstElementFail;
+            A.state[bid] = st;
+        }
+        return;
     }
-    if (tid == 0) flag = 0;
+    for (int i = tid; i < 7 * n; i += nt) uf[i] = gv[i];
+    gather(A, I, F, band, fint);
     __syncthreads();
     if (!st.setup_done) {
         // initial assembly at u (transient_nonlinear.mojo:1207), K_init (:1329-1337), revert (:1452-1453)
-        if (!assemble(A, I, F, uf, band, fint, &flag)) {
-            if (tid == 0) A.state[bid].status = kInstElementFail;
-            return;
-        }
         double *ki = A.kinit + I.kinit_base;
         for (int sl = tid; sl < n * bw; sl += nt) ki[sl] = band[sl];
         copy_instance_state(A, I, false);
+        start_step(A, I, F, 0, uf, vf, af, un, vn, an, P);
         __syncthreads();
-        st.setup_done = 1;
+        for (int i = tid; i < 7 * n; i += nt) gv[i] = uf[i];
+        if (tid == 0) {
+            st.setup_done = 1;
+            st.iter = 0;
+            A.state[bid] = st;
+        }
+        return;
     }
     const double *kinit = A.kinit + I.kinit_base;
-    int steps_run = 0;
-    while (steps_run < steps_per_launch && st.step < I.n_steps) {
-        const int step = st.step;
-        for (int i = tid; i < n; i += nt) {  // :1757-1766, :553-590
-            un[i] = uf[i];
-            vn[i] = vf[i];
-            an[i] = af[i];
-            uf[i] = un[i] + F.dt * vn[i] + F.coeff * an[i];
-        }
-        const double ag = A.ag[I.ag_base + step];
-        for (int i = tid; i < n; i += nt) {  // :1823-1852
-            double load = 0.0;
-            if (A.exc_flag[F.free_base + i]) load += -A.exc_ff[F.free_base + i] * ag;
-            double fe = 0.0;
-            fe += load;
-            P[i] = fe;
-        }
-        __syncthreads();
-        bool converged = false;
-        int iters = 0;
-        for (int it = 0; it < F.max_iters; ++it) {
-            ++iters;
-            if (!assemble(A, I, F, uf, band, fint, &flag)) {
-                if (tid == 0) A.state[bid].status = kInstElementFail;
-                return;
-            }
-            for (int i = tid; i < n; i += nt) {  // :496-531
-                const double anext = F.a0 * (uf[i] - un[i]) - F.a2 * vn[i] - F.a3 * an[i];
-                at[i] = anext;
-                vt[i] = vn[i] + F.dt * (F.omg * an[i] + F.gamma * anext);
-            }
-            __syncthreads();
-            // residual (:2337-2356): damping row dot in W-chunks over the band, inertia M_ii a_i
-            for (int i = tid; i < n; i += nt) {
-                double damping = 0.0;
-                if (F.has_damping) {
-                    const int full = (n / 8) * 8;
-                    int j0 = i - hbw < 0 ? 0 : i - hbw;
-                    int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
-                    double sum = 0.0;
-                    for (int cb = (j0 / 8) * 8; cb < full && cb <= j1; cb += 8) {
-                        double v[8];
+    const int step = st.step;
+    st.iter += 1;  // this assembly
+    for (int i = tid; i < n; i += nt) {  // :496-531
+        const double anext = F.a0 * (uf[i] - un[i]) - F.a2 * vn[i] - F.a3 * an[i];
+        at[i] = anext;
+        vt[i] = vn[i] + F.dt * (F.omg * an[i] + F.gamma * anext);
+    }
+    __syncthreads();
+    // residual (:2337-2356): damping row dot in W-chunks over the band, inertia M_ii a_i
+    for (int i = tid; i < n; i += nt) {
+        double damping = 0.0;
+        if (F.has_damping) {
+            const int full = (n / 8) * 8;
+            int j0 = i - hbw < 0 ? 0 : i - hbw;
+            int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+            double sum = 0.0;
+            for (int cb = (j0 / 8) * 8; cb < full && cb <= j1; cb += 8) {
+                double v[8];
 #pragma unroll
-                        for (int l = 0; l < 8; ++l) {
-                            const int j = cb + l;
-                            double cij = 0.0;
-                            if (j >= j0 && j <= j1) {
-                                const int sl = i * bw + (j - i + hbw);
-                                const double mij = i == j ? mass[i] : 0.0;
-                                double cc = 0.0;
-                                if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
-                                if (F.beta_k != 0.0) cc += F.beta_k * band[sl];
-                                if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
-                                cij = cc;
-                            }
-                            v[l] = (cij * vt[j]) * 1.0;
-                        }
-                        sum += tree8_seq(v);
-                    }
-                    for (int j = full > j0 ? full : j0; j <= j1; ++j) {
+                for (int l = 0; l < 8; ++l) {
+                    const int j = cb + l;
+                    double cij = 0.0;
+                    if (j >= j0 && j <= j1) {
                         const int sl = i * bw + (j - i + hbw);
                         const double mij = i == j ? mass[i] : 0.0;
                         double cc = 0.0;
                         if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
                         if (F.beta_k != 0.0) cc += F.beta_k * band[sl];
                         if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
-                        sum += (cc * vt[j]) * 1.0;
+                        cij = cc;
                     }
-                    damping = sum;
+                    v[l] = (cij * vt[j]) * 1.0;
                 }
-                const double inertia = (mass[i] * at[i]) * 1.0;
-                R[i] = P[i] - fint[i] - 0.0 - damping - inertia;
+                sum += tree8_seq(v);
             }
-            __syncthreads();
-            if (tid == 0) {  // :2357-2371
-                const int full = (n / 8) * 8;
-                double sum = 0.0;
-                for (int cb = 0; cb < full; cb += 8) {
-                    double v[8];
-                    for (int l = 0; l < 8; ++l) v[l] = (R[cb + l] * R[cb + l]) * 1.0;
-                    sum += tree8_seq(v);
-                }
-                for (int j = full; j < n; ++j) sum += (R[j] * R[j]) * 1.0;
-                s_norm = sqrt(sum);
-            }
-            __syncthreads();
-            if (s_norm <= F.tol) {
-                converged = true;
-                break;
-            }
-            // K_eff = K_ff + a1 C + a0 M on the band (:2400-2411, lsb.mojo:1877-1981)
-            for (int sl = tid; sl < n * bw; sl += nt) {
-                const int i = sl / bw, j = i + (sl % bw) - hbw;
-                if (j < 0 || j >= n) {
-                    band[sl] = 0.0;
-                    continue;
-                }
-                const double kb = band[sl];
+            for (int j = full > j0 ? full : j0; j <= j1; ++j) {
+                const int sl = i * bw + (j - i + hbw);
                 const double mij = i == j ? mass[i] : 0.0;
-                double v = kb;
-                if (F.has_damping) {
-                    double cc = 0.0;
-                    if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
-                    if (F.beta_k != 0.0) cc += F.beta_k * kb;
-                    if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
-                    v += F.a1 * cc;
-                }
-                v += F.a0 * mij;
-                band[sl] = v;
+                double cc = 0.0;
+                if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
+                if (F.beta_k != 0.0) cc += F.beta_k * band[sl];
+                if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
+                sum += (cc * vt[j]) * 1.0;
             }
-            __syncthreads();
-            // BandGeneral LU without pivoting (lsb.mojo:631-679); the band as
-            // loaded is kept for the dense fallback of :652-657.
-            double *dense = A.dense + I.dense_base;
-            double *orig = dense + (int64_t)n * n;
-            for (int sl = tid; sl < n * bw; sl += nt) orig[sl] = band[sl];
-            int32_t *piv = A.piv + I.piv_base;
-            bool dense_mode = false;
-            for (int k = 0; k < n; ++k) {
-                const int i1 = k + hbw > n - 1 ? n - 1 : k + hbw;
-                if (k > 0) {  // store the multipliers of column k - 1
-                    const int kp = k - 1;
-                    const int ip1 = kp + hbw > n - 1 ? n - 1 : kp + hbw;
-                    for (int i = kp + 1 + tid; i <= ip1; i += nt) {
-                        const int sl = i * bw + (kp - i + hbw);
-                        band[sl] = band[sl] / s_diag;
-                    }
-                }
-                if (tid == 0) {
-                    int prow = k;
-                    double pabs = 0.0;
-                    for (int row = k; row <= i1; ++row) {
-                        const double cand = fabs(band[row * bw + (k - row + hbw)]);
-                        if (cand > pabs) {
-                            pabs = cand;
-                            prow = row;
-                        }
-                    }
-                    s_stat = 0;
-                    if (pabs <= 1.0e-14) s_stat = kInstSingular;
-                    else if (prow != k) s_stat = kInstBandPivot;
-                    else {
-                        const double diag = band[k * bw + hbw];
-                        if (fabs(diag) <= 1.0e-14) s_stat = kInstSingular;
-                    }
-                }
-                __syncthreads();
-                if (s_stat == kInstBandPivot) {
-                    dense_mode = true;
-                    break;
-                }
-                if (s_stat != 0) {
-                    if (tid == 0) A.state[bid].status = s_stat;
-                    return;
-                }
-                const double diag = band[k * bw + hbw];
-                const int j1 = k + hbw > n - 1 ? n - 1 : k + hbw;
-                const int ni = i1 - k, nj = j1 - k;
-                for (int p = tid; p < ni * nj; p += nt) {
-                    const int i = k + 1 + p / nj, j = k + 1 + p % nj;
-                    const double factor = band[i * bw + (k - i + hbw)] / diag;
-                    if (factor == 0.0) continue;
-                    band[i * bw + (j - i + hbw)] -= factor * band[k * bw + (j - k + hbw)];
-                }
-                if (tid == 0) s_diag = diag;
-                __syncthreads();
-            }
-            if (dense_mode) {
-                // lsb.mojo:608-628 then :175-214: dense LU with partial pivoting
-                for (int p = tid; p < n * n; p += nt) {
-                    const int row = p / n, col = p % n;
-                    const int off = col - row + hbw;
-                    dense[p] = (off >= 0 && off < bw) ? orig[row * bw + off] : 0.0;
-                }
-                for (int i = tid; i < n; i += nt) piv[i] = i;
-                __syncthreads();
-                for (int col = 0; col < n; ++col) {
-                    if (col > 0) {  // store the multipliers of column col - 1
-                        const int cp = col - 1;
-                        for (int row = cp + 1 + tid; row < n; row += nt)
-                            dense[row * n + cp] = dense[row * n + cp] / s_diag;
-                    }
-                    __syncthreads();
-                    if (tid == 0) {
-                        int prow = col;
-                        double pabs = fabs(dense[col * n + col]);
-                        for (int row = col + 1; row < n; ++row) {
-                            const double cand = fabs(dense[row * n + col]);
-                            if (cand > pabs) {
-                                pabs = cand;
-                                prow = row;
-                            }
-                        }
-                        s_stat = pabs <= 1.0e-14 ? kInstSingular : 0;
-                        s_norm = (double)prow;  // reuse as the broadcast pivot row
-                    }
-                    __syncthreads();
-                    if (s_stat != 0) {
-                        if (tid == 0) A.state[bid].status = s_stat;
-                        return;
-                    }
-                    const int prow = (int)s_norm;
-                    if (prow != col) {
-                        for (int j = tid; j < n; j += nt) {
-                            const double tmp = dense[col * n + j];
-                            dense[col * n + j] = dense[prow * n + j];
-                            dense[prow * n + j] = tmp;
-                        }
-                        if (tid == 0) {
-                            const int tp = piv[col];
-                            piv[col] = piv[prow];
-                            piv[prow] = tp;
-                        }
-                    }
-                    __syncthreads();
-                    const double diag = dense[col * n + col];
-                    if (fabs(diag) <= 1.0e-14) {
-                        if (tid == 0) A.state[bid].status = kInstSingular;
-                        return;
-                    }
-                    const int m = n - col - 1;
-                    for (int p = tid; p < m * m; p += nt) {
-                        const int row = col + 1 + p / m, j = col + 1 + p % m;
-                        const double factor = dense[row * n + col] / diag;
-                        if (factor == 0.0) continue;
-                        dense[row * n + j] -= factor * dense[col * n + j];
-                    }
-                    if (tid == 0) s_diag = diag;
-                    __syncthreads();
-                }
-                // lsb.mojo:216-232
-                if (tid < 64) {
-                    const int lane = tid;
-                    for (int i = lane; i < n; i += 64) work[i] = R[piv[i]];
-                    wave_sync();
-                    for (int j = 0; j < n; ++j) {
-                        const double yj = work[j];
-                        for (int i = j + 1 + lane; i < n; i += 64) work[i] -= dense[i * n + j] * yj;
-                        wave_sync();
-                    }
-                    if (lane == 0) {
-                        for (int i = n - 1; i >= 0; --i) {
-                            double sum = work[i];
-                            for (int j = i + 1; j < n; ++j) sum -= dense[i * n + j] * du[j];
-                            du[i] = sum / dense[i * n + i];
-                        }
-                        st.dense_fallbacks += 1;
-                    }
-                }
-                __syncthreads();
-            } else {
-                {
-                    const int kp = n - 1;
-                    for (int i = kp + 1 + tid; i < n; i += nt) band[i * bw + (kp - i + hbw)] /= s_diag;
-                }
-                __syncthreads();
-                // forward (column order, same per-row subtraction order as lsb.mojo:746-757), then backward
-                if (tid < 64) {
-                    const int lane = tid;
-                    for (int i = lane; i < n; i += 64) work[i] = R[i];
-                    wave_sync();
-                    for (int j = 0; j < n; ++j) {
-                        const double yj = work[j];
-                        for (int t = lane; t < hbw; t += 64) {
-                            const int i = j + 1 + t;
-                            if (i < n) work[i] -= band[i * bw + (j - i + hbw)] * yj;
-                        }
-                        wave_sync();
-                    }
-                    if (lane == 0) {
-                        for (int i = n - 1; i >= 0; --i) {
-                            double sum = work[i];
-                            const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
-                            for (int j = i + 1; j <= j1; ++j) sum -= band[i * bw + (j - i + hbw)] * du[j];
-                            du[i] = sum / band[i * bw + hbw];
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            for (int i = tid; i < n; i += nt) uf[i] += du[i];
-            __syncthreads();
+            damping = sum;
         }
-        if (tid == 0) A.iters[I.iters_base + step] = iters;
-        if (!converged) {
-            if (tid == 0) A.state[bid].status = step + 1;
-            break;
+        const double inertia = (mass[i] * at[i]) * 1.0;
+        R[i] = P[i] - fint[i] - 0.0 - damping - inertia;
+    }
+    __syncthreads();
+    if (tid == 0) {  // :2357-2371
+        const int full = (n / 8) * 8;
+        double sum = 0.0;
+        for (int cb = 0; cb < full; cb += 8) {
+            double v[8];
+            for (int l = 0; l < 8; ++l) v[l] = (R[cb + l] * R[cb + l]) * 1.0;
+            sum += tree8_seq(v);
         }
-        // commit (:3081-3101), committed basic sync (helpers.mojo:1832-1883), Newmark update (:594-647)
+        for (int j = full; j < n; ++j) sum += (R[j] * R[j]) * 1.0;
+        s_norm = sqrt(sum);
+    }
+    __syncthreads();
+    if (s_norm <= F.tol) {
+        // converged: commit (:3081-3101), committed basic sync (helpers.mojo:1832-1883),
+        // Newmark update (:594-647), recorder, then the next step's predictor
+        if (tid == 0) A.iters[I.iters_base + step] = st.iter;
         copy_instance_state(A, I, true);
         for (int e = tid; e < F.n_elem; e += nt) {
             const ElemDev &el = A.elems[F.elem_base + e];
@@ -876,35 +715,241 @@ __global__ __launch_bounds__(256) void frame2d_kernel(Frame2dArgs A, int steps_p
             const int fi = A.rec_fidx[F.rec_base + r];
             h[r] = fi >= 0 ? uf[fi] : 0.0;
         }
-        st.total_iters += iters;
+        __syncthreads();
+        st.total_iters += st.iter;
+        st.iter = 0;
         st.step = step + 1;
-        ++steps_run;
+        if (st.step < I.n_steps) {
+            start_step(A, I, F, st.step, uf, vf, af, un, vn, an, P);
+            __syncthreads();
+        }
+        for (int i = tid; i < 7 * n; i += nt) gv[i] = uf[i];
+        if (tid == 0) A.state[bid] = st;
+        return;
+    }
+    if (st.iter >= F.max_iters) {  // :2846-2850 (the reference aborts)
+        if (tid == 0) {
+            A.iters[I.iters_base + step] = st.iter;
+            st.status = step + 1;
+            A.state[bid] = st;
+        }
+        return;
+    }
+    // K_eff = K_ff + a1 C + a0 M on the band (:2400-2411, lsb.mojo:1877-1981)
+    for (int sl = tid; sl < n * bw; sl += nt) {
+        const int i = sl / bw, j = i + (sl % bw) - hbw;
+        if (j < 0 || j >= n) {
+            band[sl] = 0.0;
+            continue;
+        }
+        const double kb = band[sl];
+        const double mij = i == j ? mass[i] : 0.0;
+        double v = kb;
+        if (F.has_damping) {
+            double cc = 0.0;
+            if (F.alpha_m != 0.0) cc = F.alpha_m * mij;
+            if (F.beta_k != 0.0) cc += F.beta_k * kb;
+            if (F.beta_k_init != 0.0) cc += F.beta_k_init * kinit[sl];
+            v += F.a1 * cc;
+        }
+        v += F.a0 * mij;
+        band[sl] = v;
+    }
+    __syncthreads();
+    // BandGeneral LU without pivoting (lsb.mojo:631-679); the band as
+    // loaded is kept for the dense fallback of :652-657.
+    double *dense = A.dense + I.dense_base;
+    double *orig = dense + (int64_t)n * n;
+    for (int sl = tid; sl < n * bw; sl += nt) orig[sl] = band[sl];
+    int32_t *piv = A.piv + I.piv_base;
+    bool dense_mode = false;
+    for (int k = 0; k < n; ++k) {
+        const int i1 = k + hbw > n - 1 ? n - 1 : k + hbw;
+        if (k > 0) {  // store the multipliers of column k - 1
+            const int kp = k - 1;
+            const int ip1 = kp + hbw > n - 1 ? n - 1 : kp + hbw;
+            for (int i = kp + 1 + tid; i <= ip1; i += nt) {
+                const int sl = i * bw + (kp - i + hbw);
+                band[sl] = band[sl] / s_diag;
+            }
+        }
+        if (tid == 0) {
+            int prow = k;
+            double pabs = 0.0;
+            for (int row = k; row <= i1; ++row) {
+                const double cand = fabs(band[row * bw + (k - row + hbw)]);
+                if (cand > pabs) {
+                    pabs = cand;
+                    prow = row;
+                }
+            }
+            s_stat = 0;
+            if (pabs <= 1.0e-14) s_stat = kInstSingular;
+            else if (prow != k) s_stat = kInstBandPivot;
+            else {
+                const double diag = band[k * bw + hbw];
+                if (fabs(diag) <= 1.0e-14) s_stat = kInstSingular;
+            }
+        }
+        __syncthreads();
+        if (s_stat == kInstBandPivot) {
+            dense_mode = true;
+            break;
+        }
+        if (s_stat != 0) {
+            if (tid == 0) {
+                st.status = s_stat;
+                A.state[bid] = st;
+            }
+            return;
+        }
+        const double diag = band[k * bw + hbw];
+        const int j1 = k + hbw > n - 1 ? n - 1 : k + hbw;
+        const int ni = i1 - k, nj = j1 - k;
+        for (int p = tid; p < ni * nj; p += nt) {
+            const int i = k + 1 + p / nj, j = k + 1 + p % nj;
+            const double factor = band[i * bw + (k - i + hbw)] / diag;
+            if (factor == 0.0) continue;
+            band[i * bw + (j - i + hbw)] -= factor * band[k * bw + (j - k + hbw)];
+        }
+        if (tid == 0) s_diag = diag;
         __syncthreads();
     }
-    for (int i = tid; i < n; i += nt) {
-        vec[i] = uf[i];
-        vec[n + i] = vf[i];
-        vec[2 * n + i] = af[i];
+    if (dense_mode) {
+        // lsb.mojo:608-628 then :175-214: dense LU with partial pivoting
+        for (int p = tid; p < n * n; p += nt) {
+            const int row = p / n, col = p % n;
+            const int off = col - row + hbw;
+            dense[p] = (off >= 0 && off < bw) ? orig[row * bw + off] : 0.0;
+        }
+        for (int i = tid; i < n; i += nt) piv[i] = i;
+        __syncthreads();
+        for (int col = 0; col < n; ++col) {
+            if (col > 0) {  // store the multipliers of column col - 1
+                const int cp = col - 1;
+                for (int row = cp + 1 + tid; row < n; row += nt)
+                    dense[row * n + cp] = dense[row * n + cp] / s_diag;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int prow = col;
+                double pabs = fabs(dense[col * n + col]);
+                for (int row = col + 1; row < n; ++row) {
+                    const double cand = fabs(dense[row * n + col]);
+                    if (cand > pabs) {
+                        pabs = cand;
+                        prow = row;
+                    }
+                }
+                s_stat = pabs <= 1.0e-14 ? kInstSingular : 0;
+                s_norm = (double)prow;  // reuse as the broadcast pivot row
+            }
+            __syncthreads();
+            if (s_stat != 0) {
+                if (tid == 0) {
+                    st.status = s_stat;
+                    A.state[bid] = st;
+                }
+                return;
+            }
+            const int prow = (int)s_norm;
+            if (prow != col) {
+                for (int j = tid; j < n; j += nt) {
+                    const double tmp = dense[col * n + j];
+                    dense[col * n + j] = dense[prow * n + j];
+                    dense[prow * n + j] = tmp;
+                }
+                if (tid == 0) {
+                    const int tp = piv[col];
+                    piv[col] = piv[prow];
+                    piv[prow] = tp;
+                }
+            }
+            __syncthreads();
+            const double diag = dense[col * n + col];
+            if (fabs(diag) <= 1.0e-14) {
+                if (tid == 0) {
+                    st.status = kInstSingular;
+                    A.state[bid] = st;
+                }
+                return;
+            }
+            const int m = n - col - 1;
+            for (int p = tid; p < m * m; p += nt) {
+                const int row = col + 1 + p / m, j = col + 1 + p % m;
+                const double factor = dense[row * n + col] / diag;
+                if (factor == 0.0) continue;
+                dense[row * n + j] -= factor * dense[col * n + j];
+            }
+            if (tid == 0) s_diag = diag;
+            __syncthreads();
+        }
+        // lsb.mojo:216-232
+        if (tid < 64) {
+            const int lane = tid;
+            for (int i = lane; i < n; i += 64) work[i] = R[piv[i]];
+            wave_sync();
+            for (int j = 0; j < n; ++j) {
+                const double yj = work[j];
+                for (int i = j + 1 + lane; i < n; i += 64) work[i] -= dense[i * n + j] * yj;
+                wave_sync();
+            }
+            if (lane == 0) {
+                for (int i = n - 1; i >= 0; --i) {
+                    double sum = work[i];
+                    for (int j = i + 1; j < n; ++j) sum -= dense[i * n + j] * du[j];
+                    du[i] = sum / dense[i * n + i];
+                }
+                st.dense_fallbacks += 1;
+            }
+        }
+        __syncthreads();
+    } else {
+        {
+            const int kp = n - 1;
+            for (int i = kp + 1 + tid; i < n; i += nt) band[i * bw + (kp - i + hbw)] /= s_diag;
+        }
+        __syncthreads();
+        // forward (column order, same per-row subtraction order as lsb.mojo:746-757), then backward
+        if (tid < 64) {
+            const int lane = tid;
+            for (int i = lane; i < n; i += 64) work[i] = R[i];
+            wave_sync();
+            for (int j = 0; j < n; ++j) {
+                const double yj = work[j];
+                for (int t = lane; t < hbw; t += 64) {
+                    const int i = j + 1 + t;
+                    if (i < n) work[i] -= band[i * bw + (j - i + hbw)] * yj;
+                }
+                wave_sync();
+            }
+            if (lane == 0) {
+                for (int i = n - 1; i >= 0; --i) {
+                    double sum = work[i];
+                    const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+                    for (int j = i + 1; j <= j1; ++j) sum -= band[i * bw + (j - i + hbw)] * du[j];
+                    du[i] = sum / band[i * bw + hbw];
+                }
+            }
+        }
+        __syncthreads();
     }
-    if (tid == 0) {
-        InstState o = st;
-        o.status = A.state[bid].status;
-        A.state[bid] = o;
-    }
+    for (int i = tid; i < n; i += nt) gv[i] = uf[i] + du[i];
+    if (tid == 0) A.state[bid] = st;
 }
 
 size_t frame2d_lds_bytes(int n_free, int bw) { return sizeof(double) * ((size_t)13 * n_free + (size_t)n_free * bw); }
 
-hipError_t launch_frame2d(hipStream_t stream, const Frame2dArgs &args, int n_inst, int steps_per_launch,
-                          size_t lds_bytes) {
-    static size_t configured = 0;
-    if (lds_bytes > configured) {
-        hipError_t e = hipFuncSetAttribute((const void *)frame2d_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds_bytes);
-        if (e != hipSuccess) return e;
-        configured = lds_bytes;
-    }
-    hipLaunchKernelGGL(frame2d_kernel, dim3(n_inst), dim3(256), lds_bytes, stream, args, steps_per_launch);
+hipError_t frame2d_set_lds(size_t lds_bytes) {
+    return hipFuncSetAttribute((const void *)frame2d_sys_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes);
+}
+
+// One round: a Newton iteration (or the set-up assembly) of every active instance.
+hipError_t launch_frame2d_round(hipStream_t stream, const Frame2dArgs &args, int n_pairs, int n_inst,
+                                size_t lds_bytes) {
+    hipLaunchKernelGGL(frame2d_elem_kernel, dim3(n_pairs), dim3(kElemThreads), 0, stream, args);
+    hipLaunchKernelGGL(frame2d_sys_kernel, dim3(n_inst), dim3(256), lds_bytes, stream, args);
     return hipGetLastError();
 }
 

@@ -15,8 +15,9 @@ namespace sgpu {
 
 
 size_t frame2d_lds_bytes(int n_free, int bw);
-hipError_t launch_frame2d(hipStream_t stream, const Frame2dArgs &args, int n_inst, int steps_per_launch,
-                          size_t lds_bytes);
+hipError_t frame2d_set_lds(size_t lds_bytes);
+hipError_t launch_frame2d_round(hipStream_t stream, const Frame2dArgs &args, int n_pairs, int n_inst,
+                                size_t lds_bytes);
 
 struct FrameDef {
     FrameDev dev{};
@@ -46,9 +47,20 @@ struct FrameHost {
     double *d_hist = nullptr;
     int32_t *d_iters = nullptr;
     std::vector<int64_t> hist_base, iters_base;
+    int n_pairs = 0;
+    int64_t max_rounds = 0;  // rounds after which every instance has finished
+    // rounds captured in a HIP graph and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_rounds = 0;
 };
 
-void frame2d_host_free(FrameHost *fh) { delete fh; }
+void frame2d_host_free(FrameHost *fh) {
+    if (!fh) return;
+    if (fh->graph_exec) (void)hipGraphExecDestroy(fh->graph_exec);
+    if (fh->graph) (void)hipGraphDestroy(fh->graph);
+    delete fh;
+}
 
 }  // namespace sgpu
 
@@ -314,7 +326,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         I.q_base = q;
         q += d.q_len;
         I.vec_base = vec;
-        vec += 3 * d.n_free;
+        vec += 7 * d.n_free;
         I.kinit_base = kinit;
         kinit += (int64_t)d.n_free * d.bw;
         I.scratch_base = scratch;
@@ -339,8 +351,16 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
             I.aux_count[r] = (int64_t)in.range_count[r] * sd.n_aux;
         }
         fh.max_steps = std::max(fh.max_steps, in.n_steps);
+        fh.max_rounds = std::max(fh.max_rounds, 1 + (int64_t)in.n_steps * d.max_iters);
         idev.push_back(I);
     }
+    std::vector<int32_t> pair_inst, pair_elem;
+    for (size_t k = 0; k < fh.insts.size(); ++k)
+        for (int e = 0; e < fdev[fh.insts[k].frame].n_elem; ++e) {
+            pair_inst.push_back((int32_t)k);
+            pair_elem.push_back(e);
+        }
+    fh.n_pairs = (int)pair_inst.size();
     Frame2dArgs &A = fh.args;
     FrameDev *d_frames;
     InstDev *d_insts;
@@ -360,6 +380,10 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &d_fg, fg))) return rc;
     if ((rc = upload(ctx, &d_ei, elem_inst))) return rc;
     if ((rc = upload(ctx, &d_ag, ag))) return rc;
+    int32_t *d_pi, *d_pe, *d_ef;
+    if ((rc = upload(ctx, &d_pi, pair_inst))) return rc;
+    if ((rc = upload(ctx, &d_pe, pair_elem))) return rc;
+    if ((rc = upload(ctx, &d_ef, std::vector<int32_t>(fh.insts.size(), 0)))) return rc;
     std::vector<InstState> st(fh.insts.size(), InstState{0, 0, 0, 0, 0, 0});
     if ((rc = upload(ctx, &fh.d_state, st))) return rc;
     std::vector<double> zq(q, 0.0), zv(vec, 0.0);
@@ -388,6 +412,9 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     A.fg_off = d_fgo;
     A.fg = d_fg;
     A.elem_inst = d_ei;
+    A.pair_inst = d_pi;
+    A.pair_elem = d_pe;
+    A.eflag = d_ef;
     A.ag = d_ag;
     A.q = d_q;
     A.vec = d_vec;
@@ -400,36 +427,69 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     A.defs = ctx->d_defs;
     A.tb = ctx->tables();
     A.mats = ctx->d_mats;
+    A.n_mats = (int32_t)ctx->mats.size();
     A.c = ctx->c;
     A.t = ctx->t;
     fh.lds = lds;
+    HIP_OK(ctx, frame2d_set_lds(lds));
     fh.finalized = true;
     return SGPU_OK;
 }
 
-int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t steps_per_launch) {
+int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds) {
     if (!ctx) return SGPU_ERR_ARG;
     FrameHost &fh = frames_of(ctx);
     if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
-    if (steps_per_launch <= 0) return fail(ctx, SGPU_ERR_ARG, "steps_per_launch must be > 0");
+    if (rounds <= 0) return fail(ctx, SGPU_ERR_ARG, "rounds must be > 0");
+    if (fh.graph_rounds != rounds) {  // capture `rounds` rounds once, replay them afterwards
+        if (fh.graph_exec) HIP_OK(ctx, hipGraphExecDestroy(fh.graph_exec));
+        if (fh.graph) HIP_OK(ctx, hipGraphDestroy(fh.graph));
+        fh.graph_exec = nullptr;
+        fh.graph = nullptr;
+        fh.graph_rounds = 0;
+        HIP_OK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipSuccess;
+        for (int r = 0; r < rounds && le == hipSuccess; ++r)
+            le = launch_frame2d_round(ctx->stream, fh.args, fh.n_pairs, (int)fh.insts.size(), fh.lds);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(ctx, SGPU_ERR_HIP, "capturing the frame rounds: %s",
+                        hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        fh.graph = g;
+        HIP_OK(ctx, hipGraphInstantiate(&fh.graph_exec, g, nullptr, nullptr, 0));
+        fh.graph_rounds = rounds;
+    }
     {
         ProfScope ps(ctx, SGPU_KERNEL_FRAME2D);
-        HIP_OK(ctx, launch_frame2d(ctx->stream, fh.args, (int)fh.insts.size(), steps_per_launch, fh.lds));
+        HIP_OK(ctx, hipGraphLaunch(fh.graph_exec, ctx->stream));
     }
     return SGPU_OK;
 }
 
-int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t steps_per_launch) {
+int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
     if (!ctx) return SGPU_ERR_ARG;
     FrameHost &fh = frames_of(ctx);
     if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
-    if (steps_per_launch <= 0) return fail(ctx, SGPU_ERR_ARG, "steps_per_launch must be > 0");
-    const int launches = (fh.max_steps + steps_per_launch - 1) / steps_per_launch;
-    for (int k = 0; k < launches; ++k) {
-        int rc = sgpu_frame2d_launch(ctx, steps_per_launch);
+    if (rounds <= 0) return fail(ctx, SGPU_ERR_ARG, "rounds must be > 0");
+    std::vector<InstState> st(fh.insts.size());
+    int64_t done_rounds = 0;
+    while (true) {
+        int rc = sgpu_frame2d_launch(ctx, rounds);
         if (rc) return rc;
+        done_rounds += rounds;
+        HIP_OK(ctx, hipMemcpyAsync(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+        bool active = false;
+        for (size_t k = 0; k < st.size(); ++k)
+            if (st[k].status == 0 && st[k].step < fh.insts[k].n_steps) active = true;
+        if (!active) break;
+        if (done_rounds > fh.max_rounds)
+            return fail(ctx, SGPU_ERR_STATE, "instances still active after %lld rounds", (long long)done_rounds);
     }
-    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
     return SGPU_OK;
 }
 

@@ -43,7 +43,7 @@ struct InstDev {
     int32_t n_ranges, pad_;
     int64_t ag_base;        // ground acceleration at t = (step + 1) dt
     int64_t elem_inst_base; // section instance of integration point 0, per element
-    int64_t q_base, vec_base, kinit_base, scratch_base, hist_base, iters_base;
+    int64_t q_base, vec_base, kinit_base, scratch_base, hist_base, iters_base;  // vec: 7 n (u v a un vn an P)
     int64_t dense_base;     // n*n dense fallback matrix followed by the n*bw band copy
     int64_t piv_base;       // n row permutation of the dense fallback
     int64_t slot_begin[4], slot_count[4], aux_begin[4], aux_count[4];  // fiber state ranges
@@ -55,7 +55,7 @@ struct InstState {
     int32_t setup_done;
     int32_t total_iters;
     int32_t dense_fallbacks;  // BandGeneral factorizations that switched to the dense LU
-    int32_t pad_;
+    int32_t iter;       // assemblies so far in the current step
 };
 
 enum InstStatus : int32_t {
@@ -78,12 +78,15 @@ struct Frame2dArgs {
     const int32_t *kg_off, *kg;
     const int32_t *fg_off, *fg;
     const int32_t *elem_inst;
+    const int32_t *pair_inst, *pair_elem;  // element-kernel block -> (instance, element)
+    int32_t *eflag;                        // per instance: an element state failed this round
     const double *ag;
     double *q, *vec, *kinit, *scratch, *hist, *dense;
     int32_t *iters, *piv;
     const SecDefDev *defs;
     SlotTables tb;
     const MatDev *mats;
+    int32_t n_mats;
     StateArrays c, t;
 };
 

@@ -135,67 +135,160 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
+constexpr int kWaveRedStride = kBatch + 1;  // LDS row stride of wave_section_2d (odd: rows in different banks)
+
+// Orders the LDS traffic of one wavefront: writes before the barrier are
+// seen by reads after it (LDS requests of one wave complete in order; this
+// only keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// What one lane reads for its slot of a batch before the law runs: the
+// table entry and the committed common history (read for every slot: the
+// address does not depend on the table, so the loads of a batch issue
+// together).
+struct WaveSlotIn {
+    int mi, code, aux;
+    double y, area;
+    double eps, sig, tan, h0, h1;
+};
+
+__device__ __forceinline__ void wave_slot_load(const SecDefDev &d, const SlotTables &tb, const StateArrays &c,
+                                               int64_t inst_state, int s, WaveSlotIn &in) {
+    in.mi = -1;
+    in.code = -1;
+    in.aux = -1;
+    in.y = in.area = 0.0;
+    in.eps = in.sig = in.tan = in.h0 = in.h1 = 0.0;
+    if (s >= d.n_slots) return;
+    const int64_t gs = d.slot_base + s;
+    const int64_t gi = inst_state + s;
+    in.mi = tb.mat[gs];
+    in.code = tb.chunk[gs];
+    in.aux = tb.aux[gs];
+    in.y = tb.y[gs];
+    in.area = tb.area[gs];
+    in.eps = c.eps[gi];
+    in.sig = c.sig[gi];
+    in.tan = c.tan[gi];
+    in.h0 = c.h0[gi];
+    in.h1 = c.h1[gi];
+}
+
 // Section resultants of a 2D fiber section evaluated by one whole wave
-// (fiber2d.mojo:3982-4024): every lane returns the same five sums, formed
-// in the reference's order (chunk trees, then chunks one after another).
+// (fiber2d.mojo:3982-4024).  Each batch leaves its chunk partials in the
+// wave's LDS rows `red` (5 x kWaveRedStride doubles); lanes 0..4 add them
+// one after another onto their running sums, so the sums see the chunks in
+// the reference's order; every lane returns the same five sums.
 struct Sec2dOut {
     double N, Mz, k11, k12, k22;
 };
 
 __device__ __forceinline__ Sec2dOut wave_section_2d(const SecDefDev &d, const SlotTables &tb,
                                                     const MatDev *__restrict__ mats, const StateArrays &c,
-                                                    const StateArrays &t, int inst, double eps0, double kappa) {
+                                                    const StateArrays &t, int inst, double eps0, double kappa,
+                                                    double *red) {
     const int lane = threadIdx.x & 63;
     const int64_t inst_state = d.state_base + (int64_t)inst * d.n_slots;
     const int64_t inst_aux = d.aux_base + (int64_t)inst * d.n_aux;
     const int32_t *bcb = tb.batch_chunk_begin + d.batch_base;
-    const int32_t *cslot = tb.chunk_slot + d.chunk_base;
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0, acc4 = 0.0;
+    const bool zero_lane = d.zero_k12 && lane == 3;
+    double acc = 0.0;  // lanes 0..4: running sum of N, Mz, k11, k12, k22
+    WaveSlotIn cur;
+    wave_slot_load(d, tb, c, inst_state, lane, cur);
     for (int b = 0; b < d.n_batches; ++b) {
-        const int s = b * kBatch + lane;
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
-        int code = -1;
-        if (s < d.n_slots) {
-            const int64_t gs = d.slot_base + s;
-            const int mi = tb.mat[gs];
-            code = tb.chunk[gs];
-            if (mi >= 0) {
-                const double y = tb.y[gs];
-                const double area = tb.area[gs];
-                const double strain = eps0 - y * kappa;
-                const int32_t aux = tb.aux[gs];
-                double fs, ks;
-                eval_slot(mats, mi, strain, area, c, t, inst_state + s, inst_aux + (aux < 0 ? 0 : aux), fs, ks);
-                t0 = fs;
-                t1 = -fs * y;
-                t2 = ks;
-                t3 = -ks * y;
-                t4 = ks * y * y;
+        WaveSlotIn nxt;
+        if (b + 1 < d.n_batches) wave_slot_load(d, tb, c, inst_state, (b + 1) * kBatch + lane, nxt);
+        double fs = 0.0, ks = 0.0;
+        if (cur.mi >= 0) {
+            const MatDev &m = mats[cur.mi];
+            const double strain = eps0 - cur.y * kappa;
+            if (m.type == kElastic) {
+                ks = m.p[0] * cur.area;
+                fs = ks * strain;
+            } else {
+                const int64_t gi = inst_state + b * kBatch + lane;
+                const int64_t ai = inst_aux + (cur.aux < 0 ? 0 : cur.aux);
+                FiberHist hc;
+                hc.eps = cur.eps;
+                hc.sig = cur.sig;
+                hc.tan = cur.tan;
+                hc.h0 = cur.h0;
+                hc.h1 = cur.h1;
+                hc.a0 = hc.a1 = hc.a2 = hc.a3 = hc.a4 = 0.0;
+                hc.kon = 0;
+                if (m.type == kSteel02) {
+                    hc.tan = 0.0;  // Steel02 never reads its committed tangent
+                    hc.a0 = c.a0[ai];
+                    hc.a1 = c.a1[ai];
+                    hc.a2 = c.a2[ai];
+                    hc.a3 = c.a3[ai];
+                    hc.a4 = c.a4[ai];
+                    hc.kon = c.kon[ai];
+                } else if (m.type == kConcrete01) {
+                    hc.a0 = c.a0[ai];
+                }
+                FiberHist ht = hc;
+                switch (m.type) {
+                case kConcrete02: concrete02_trial(m, hc, strain, ht); break;
+                case kSteel02: steel02_trial(m, hc, strain, ht); break;
+                case kSteel01: steel01_trial(m, hc, strain, ht); break;
+                default: concrete01_trial(m, hc, strain, ht); break;
+                }
+                store_hist(t, gi, ai, m.type, ht);
+                fs = ht.sig * cur.area;
+                ks = ht.tan * cur.area;
             }
         }
+        const double y = cur.y;
+        const double t0 = fs, t1 = -fs * y, t2 = ks, t3 = -ks * y, t4 = ks * y * y;
         // chunk partial in the chunk's first lane: W-wide tree or the lane's own term
-        const double r0 = tree8(t0), r1 = tree8(t1), r2 = tree8(t2), r3 = tree8(t3), r4 = tree8(t4);
-        const bool w8 = code >= 0 && (code & 1);
-        const double p0 = w8 ? r0 : t0, p1 = w8 ? r1 : t1, p2 = w8 ? r2 : t2, p3 = w8 ? r3 : t3,
-                     p4 = w8 ? r4 : t4;
-        const int c0 = bcb[b], c1 = bcb[b + 1];
-        for (int cc = c0; cc < c1; ++cc) {
-            const int src = cslot[cc] & (kBatch - 1);
-            if (d.zero_k12 && cc == d.n_elastic_chunks) acc3 = 0.0;
-            acc0 += readlane_d(p0, src);
-            acc1 += readlane_d(p1, src);
-            acc2 += readlane_d(p2, src);
-            acc3 += readlane_d(p3, src);
-            acc4 += readlane_d(p4, src);
+        const bool w8 = cur.code >= 0 && (cur.code & 1);
+        double p0 = t0, p1 = t1, p2 = t2, p3 = t3, p4 = t4;
+        if (__builtin_amdgcn_ballot_w64(w8) != 0) {
+            const double r0 = tree8_dpp(t0), r1 = tree8_dpp(t1), r2 = tree8_dpp(t2), r3 = tree8_dpp(t3),
+                         r4 = tree8_dpp(t4);
+            if (w8) {
+                p0 = r0;
+                p1 = r1;
+                p2 = r2;
+                p3 = r3;
+                p4 = r4;
+            }
         }
+        if (cur.code >= 0) {
+            const int dst = cur.code >> 1;
+            red[dst] = p0;
+            red[kWaveRedStride + dst] = p1;
+            red[2 * kWaveRedStride + dst] = p2;
+            red[3 * kWaveRedStride + dst] = p3;
+            red[4 * kWaveRedStride + dst] = p4;
+        }
+        wave_sync();
+        if (lane < 5) {
+            const int c0 = bcb[b], c1 = bcb[b + 1];
+            const double *row = red + lane * kWaveRedStride - c0;
+            int cc = c0;
+            if (zero_lane && d.n_elastic_chunks >= c0 && d.n_elastic_chunks < c1) {
+                for (; cc < d.n_elastic_chunks; ++cc) acc += row[cc];
+                acc = 0.0;  // uniform elastic modulus: k12 restarts with the nonlinear families
+            }
+#pragma unroll 8
+            for (; cc < c1; ++cc) acc += row[cc];
+        }
+        wave_sync();
+        cur = nxt;
     }
-    if (d.zero_k12 && bcb[d.n_batches] <= d.n_elastic_chunks) acc3 = 0.0;
+    if (zero_lane && bcb[d.n_batches] <= d.n_elastic_chunks) acc = 0.0;
     Sec2dOut o;
-    o.N = acc0;
-    o.Mz = acc1;
-    o.k11 = acc2;
-    o.k12 = acc3;
-    o.k22 = acc4;
+    o.N = readlane_d(acc, 0);
+    o.Mz = readlane_d(acc, 1);
+    o.k11 = readlane_d(acc, 2);
+    o.k12 = readlane_d(acc, 3);
+    o.k22 = readlane_d(acc, 4);
     return o;
 }
 

@@ -27,6 +27,7 @@ struct MatDev {
     double p[11];
     double pre[5];
 };
+static_assert(sizeof(MatDev) % sizeof(double) == 0, "MatDev is copied as doubles");
 
 // One fiber-section definition in slot layout.  Slots are the section's
 // fibers placed in the reference's reduction order, 64 to a batch:

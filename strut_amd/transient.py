@@ -4,7 +4,9 @@
 number of ground-motion instances per frame, then runs them all on one GPU
 through the C ABI (sgpu_frame2d_*).  It mirrors what Strut's
 run_transient_nonlinear (transient_nonlinear.mojo:830) does for one case,
-for a whole ensemble at once.
+for a whole ensemble at once: every instance advances its own Newmark /
+Newton loop, one Newton iteration per round, with its own step count and
+convergence.
 """
 
 from __future__ import annotations
@@ -126,11 +128,15 @@ class FrameBatch:
         self._check(self.lib.sgpu_frame2d_finalize(self.rt.ctx))
         self.finalized = True
 
-    def launch(self, steps_per_launch: int) -> None:
-        self._check(self.lib.sgpu_frame2d_launch(self.rt.ctx, steps_per_launch))
+    def launch(self, rounds: int) -> None:
+        """Enqueue `rounds` rounds (one Newton iteration of every active
+        instance each), replayed from a captured HIP graph."""
+        self._check(self.lib.sgpu_frame2d_launch(self.rt.ctx, rounds))
 
-    def run(self, steps_per_launch: int = 100) -> None:
-        self._check(self.lib.sgpu_frame2d_run(self.rt.ctx, steps_per_launch))
+    def run(self, rounds: int = 64) -> None:
+        """Run until every instance has finished or stopped, checking after
+        every `rounds` rounds."""
+        self._check(self.lib.sgpu_frame2d_run(self.rt.ctx, rounds))
 
     def synchronize(self) -> None:
         self.rt.synchronize()

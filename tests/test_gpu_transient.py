@@ -34,7 +34,7 @@ def test_gpu_config0_plain_matches_oracle():
     f = fb.add_frame(fc, record_all=True)
     i = fb.add_case_motion(f, fc)
     fb.finalize()
-    fb.run(steps_per_launch=100)
+    fb.run(rounds=64)
     got = fb.result(i)
     fb.close()
     print(f"config0 400 steps: status {got['status']}, iterations {got['iters'].sum()} vs {ref['iters'].sum()}")
@@ -58,7 +58,7 @@ def test_gpu_ensemble_of_uneven_instances_matches_oracle():
             frame_of[(s, b)] = fb.add_frame(fc, record_all=True)
         ids.append(fb.add_case_motion(frame_of[(s, b)], fc))
     fb.finalize()
-    fb.run(steps_per_launch=64)
+    fb.run(rounds=64)
     for fc, iid in zip(fcs, ids):
         got = fb.result(iid)
         ref = oracle.run_transient2d(fc)
