@@ -220,11 +220,12 @@ def gm_leg(n_batch: int, steps: int) -> dict:
         dt = time.perf_counter() - t0
         kms, kn = fb.rt.profile_read(_native.KERNEL_FRAME2D)
         fb.rt.profile_enable(False)
+        sched = fb.schedule_info()
         res = [fb.result(i) for i in ids]
         done = sum(r["steps_done"] for r in res)
         iters = sum(int(r["iters"][:r["steps_done"]].sum()) for r in res)
         out[label] = {"records": len(ms), "gm_steps": int(done), "seconds": dt, "gm_steps_per_s": done / dt,
-                      "kernel_ms": kms, "launches": kn, "newton_iters": iters,
+                      "kernel_ms": kms, "launches": kn, "newton_iters": iters, "schedule": sched,
                       "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0))}
         fb.close()
     out["batched_vs_plain"] = out["batched"]["gm_steps_per_s"] / out["plain"]["gm_steps_per_s"]

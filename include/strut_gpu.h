@@ -222,6 +222,21 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx);
 int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds);
 int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds);
 
+/* The persistent schedule: one workgroup per instance runs its own Newton
+ * loop for up to max_steps steps without waiting for any other instance
+ * (asynchronous).  Same per-instance state and arithmetic as the rounds, so
+ * the two schedules can alternate within one run with identical results. */
+int sgpu_frame2d_launch_persistent(sgpu_ctx *ctx, int32_t max_steps);
+
+/* Schedule used by sgpu_frame2d_run.  AUTO (the default) times a chunk of
+ * each schedule on the live ensemble, keeps the faster one and measures
+ * again whenever the number of active instances has halved.
+ * sgpu_frame2d_schedule_info: out4 = {schedule in use, last measured
+ * GM-steps/s of the rounds, of the persistent schedule, tunings}. */
+enum { SGPU_SCHEDULE_AUTO = 0, SGPU_SCHEDULE_ROUNDS = 1, SGPU_SCHEDULE_PERSISTENT = 2 };
+int sgpu_frame2d_set_schedule(sgpu_ctx *ctx, int schedule);
+int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out4);
+
 /* Per-step recorded displacements (n_steps x n_record), per-step Newton
  * iteration counts (assemblies, including the converged one), status
  * (0 = ran to the end; k > 0 = step k did not converge, where the reference

@@ -54,6 +54,9 @@ EXPORTED = (
     "sgpu_frame2d_run",
     "sgpu_frame2d_result",
     "sgpu_frame2d_info",
+    "sgpu_frame2d_launch_persistent",
+    "sgpu_frame2d_set_schedule",
+    "sgpu_frame2d_schedule_info",
     "sgpu_profile_enable",
     "sgpu_profile_read",
 )
@@ -63,6 +66,11 @@ KERNEL_AUX_EVAL = 0
 KERNEL_SECTION_EVAL = 1
 KERNEL_COMMIT = 2
 KERNEL_FRAME2D = 3
+
+# Schedules of sgpu_frame2d_run.
+SCHEDULE_AUTO = 0
+SCHEDULE_ROUNDS = 1
+SCHEDULE_PERSISTENT = 2
 
 
 class SectionPoint(ctypes.Structure):

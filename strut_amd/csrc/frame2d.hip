@@ -71,6 +71,7 @@ struct ElemCtx {
     ElemShared *sh;
     const MatDev *mats;  // material table (LDS copy when it fits)
     double *red;         // this wavefront's reduction rows
+    double *last;        // this element's last evaluated (eps0, kappa) per integration point
 };
 
 // fiber_section2d_batch_eval_same_def (fiber2d.mojo:3042-3086) for the
@@ -85,8 +86,11 @@ __device__ bool eval_sections(const ElemCtx &E, const double *pe, const double *
     const int w = threadIdx.x >> 6;
     __syncthreads();  // every wavefront has read the previous results
     if (w < nip) {
-        const Sec2dOut r = wave_section_2d(d, E.A->tb, E.mats, E.A->c, E.A->t, E.inst0 + w, pe[w], pk[w], E.red);
+        const double e0 = pe[w], k0 = pk[w];
+        const Sec2dOut r = wave_section_2d<false>(d, E.A->tb, E.mats, E.A->c, E.A->t, E.inst0 + w, e0, k0, E.red);
         if ((threadIdx.x & 63) == 0) {
+            E.last[2 * w] = e0;
+            E.last[2 * w + 1] = k0;
             const double det = r.k11 * r.k22 - r.k12 * r.k12;
             double *o = E.sh->res[w];
             o[0] = r.N;
@@ -120,7 +124,9 @@ struct FbcScratch {
     double best[10], solved[10];
 };
 
-// force_beam_column2d.mojo:804-1189 (try_increment); best = {q0,q1,q2,k00,k01,k02,k10,k11,k12,k22}
+// force_beam_column2d.mojo:804-1189 (try_increment); best = {q0,q1,q2,k00,k01,k02,k10,k11,k12,k22}.
+// The predictor evaluation (:877-926) runs as iteration -1 of the local loop
+// so that the section evaluation is instantiated once.
 __device__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, double bv0, double bv1, double bv2,
                               double dv0, double dv1, double dv2, int scheme, double *best) {
     const ElemDev &el = *E.el;
@@ -129,59 +135,64 @@ __device__ bool try_increment(const ElemCtx &E, double *q, FbcScratch &sc, doubl
     double q0 = q[0], q1 = q[1], q2 = q[2];
     const double tv0 = bv0 + dv0, tv1 = bv1 + dv1, tv2 = bv2 + dv2;
     if (!el.if_valid) return false;
-    for (int ip = 0; ip < nip; ++ip) {
-        sc.pe[ip] = q[3 + ip];
-        sc.pk[ip] = q[3 + nip + ip];
-        sc.vs_e[ip] = sc.pe[ip];
-        sc.vs_k[ip] = sc.pk[ip];
-    }
-    if (!eval_sections(E, sc.pe, sc.pk)) return false;
-    double pf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int ip = 0; ip < nip; ++ip) {
-        const double *r = E.sh->res[ip];
-        sc.sub_e[ip] = sc.vs_e[ip];
-        sc.sub_k[ip] = sc.vs_k[ip];
-        sc.sub_ax[ip] = r[0];
-        sc.sub_mo[ip] = r[1];
-        sc.sub_f00[ip] = r[2];
-        sc.sub_f01[ip] = r[3];
-        sc.sub_f11[ip] = r[4];
-        const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
-        const double a00 = r[2], a01 = r[3], a10 = a01, a11 = r[4];
-        pf[0] += wL * a00;
-        pf[1] += wL * a01 * b_mi;
-        pf[2] += wL * a01 * b_mj;
-        pf[3] += wL * b_mi * a10;
-        pf[4] += wL * b_mi * a11 * b_mi;
-        pf[5] += wL * b_mi * a11 * b_mj;
-        pf[6] += wL * b_mj * a10;
-        pf[7] += wL * b_mj * a11 * b_mi;
-        pf[8] += wL * b_mj * a11 * b_mj;
-    }
-    double pk3[9];
-    if (invert3(pf[0], pf[1], pf[2], pf[3], pf[4], pf[5], pf[6], pf[7], pf[8], pk3)) {
-        q0 += pk3[0] * dv0 + pk3[1] * dv1 + pk3[2] * dv2;
-        q1 += pk3[3] * dv0 + pk3[4] * dv1 + pk3[5] * dv2;
-        q2 += pk3[6] * dv0 + pk3[7] * dv1 + pk3[8] * dv2;
-    }
     const int n_iters = scheme == 1 ? 100 : 10;
-    for (int it = 0; it < n_iters; ++it) {
+    for (int it = -1; it < n_iters; ++it) {
         const bool use_init = scheme == 1 || (scheme == 2 && it == 0);
         for (int ip = 0; ip < nip; ++ip) {
-            const double xi = el.xi[ip], b_mi = xi - 1.0, b_mj = xi;
-            const double ss0 = q0 + 0.0;
-            const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
-            const double dss0 = ss0 - sc.sub_ax[ip];
-            const double dss1 = ss1 - sc.sub_mo[ip];
-            const double s00 = use_init ? el.if00 : sc.sub_f00[ip];
-            const double s01 = use_init ? el.if01 : sc.sub_f01[ip];
-            const double s11 = use_init ? el.if11 : sc.sub_f11[ip];
-            const double dvs0 = s00 * dss0 + s01 * dss1;
-            const double dvs1 = s01 * dss0 + s11 * dss1;
-            sc.pe[ip] = sc.sub_e[ip] + dvs0;
-            sc.pk[ip] = sc.sub_k[ip] + dvs1;
+            if (it < 0) {  // predictor sections at the element's current deformations
+                sc.pe[ip] = q[3 + ip];
+                sc.pk[ip] = q[3 + nip + ip];
+                sc.vs_e[ip] = sc.pe[ip];
+                sc.vs_k[ip] = sc.pk[ip];
+            } else {
+                const double xi = el.xi[ip], b_mi = xi - 1.0, b_mj = xi;
+                const double ss0 = q0 + 0.0;
+                const double ss1 = b_mi * q1 + b_mj * q2 + 0.0;
+                const double dss0 = ss0 - sc.sub_ax[ip];
+                const double dss1 = ss1 - sc.sub_mo[ip];
+                const double s00 = use_init ? el.if00 : sc.sub_f00[ip];
+                const double s01 = use_init ? el.if01 : sc.sub_f01[ip];
+                const double s11 = use_init ? el.if11 : sc.sub_f11[ip];
+                const double dvs0 = s00 * dss0 + s01 * dss1;
+                const double dvs1 = s01 * dss0 + s11 * dss1;
+                sc.pe[ip] = sc.sub_e[ip] + dvs0;
+                sc.pk[ip] = sc.sub_k[ip] + dvs1;
+            }
         }
         if (!eval_sections(E, sc.pe, sc.pk)) return false;
+        if (it < 0) {
+            // predictor stiffness and force update (:928-998)
+            double pf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int ip = 0; ip < nip; ++ip) {
+                const double *r = E.sh->res[ip];
+                sc.sub_e[ip] = sc.vs_e[ip];
+                sc.sub_k[ip] = sc.vs_k[ip];
+                sc.sub_ax[ip] = r[0];
+                sc.sub_mo[ip] = r[1];
+                sc.sub_f00[ip] = r[2];
+                sc.sub_f01[ip] = r[3];
+                sc.sub_f11[ip] = r[4];
+                const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+                const double a00 = r[2], a01 = r[3], a10 = a01, a11 = r[4];
+                pf[0] += wL * a00;
+                pf[1] += wL * a01 * b_mi;
+                pf[2] += wL * a01 * b_mj;
+                pf[3] += wL * b_mi * a10;
+                pf[4] += wL * b_mi * a11 * b_mi;
+                pf[5] += wL * b_mi * a11 * b_mj;
+                pf[6] += wL * b_mj * a10;
+                pf[7] += wL * b_mj * a11 * b_mi;
+                pf[8] += wL * b_mj * a11 * b_mj;
+            }
+            double pk3[9];
+            if (invert3(pf[0], pf[1], pf[2], pf[3], pf[4], pf[5], pf[6], pf[7], pf[8], pk3)) {
+                q0 += pk3[0] * dv0 + pk3[1] * dv1 + pk3[2] * dv2;
+                q1 += pk3[3] * dv0 + pk3[4] * dv1 + pk3[5] * dv2;
+                q2 += pk3[6] * dv0 + pk3[7] * dv1 + pk3[8] * dv2;
+            }
+            continue;
+        }
+        // local iteration (:1000-1189)
         double f[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         double vr0 = 0.0, vr1 = 0.0, vr2 = 0.0;
         for (int ip = 0; ip < nip; ++ip) {
@@ -316,7 +327,7 @@ __device__ __forceinline__ double kglobal(const double *kb, double inv_L, double
 __device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q_global,
                               const double *uf, double *out42, ElemShared *sh, double *q, FbcScratch &sc,
                               const MatDev *mats, double *red) {
-    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e], sh, mats, red};
+    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e], sh, mats, red, A.last + I.last_base + 2 * kMaxNipDev * e};
     const int nip = el.nip;
     const int q_len = 3 + 2 * nip + 3;
     // the wavefront's private copy of the element state (uniform in all wavefronts)
@@ -349,69 +360,63 @@ __device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const Elem
     double *best = sc.best;
     int num_sub = 1;
     int guard = 0;  // bounds the subdivision loop (the reference's has no bound)
-    if (max_abs3(rem0, rem1, rem2) <= 1.0e-16) {
+    // The zero-increment attempt (force_beam_column2d.mojo:1623-1650) and the
+    // subdivision loop (:1651-1763) share one call site of try_increment: the
+    // former is the scheme loop run once with a zero increment.
+    const bool zero_inc = max_abs3(rem0, rem1, rem2) <= 1.0e-16;
+    if (zero_inc) at0 = at1 = at2 = 0.0;
+    while (true) {
+        if (!zero_inc && (num_sub > 4 || ++guard > 4096)) break;
+        const double tv0 = ab0 + at0, tv1 = ab1 + at1, tv2 = ab2 + at2;
+        bool scheme_success = false;
         for (int scheme = 0; scheme < 3; ++scheme) {
             restore_predictor(q, nip, aq0, aq1, aq2, sc);
             q[boff] = ab0;
             q[boff + 1] = ab1;
             q[boff + 2] = ab2;
-            if (!try_increment(E, q, sc, ab0, ab1, ab2, 0.0, 0.0, 0.0, scheme, best)) continue;
-            have_best = true;
-            converged = true;
-            break;
-        }
-    } else {
-        while (true) {
-            if (num_sub > 4 || ++guard > 4096) break;
-            const double tv0 = ab0 + at0, tv1 = ab1 + at1, tv2 = ab2 + at2;
-            bool scheme_success = false;
-            for (int scheme = 0; scheme < 3; ++scheme) {
-                restore_predictor(q, nip, aq0, aq1, aq2, sc);
-                q[boff] = ab0;
-                q[boff + 1] = ab1;
-                q[boff + 2] = ab2;
-                double *solved = sc.solved;
-                if (!try_increment(E, q, sc, ab0, ab1, ab2, at0, at1, at2, scheme, solved)) continue;
-                ab0 = tv0;
-                ab1 = tv1;
-                ab2 = tv2;
-                q[boff] = ab0;
-                q[boff + 1] = ab1;
-                q[boff + 2] = ab2;
-                aq0 = solved[0];
-                aq1 = solved[1];
-                aq2 = solved[2];
-#pragma unroll
-                for (int ip = 0; ip < kMaxNipDev; ++ip) {
-                    if (ip < nip) {
-                        sc.vs_e[ip] = q[3 + ip];
-                        sc.vs_k[ip] = q[3 + nip + ip];
-                    }
-                }
-#pragma unroll
+            double *solved = sc.solved;
+            if (!try_increment(E, q, sc, ab0, ab1, ab2, at0, at1, at2, scheme, solved)) continue;
+            if (zero_inc) {
                 for (int k = 0; k < 10; ++k) best[k] = solved[k];
                 have_best = true;
-                rem0 = v0 - ab0;
-                rem1 = v1 - ab1;
-                rem2 = v2 - ab2;
-                if (max_abs3(rem0, rem1, rem2) <= 1.0e-12) {
-                    converged = true;
-                } else {
-                    at0 = rem0;
-                    at1 = rem1;
-                    at2 = rem2;
-                    num_sub = 1;
-                }
-                scheme_success = true;
+                converged = true;
                 break;
             }
-            if (converged) break;
-            if (scheme_success) continue;
-            at0 /= 10.0;
-            at1 /= 10.0;
-            at2 /= 10.0;
-            num_sub += 1;
+            ab0 = tv0;
+            ab1 = tv1;
+            ab2 = tv2;
+            q[boff] = ab0;
+            q[boff + 1] = ab1;
+            q[boff + 2] = ab2;
+            aq0 = solved[0];
+            aq1 = solved[1];
+            aq2 = solved[2];
+            for (int ip = 0; ip < nip; ++ip) {
+                sc.vs_e[ip] = q[3 + ip];
+                sc.vs_k[ip] = q[3 + nip + ip];
+            }
+            for (int k = 0; k < 10; ++k) best[k] = solved[k];
+            have_best = true;
+            rem0 = v0 - ab0;
+            rem1 = v1 - ab1;
+            rem2 = v2 - ab2;
+            if (max_abs3(rem0, rem1, rem2) <= 1.0e-12) {
+                converged = true;
+            } else {
+                at0 = rem0;
+                at1 = rem1;
+                at2 = rem2;
+                num_sub = 1;
+            }
+            scheme_success = true;
+            break;
         }
+        if (zero_inc || converged) break;
+        if (scheme_success) continue;
+        at0 /= 10.0;
+        at1 /= 10.0;
+        at2 /= 10.0;
+        num_sub += 1;
     }
     if (!converged) {
         restore_predictor(q, nip, aq0, aq1, aq2, sc);
@@ -486,58 +491,65 @@ __device__ __forceinline__ double tree8_seq(const double *v) {
     return x0 + x1;
 }
 
-// Copy the instance's committed <- trial (commit) or trial <- committed.
-__device__ void copy_instance_state(const Frame2dArgs &A, const InstDev &I, bool commit) {
-    const StateArrays &dst = commit ? A.c : A.t;
-    const StateArrays &src = commit ? A.t : A.c;
-    for (int r = 0; r < I.n_ranges; ++r) {
-        for (int64_t k = threadIdx.x; k < I.slot_count[r]; k += blockDim.x) {
-            const int64_t g = I.slot_begin[r] + k;
-            dst.eps[g] = src.eps[g];
-            dst.sig[g] = src.sig[g];
-            dst.tan[g] = src.tan[g];
-            dst.h0[g] = src.h0[g];
-            dst.h1[g] = src.h1[g];
-        }
-        for (int64_t k = threadIdx.x; k < I.aux_count[r]; k += blockDim.x) {
-            const int64_t g = I.aux_begin[r] + k;
-            dst.a0[g] = src.a0[g];
-            dst.a1[g] = src.a1[g];
-            dst.a2[g] = src.a2[g];
-            dst.a3[g] = src.a3[g];
-            dst.a4[g] = src.a4[g];
-            dst.kon[g] = src.kon[g];
-        }
+// Commit every fiber section of the instance (transient_nonlinear.mojo:3081-3101):
+// one wavefront per (element, integration point) rebuilds the trial state of
+// the section's last evaluation into the committed arrays.
+__device__ void commit_instance(const Frame2dArgs &A, const InstDev &I, const FrameDev &F) {
+    const int wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+    const int n_pairs = F.n_elem * kMaxNipDev;
+    for (int k = wave; k < n_pairs; k += n_waves) {
+        const int e = k / kMaxNipDev, ip = k % kMaxNipDev;
+        const ElemDev &el = A.elems[F.elem_base + e];
+        if (ip >= el.nip) continue;
+        const double *last = A.last + I.last_base + 2 * kMaxNipDev * e + 2 * ip;
+        commit_section_2d(A.defs[el.def], A.tb, A.mats, A.c, A.elem_inst[I.elem_inst_base + e] + ip, last[0],
+                          last[1]);
     }
+}
+
+// LDS of the element state determination: block-wide section results,
+// per-wavefront element state, scratch and reduction rows, and the staged
+// material table.
+struct ElemLds {
+    ElemShared sh;
+    double qbuf[kMaxNipDev][3 + 2 * kMaxNipDev + 3];
+    FbcScratch scr[kMaxNipDev];
+    double wred[kMaxNipDev][5 * kWaveRedStride];
+    MatDev smats[kMatsInLds];
+};
+
+// Material parameters through LDS when the table fits; called by every thread.
+__device__ __forceinline__ const MatDev *stage_materials(const Frame2dArgs &A, ElemLds &L) {
+    if (A.n_mats > kMatsInLds) return A.mats;
+    const double *src = reinterpret_cast<const double *>(A.mats);
+    double *dst = reinterpret_cast<double *>(L.smats);
+    const int nd = A.n_mats * (int)(sizeof(MatDev) / sizeof(double));
+    for (int k = threadIdx.x; k < nd; k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
+    return L.smats;
+}
+
+// One element of one instance by the whole block (kElemThreads threads).
+__device__ __forceinline__ void element_state(const Frame2dArgs &A, int inst, int e, ElemLds &L, const MatDev *mats) {
+    const InstDev &I = A.insts[inst];
+    const FrameDev &F = A.frames[I.frame];
+    const ElemDev &el = A.elems[F.elem_base + e];
+    const int w = threadIdx.x >> 6;
+    const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
+                                  A.scratch + I.scratch_base + 42 * e, &L.sh, L.qbuf[w], L.scr[w], mats, L.wred[w]);
+    if (!ok && threadIdx.x == 0) A.eflag[inst] = 1;
 }
 
 // Element state determination of one (instance, element) pair per block.
 __global__ __launch_bounds__(kElemThreads) void frame2d_elem_kernel(Frame2dArgs A) {
-    __shared__ ElemShared sh;
-    __shared__ double qbuf[kMaxNipDev][3 + 2 * kMaxNipDev + 3];
-    __shared__ FbcScratch scr[kMaxNipDev];
-    __shared__ double wred[kMaxNipDev][5 * kWaveRedStride];
-    __shared__ MatDev smats[kMatsInLds];
+    __shared__ ElemLds L;
     const int p = blockIdx.x;
     const int inst = A.pair_inst[p], e = A.pair_elem[p];
     const InstState st = A.state[inst];
     const InstDev &I = A.insts[inst];
     if (st.status != 0 || st.step >= I.n_steps) return;  // finished or stopped (block-uniform)
-    const FrameDev &F = A.frames[I.frame];
-    const ElemDev &el = A.elems[F.elem_base + e];
-    const int w = threadIdx.x >> 6;
-    const MatDev *mats = A.mats;
-    if (A.n_mats <= kMatsInLds) {  // material parameters through LDS
-        const double *src = reinterpret_cast<const double *>(A.mats);
-        double *dst = reinterpret_cast<double *>(smats);
-        const int nd = A.n_mats * (int)(sizeof(MatDev) / sizeof(double));
-        for (int k = threadIdx.x; k < nd; k += blockDim.x) dst[k] = src[k];
-        __syncthreads();
-        mats = smats;
-    }
-    const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
-                                  A.scratch + I.scratch_base + 42 * e, &sh, qbuf[w], scr[w], mats, wred[w]);
-    if (!ok && threadIdx.x == 0) A.eflag[inst] = 1;
+    const MatDev *mats = stage_materials(A, L);
+    element_state(A, inst, e, L, mats);
 }
 
 // K_ff band (gather in element order) into `band`, F_int into `fint`.
@@ -577,12 +589,16 @@ __device__ void start_step(const Frame2dArgs &A, const InstDev &I, const FrameDe
     }
 }
 
-// One Newton iteration of one instance per block (after the element kernel).
-__global__ __launch_bounds__(256) void frame2d_sys_kernel(Frame2dArgs A) {
-    extern __shared__ double lds[];
-    __shared__ double s_norm, s_diag;
-    __shared__ int s_stat;
-    const int bid = blockIdx.x;
+struct SysShared {
+    double norm, diag;
+    int stat;
+};
+
+// One Newton iteration of instance `bid` by the whole block, after its
+// element states (the set-up assembly on an instance's first call).
+__device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double *lds, SysShared &ss) {
+    double &s_norm = ss.norm, &s_diag = ss.diag;
+    int &s_stat = ss.stat;
     const InstDev I = A.insts[bid];
     InstState st = A.state[bid];
     if (st.status != 0 || st.step >= I.n_steps) return;
@@ -608,7 +624,7 @@ __global__ __launch_bounds__(256) void frame2d_sys_kernel(Frame2dArgs A) {
         // initial assembly at u (transient_nonlinear.mojo:1207), K_init (:1329-1337), revert (:1452-1453)
         double *ki = A.kinit + I.kinit_base;
         for (int sl = tid; sl < n * bw; sl += nt) ki[sl] = band[sl];
-        copy_instance_state(A, I, false);
+        // the revert of :1452-1453 needs no work: trial fiber states are never stored
         start_step(A, I, F, 0, uf, vf, af, un, vn, an, P);
         __syncthreads();
         for (int i = tid; i < 7 * n; i += nt) gv[i] = uf[i];
@@ -686,7 +702,7 @@ __global__ __launch_bounds__(256) void frame2d_sys_kernel(Frame2dArgs A) {
         // converged: commit (:3081-3101), committed basic sync (helpers.mojo:1832-1883),
         // Newmark update (:594-647), recorder, then the next step's predictor
         if (tid == 0) A.iters[I.iters_base + step] = st.iter;
-        copy_instance_state(A, I, true);
+        commit_instance(A, I, F);
         for (int e = tid; e < F.n_elem; e += nt) {
             const ElemDev &el = A.elems[F.elem_base + e];
             double ug[6];
@@ -938,11 +954,51 @@ __global__ __launch_bounds__(256) void frame2d_sys_kernel(Frame2dArgs A) {
     if (tid == 0) A.state[bid] = st;
 }
 
+// One Newton iteration of one instance per block (after the element kernel).
+__global__ __launch_bounds__(256) void frame2d_sys_kernel(Frame2dArgs A) {
+    extern __shared__ double lds[];
+    __shared__ SysShared ss;
+    sys_round(A, blockIdx.x, lds, ss);
+}
+
+// Persistent schedule: one block per instance runs its own Newton loop
+// (elements one after another, their integration points in parallel) for up
+// to max_steps steps, without waiting for any other instance.
+__global__ __launch_bounds__(kElemThreads) void frame2d_inst_kernel(Frame2dArgs A, int max_steps) {
+    extern __shared__ double lds[];
+    __shared__ ElemLds L;
+    __shared__ SysShared ss;
+    const int bid = blockIdx.x;
+    const InstDev &I = A.insts[bid];
+    const FrameDev &F = A.frames[I.frame];
+    const MatDev *mats = stage_materials(A, L);
+    const int step0 = A.state[bid].step;
+    while (true) {
+        __syncthreads();  // the previous round's state is visible to every thread
+        const InstState st = A.state[bid];
+        if (st.status != 0 || st.step >= I.n_steps) break;
+        if (st.setup_done && st.step - step0 >= max_steps) break;
+        for (int e = 0; e < F.n_elem; ++e) element_state(A, bid, e, L, mats);
+        __syncthreads();
+        sys_round(A, bid, lds, ss);
+    }
+}
+
 size_t frame2d_lds_bytes(int n_free, int bw) { return sizeof(double) * ((size_t)13 * n_free + (size_t)n_free * bw); }
 
 hipError_t frame2d_set_lds(size_t lds_bytes) {
-    return hipFuncSetAttribute((const void *)frame2d_sys_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+    hipError_t e = hipFuncSetAttribute((const void *)frame2d_sys_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)frame2d_inst_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_bytes);
+}
+
+// Persistent schedule: every unfinished instance advances up to max_steps steps.
+hipError_t launch_frame2d_persistent(hipStream_t stream, const Frame2dArgs &args, int n_inst, int max_steps,
+                                     size_t lds_bytes) {
+    hipLaunchKernelGGL(frame2d_inst_kernel, dim3(n_inst), dim3(kElemThreads), lds_bytes, stream, args, max_steps);
+    return hipGetLastError();
 }
 
 // One round: a Newton iteration (or the set-up assembly) of every active instance.

@@ -3,6 +3,7 @@
 // adds (frame, ground motion) instances and launches frame2d.hip.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -18,6 +19,8 @@ size_t frame2d_lds_bytes(int n_free, int bw);
 hipError_t frame2d_set_lds(size_t lds_bytes);
 hipError_t launch_frame2d_round(hipStream_t stream, const Frame2dArgs &args, int n_pairs, int n_inst,
                                 size_t lds_bytes);
+hipError_t launch_frame2d_persistent(hipStream_t stream, const Frame2dArgs &args, int n_inst, int max_steps,
+                                     size_t lds_bytes);
 
 struct FrameDef {
     FrameDev dev{};
@@ -53,6 +56,12 @@ struct FrameHost {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_rounds = 0;
+    // schedule: requested (SGPU_SCHEDULE_*), the one in use, and the last
+    // measured rates of both (GM-steps/s)
+    int schedule = SGPU_SCHEDULE_AUTO;
+    int chosen = SGPU_SCHEDULE_ROUNDS;
+    double rate_rounds = 0.0, rate_persistent = 0.0;
+    int tunings = 0;
 };
 
 void frame2d_host_free(FrameHost *fh) {
@@ -122,6 +131,8 @@ int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *fram
         const SecDefHost &sd = ctx->defs[def];
         if (!sd.init_flex_valid)
             return fail(ctx, SGPU_ERR_ARG, "element %d: section initial flexibility is singular", e);
+        if ((int)sd.batch_chunk_begin.size() - 1 > 63)
+            return fail(ctx, SGPU_ERR_ARG, "element %d: sections of more than 63 x 64 slots are not on the frame path", e);
         bool any_nonlinear = false;
         for (size_t s = 0; s < sd.mat.size(); ++s)
             if (sd.mat[s] >= 0 && ctx->mats[sd.mat[s]].type != kElastic) any_nonlinear = true;
@@ -230,7 +241,7 @@ int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *fram
     fd.beta_k = d->beta_k;
     fd.beta_k_init = d->beta_k_init;
     const size_t lds = frame2d_lds_bytes(n, bw);
-    if (lds > 150 * 1024)
+    if (lds > 136 * 1024)  // the persistent kernel adds its static element LDS (about 19 KB)
         return fail(ctx, SGPU_ERR_ARG, "frame band of %d x %d does not fit the LDS of one workgroup", n, bw);
     *frame_id = (int32_t)fh.frames.size();
     fh.frames.push_back(std::move(f));
@@ -310,7 +321,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         lds = std::max(lds, frame2d_lds_bytes(d.n_free, d.bw));
     }
     std::vector<InstDev> idev;
-    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0, dense = 0, piv = 0;
+    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0, dense = 0, piv = 0, last = 0;
     fh.hist_base.clear();
     fh.iters_base.clear();
     fh.max_steps = 0;
@@ -335,6 +346,8 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         dense += (int64_t)d.n_free * d.n_free + (int64_t)d.n_free * d.bw;
         I.piv_base = piv;
         piv += d.n_free;
+        I.last_base = last;
+        last += 2 * kMaxNipDev * d.n_elem;
         I.hist_base = hist;
         hist += (int64_t)in.n_steps * d.n_record;
         I.iters_base = iters;
@@ -387,7 +400,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     std::vector<InstState> st(fh.insts.size(), InstState{0, 0, 0, 0, 0, 0});
     if ((rc = upload(ctx, &fh.d_state, st))) return rc;
     std::vector<double> zq(q, 0.0), zv(vec, 0.0);
-    double *d_q, *d_vec, *d_kinit, *d_scratch, *d_dense;
+    double *d_q, *d_vec, *d_kinit, *d_scratch, *d_dense, *d_last;
     int32_t *d_piv;
     if ((rc = upload(ctx, &d_q, zq))) return rc;
     if ((rc = upload(ctx, &d_vec, zv))) return rc;
@@ -395,6 +408,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     if ((rc = dalloc(ctx, &d_scratch, (size_t)scratch))) return rc;
     if ((rc = dalloc(ctx, &d_dense, (size_t)dense))) return rc;
     if ((rc = dalloc(ctx, &d_piv, (size_t)piv))) return rc;
+    if ((rc = upload(ctx, &d_last, std::vector<double>((size_t)last, 0.0)))) return rc;
     if ((rc = dalloc(ctx, &fh.d_hist, (size_t)hist))) return rc;
     if ((rc = dalloc(ctx, &fh.d_iters, (size_t)iters))) return rc;
     HIP_OK(ctx, hipMemset(fh.d_hist, 0, sizeof(double) * (size_t)(hist > 0 ? hist : 1)));
@@ -422,6 +436,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     A.scratch = d_scratch;
     A.dense = d_dense;
     A.piv = d_piv;
+    A.last = d_last;
     A.hist = fh.d_hist;
     A.iters = fh.d_iters;
     A.defs = ctx->d_defs;
@@ -469,26 +484,104 @@ int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds) {
     return SGPU_OK;
 }
 
+int sgpu_frame2d_launch_persistent(sgpu_ctx *ctx, int32_t max_steps) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (max_steps <= 0) return fail(ctx, SGPU_ERR_ARG, "max_steps must be > 0");
+    ProfScope ps(ctx, SGPU_KERNEL_FRAME2D);
+    HIP_OK(ctx, launch_frame2d_persistent(ctx->stream, fh.args, (int)fh.insts.size(), max_steps, fh.lds));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_set_schedule(sgpu_ctx *ctx, int schedule) {
+    if (!ctx) return SGPU_ERR_ARG;
+    if (schedule != SGPU_SCHEDULE_AUTO && schedule != SGPU_SCHEDULE_ROUNDS && schedule != SGPU_SCHEDULE_PERSISTENT)
+        return fail(ctx, SGPU_ERR_ARG, "unknown schedule %d", schedule);
+    FrameHost &fh = frames_of(ctx);
+    fh.schedule = schedule;
+    if (schedule != SGPU_SCHEDULE_AUTO) fh.chosen = schedule;
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out4) {
+    if (!ctx || !out4) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    out4[0] = fh.chosen;
+    out4[1] = fh.rate_rounds;
+    out4[2] = fh.rate_persistent;
+    out4[3] = fh.tunings;
+    return SGPU_OK;
+}
+
+namespace {
+
+// Launch one chunk of the given schedule, wait, and report the GM-steps the
+// instances advanced and whether any instance is still active.
+int run_chunk(sgpu_ctx *ctx, FrameHost &fh, int schedule, int rounds, std::vector<InstState> &st, int64_t &advanced,
+              int &active, double &seconds) {
+    int64_t before = 0;
+    for (const InstState &s : st) before += s.step;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = schedule == SGPU_SCHEDULE_PERSISTENT ? sgpu_frame2d_launch_persistent(ctx, rounds / 4 > 0 ? rounds / 4 : 1)
+                                                  : sgpu_frame2d_launch(ctx, rounds);
+    if (rc) return rc;
+    HIP_OK(ctx, hipMemcpyAsync(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int64_t after = 0;
+    active = 0;
+    for (size_t k = 0; k < st.size(); ++k) {
+        after += st[k].step;
+        if (st[k].status == 0 && st[k].step < fh.insts[k].n_steps) ++active;
+    }
+    advanced = after - before;
+    return SGPU_OK;
+}
+
+}  // namespace
+
 int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
     if (!ctx) return SGPU_ERR_ARG;
     FrameHost &fh = frames_of(ctx);
     if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
     if (rounds <= 0) return fail(ctx, SGPU_ERR_ARG, "rounds must be > 0");
     std::vector<InstState> st(fh.insts.size());
-    int64_t done_rounds = 0;
-    while (true) {
-        int rc = sgpu_frame2d_launch(ctx, rounds);
-        if (rc) return rc;
-        done_rounds += rounds;
-        HIP_OK(ctx, hipMemcpyAsync(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost,
-                                   ctx->stream));
-        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
-        bool active = false;
-        for (size_t k = 0; k < st.size(); ++k)
-            if (st[k].status == 0 && st[k].step < fh.insts[k].n_steps) active = true;
-        if (!active) break;
-        if (done_rounds > fh.max_rounds)
-            return fail(ctx, SGPU_ERR_STATE, "instances still active after %lld rounds", (long long)done_rounds);
+    HIP_OK(ctx, hipMemcpy(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost));
+    int active = 0;
+    for (size_t k = 0; k < st.size(); ++k)
+        if (st[k].status == 0 && st[k].step < fh.insts[k].n_steps) ++active;
+    // Autotuner: time one chunk of each schedule on the live ensemble (both
+    // advance the same per-instance state with the same arithmetic) and keep
+    // the faster; measure again when the active set has halved.
+    bool tune = fh.schedule == SGPU_SCHEDULE_AUTO;
+    int active_at_tune = active;
+    int64_t chunks = 0;
+    const int64_t max_chunks = 4 + 2 * fh.max_rounds / rounds;
+    while (active > 0) {
+        if (++chunks > max_chunks)
+            return fail(ctx, SGPU_ERR_STATE, "instances still active after %lld chunks", (long long)chunks);
+        int64_t adv = 0;
+        double sec = 0.0;
+        int rc;
+        if (tune) {
+            double rate[2] = {0.0, 0.0};
+            const int order[2] = {SGPU_SCHEDULE_ROUNDS, SGPU_SCHEDULE_PERSISTENT};
+            for (int k = 0; k < 2 && active > 0; ++k) {
+                if ((rc = run_chunk(ctx, fh, order[k], rounds, st, adv, active, sec))) return rc;
+                rate[k] = sec > 0.0 ? (double)adv / sec : 0.0;
+            }
+            fh.rate_rounds = rate[0];
+            fh.rate_persistent = rate[1];
+            fh.chosen = rate[1] > rate[0] ? SGPU_SCHEDULE_PERSISTENT : SGPU_SCHEDULE_ROUNDS;
+            fh.tunings += 1;
+            tune = false;
+            active_at_tune = active;
+            continue;
+        }
+        if ((rc = run_chunk(ctx, fh, fh.chosen, rounds, st, adv, active, sec))) return rc;
+        if (fh.schedule == SGPU_SCHEDULE_AUTO && active > 0 && 2 * active <= active_at_tune) tune = true;
     }
     return SGPU_OK;
 }

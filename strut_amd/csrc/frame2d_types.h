@@ -46,6 +46,7 @@ struct InstDev {
     int64_t q_base, vec_base, kinit_base, scratch_base, hist_base, iters_base;  // vec: 7 n (u v a un vn an P)
     int64_t dense_base;     // n*n dense fallback matrix followed by the n*bw band copy
     int64_t piv_base;       // n row permutation of the dense fallback
+    int64_t last_base;      // per element and integration point: last evaluated (eps0, kappa)
     int64_t slot_begin[4], slot_count[4], aux_begin[4], aux_count[4];  // fiber state ranges
 };
 
@@ -81,7 +82,7 @@ struct Frame2dArgs {
     const int32_t *pair_inst, *pair_elem;  // element-kernel block -> (instance, element)
     int32_t *eflag;                        // per instance: an element state failed this round
     const double *ag;
-    double *q, *vec, *kinit, *scratch, *hist, *dense;
+    double *q, *vec, *kinit, *scratch, *hist, *dense, *last;
     int32_t *iters, *piv;
     const SecDefDev *defs;
     SlotTables tb;

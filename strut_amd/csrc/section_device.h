@@ -183,62 +183,73 @@ __device__ __forceinline__ void wave_slot_load(const SecDefDev &d, const SlotTab
 // wave's LDS rows `red` (5 x kWaveRedStride doubles); lanes 0..4 add them
 // one after another onto their running sums, so the sums see the chunks in
 // the reference's order; every lane returns the same five sums.
+//
+// kTrial = false evaluates without storing the trial fiber states: a trial
+// state is a function of the committed state and the strain only, so a
+// caller that records the section's last deformation can rebuild it at
+// commit (commit_section_2d) and keeps the Newton loop free of stores.
+// The definition needs at most 63 batches (chunk offsets live in one lane
+// register each).
 struct Sec2dOut {
     double N, Mz, k11, k12, k22;
 };
 
+template <bool kTrial>
 __device__ __forceinline__ Sec2dOut wave_section_2d(const SecDefDev &d, const SlotTables &tb,
                                                     const MatDev *__restrict__ mats, const StateArrays &c,
                                                     const StateArrays &t, int inst, double eps0, double kappa,
                                                     double *red) {
     const int lane = threadIdx.x & 63;
+    const int n_batches = d.n_batches, n_el = d.n_elastic_chunks;
     const int64_t inst_state = d.state_base + (int64_t)inst * d.n_slots;
     const int64_t inst_aux = d.aux_base + (int64_t)inst * d.n_aux;
-    const int32_t *bcb = tb.batch_chunk_begin + d.batch_base;
+    // chunk offset of batch `lane`, read back with readlane
+    const int bcb_lane = lane <= n_batches ? tb.batch_chunk_begin[d.batch_base + lane] : 0;
     const bool zero_lane = d.zero_k12 && lane == 3;
     double acc = 0.0;  // lanes 0..4: running sum of N, Mz, k11, k12, k22
     WaveSlotIn cur;
     wave_slot_load(d, tb, c, inst_state, lane, cur);
-    for (int b = 0; b < d.n_batches; ++b) {
+    for (int b = 0; b < n_batches; ++b) {
+        // aux history of this batch's Steel02 / Concrete01 lanes, issued before the prefetch
+        const int mt = cur.mi >= 0 ? mats[cur.mi].type : kDead;
+        const int64_t ai = inst_aux + (cur.aux < 0 ? 0 : cur.aux);
+        FiberHist hc;
+        hc.eps = cur.eps;
+        hc.sig = cur.sig;
+        hc.tan = cur.tan;
+        hc.h0 = cur.h0;
+        hc.h1 = cur.h1;
+        hc.a0 = hc.a1 = hc.a2 = hc.a3 = hc.a4 = 0.0;
+        hc.kon = 0;
+        if (mt == kSteel02) {
+            hc.tan = 0.0;  // Steel02 never reads its committed tangent
+            hc.a0 = c.a0[ai];
+            hc.a1 = c.a1[ai];
+            hc.a2 = c.a2[ai];
+            hc.a3 = c.a3[ai];
+            hc.a4 = c.a4[ai];
+            hc.kon = c.kon[ai];
+        } else if (mt == kConcrete01) {
+            hc.a0 = c.a0[ai];
+        }
         WaveSlotIn nxt;
-        if (b + 1 < d.n_batches) wave_slot_load(d, tb, c, inst_state, (b + 1) * kBatch + lane, nxt);
+        if (b + 1 < n_batches) wave_slot_load(d, tb, c, inst_state, (b + 1) * kBatch + lane, nxt);
         double fs = 0.0, ks = 0.0;
         if (cur.mi >= 0) {
             const MatDev &m = mats[cur.mi];
             const double strain = eps0 - cur.y * kappa;
-            if (m.type == kElastic) {
+            if (mt == kElastic) {
                 ks = m.p[0] * cur.area;
                 fs = ks * strain;
             } else {
-                const int64_t gi = inst_state + b * kBatch + lane;
-                const int64_t ai = inst_aux + (cur.aux < 0 ? 0 : cur.aux);
-                FiberHist hc;
-                hc.eps = cur.eps;
-                hc.sig = cur.sig;
-                hc.tan = cur.tan;
-                hc.h0 = cur.h0;
-                hc.h1 = cur.h1;
-                hc.a0 = hc.a1 = hc.a2 = hc.a3 = hc.a4 = 0.0;
-                hc.kon = 0;
-                if (m.type == kSteel02) {
-                    hc.tan = 0.0;  // Steel02 never reads its committed tangent
-                    hc.a0 = c.a0[ai];
-                    hc.a1 = c.a1[ai];
-                    hc.a2 = c.a2[ai];
-                    hc.a3 = c.a3[ai];
-                    hc.a4 = c.a4[ai];
-                    hc.kon = c.kon[ai];
-                } else if (m.type == kConcrete01) {
-                    hc.a0 = c.a0[ai];
-                }
                 FiberHist ht = hc;
-                switch (m.type) {
+                switch (mt) {
                 case kConcrete02: concrete02_trial(m, hc, strain, ht); break;
                 case kSteel02: steel02_trial(m, hc, strain, ht); break;
                 case kSteel01: steel01_trial(m, hc, strain, ht); break;
                 default: concrete01_trial(m, hc, strain, ht); break;
                 }
-                store_hist(t, gi, ai, m.type, ht);
+                if (kTrial) store_hist(t, inst_state + b * kBatch + lane, ai, mt, ht);
                 fs = ht.sig * cur.area;
                 ks = ht.tan * cur.area;
             }
@@ -268,12 +279,12 @@ __device__ __forceinline__ Sec2dOut wave_section_2d(const SecDefDev &d, const Sl
             red[4 * kWaveRedStride + dst] = p4;
         }
         wave_sync();
+        const int c0 = __builtin_amdgcn_readlane(bcb_lane, b), c1 = __builtin_amdgcn_readlane(bcb_lane, b + 1);
         if (lane < 5) {
-            const int c0 = bcb[b], c1 = bcb[b + 1];
             const double *row = red + lane * kWaveRedStride - c0;
             int cc = c0;
-            if (zero_lane && d.n_elastic_chunks >= c0 && d.n_elastic_chunks < c1) {
-                for (; cc < d.n_elastic_chunks; ++cc) acc += row[cc];
+            if (zero_lane && n_el >= c0 && n_el < c1) {
+                for (; cc < n_el; ++cc) acc += row[cc];
                 acc = 0.0;  // uniform elastic modulus: k12 restarts with the nonlinear families
             }
 #pragma unroll 8
@@ -282,7 +293,7 @@ __device__ __forceinline__ Sec2dOut wave_section_2d(const SecDefDev &d, const Sl
         wave_sync();
         cur = nxt;
     }
-    if (zero_lane && bcb[d.n_batches] <= d.n_elastic_chunks) acc = 0.0;
+    if (zero_lane && __builtin_amdgcn_readlane(bcb_lane, n_batches) <= n_el) acc = 0.0;
     Sec2dOut o;
     o.N = readlane_d(acc, 0);
     o.Mz = readlane_d(acc, 1);
@@ -290,6 +301,38 @@ __device__ __forceinline__ Sec2dOut wave_section_2d(const SecDefDev &d, const Sl
     o.k12 = readlane_d(acc, 3);
     o.k22 = readlane_d(acc, 4);
     return o;
+}
+
+// Commit one 2D section instance by one wave: every fiber's law evaluated
+// from its committed state at the section's last deformation (eps0, kappa),
+// written back as the committed state.  Identical to copying the trial state
+// of that last evaluation (fiber2d.mojo:4059-4092).
+__device__ __forceinline__ void commit_section_2d(const SecDefDev &d, const SlotTables &tb,
+                                                  const MatDev *__restrict__ mats, const StateArrays &c, int inst,
+                                                  double eps0, double kappa) {
+    const int lane = threadIdx.x & 63;
+    const int64_t inst_state = d.state_base + (int64_t)inst * d.n_slots;
+    const int64_t inst_aux = d.aux_base + (int64_t)inst * d.n_aux;
+    for (int s = lane; s < d.n_slots; s += kBatch) {
+        const int64_t gs = d.slot_base + s;
+        const int mi = tb.mat[gs];
+        if (mi < 0) continue;
+        const MatDev &m = mats[mi];
+        if (m.type == kElastic) continue;
+        const int32_t aux = tb.aux[gs];
+        const int64_t gi = inst_state + s, ai = inst_aux + (aux < 0 ? 0 : aux);
+        FiberHist hc, ht;
+        load_hist(c, gi, ai, m.type, hc);
+        ht = hc;
+        const double strain = eps0 - tb.y[gs] * kappa;
+        switch (m.type) {
+        case kConcrete02: concrete02_trial(m, hc, strain, ht); break;
+        case kSteel02: steel02_trial(m, hc, strain, ht); break;
+        case kSteel01: steel01_trial(m, hc, strain, ht); break;
+        default: concrete01_trial(m, hc, strain, ht); break;
+        }
+        store_hist(c, gi, ai, m.type, ht);
+    }
 }
 
 }  // namespace sgpu

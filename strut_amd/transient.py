@@ -55,6 +55,9 @@ def _declare(lib):
         "sgpu_frame2d_result": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_double), POINTER(c_int32),
                                                POINTER(c_int32), POINTER(c_int32)]),
         "sgpu_frame2d_info": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_int32)]),
+        "sgpu_frame2d_launch_persistent": (ctypes.c_int, [c_void_p, c_int32]),
+        "sgpu_frame2d_set_schedule": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+        "sgpu_frame2d_schedule_info": (ctypes.c_int, [c_void_p, POINTER(c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -132,6 +135,22 @@ class FrameBatch:
         """Enqueue `rounds` rounds (one Newton iteration of every active
         instance each), replayed from a captured HIP graph."""
         self._check(self.lib.sgpu_frame2d_launch(self.rt.ctx, rounds))
+
+    def launch_persistent(self, max_steps: int) -> None:
+        """Enqueue the persistent schedule: every unfinished instance advances
+        up to max_steps steps on its own."""
+        self._check(self.lib.sgpu_frame2d_launch_persistent(self.rt.ctx, max_steps))
+
+    def set_schedule(self, schedule: int) -> None:
+        """_native.SCHEDULE_AUTO (timed choice), SCHEDULE_ROUNDS or SCHEDULE_PERSISTENT."""
+        self._check(self.lib.sgpu_frame2d_set_schedule(self.rt.ctx, schedule))
+
+    def schedule_info(self) -> dict:
+        out = np.zeros(4)
+        self._check(self.lib.sgpu_frame2d_schedule_info(self.rt.ctx, dptr(out)))
+        names = {0: "auto", 1: "rounds", 2: "persistent"}
+        return {"schedule": names.get(int(out[0]), "?"), "rounds_gm_steps_per_s": out[1],
+                "persistent_gm_steps_per_s": out[2], "tunings": int(out[3])}
 
     def run(self, rounds: int = 64) -> None:
         """Run until every instance has finished or stopped, checking after

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from strut_amd import cases
+from strut_amd import _native, cases
 from strut_amd.frame2d import load_frame2d_case
 from strut_amd.transient import FrameBatch
 
@@ -26,7 +26,11 @@ def _rel_err(u_gpu, u_ref):
     return float(np.max(np.abs(u_gpu - u_ref) / scale))
 
 
-def test_gpu_config0_plain_matches_oracle():
+SCHEDULES = [_native.SCHEDULE_ROUNDS, _native.SCHEDULE_PERSISTENT, _native.SCHEDULE_AUTO]
+
+
+@pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "auto"])
+def test_gpu_config0_plain_matches_oracle(schedule):
     fc = load_frame2d_case(cases.config0_case(steps=400, seed=0))
     ref = oracle.run_transient2d(fc)
     assert ref["status"] == 0
@@ -34,6 +38,7 @@ def test_gpu_config0_plain_matches_oracle():
     f = fb.add_frame(fc, record_all=True)
     i = fb.add_case_motion(f, fc)
     fb.finalize()
+    fb.set_schedule(schedule)
     fb.run(rounds=64)
     got = fb.result(i)
     fb.close()
@@ -45,7 +50,8 @@ def test_gpu_config0_plain_matches_oracle():
     assert err <= REL
 
 
-def test_gpu_ensemble_of_uneven_instances_matches_oracle():
+@pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "auto"])
+def test_gpu_ensemble_of_uneven_instances_matches_oracle(schedule):
     """Two frames x several motions of different lengths in one launch."""
     specs = [(3, 1, 0, 0.4, 260), (3, 1, 1, 0.8, 180), (4, 3, 2, 0.6, 140), (4, 3, 3, 0.3, 220)]
     fcs = [load_frame2d_case(cases.rc_frame2d_case(s, b, steps=n, dt=0.01, seed=seed, pga_g=pga, strong=3.0))
@@ -58,7 +64,8 @@ def test_gpu_ensemble_of_uneven_instances_matches_oracle():
             frame_of[(s, b)] = fb.add_frame(fc, record_all=True)
         ids.append(fb.add_case_motion(frame_of[(s, b)], fc))
     fb.finalize()
-    fb.run(rounds=64)
+    fb.set_schedule(schedule)
+    fb.run(rounds=16)
     for fc, iid in zip(fcs, ids):
         got = fb.result(iid)
         ref = oracle.run_transient2d(fc)
