@@ -300,3 +300,40 @@ def config0_case(steps: int = 1000, seed: int = 0) -> dict:
     """3 stories x 1 bay, 3 m / 6 m, 50 t floor masses, dt 0.01 s, PGA 0.4 g,
     2 s rise / 6 s strong / 2 s decay."""
     return rc_frame2d_case(3, 1, steps=steps, dt=0.01, seed=seed, pga_g=0.4, strong=6.0)
+
+
+# ------------------------------------------------ uneven ensemble (config 2)
+CONFIG2_FRAMES = ((4, 3), (8, 4), (12, 6))  # stories x bays of the three frame variants
+CONFIG2_PGA_GRID = tuple(0.1 * (15.0 ** (k / 15.0)) for k in range(16))  # 16-point geometric grid, 0.1..1.5 g
+
+
+def config2_motion_params(seed: int, dt: float = 0.01) -> dict:
+    """Parameters of one ground motion of the uneven ensemble: corner
+    frequency drawn uniformly from 2-15 Hz, strong-motion duration from
+    5-20 s, PGA from the 16-point geometric grid over 0.1-1.5 g; record
+    length = 2 s rise + strong + 2 s decay + 1 s of free vibration, clamped
+    to 1000-3000 steps."""
+    gen = splitmix64(seed ^ 0x5DEECE66D)
+    u = [((next(gen) >> 11) + 0.5) * (1.0 / 9007199254740992.0) for _ in range(3)]
+    f_corner = 2.0 + 13.0 * u[0]
+    strong = 5.0 + 15.0 * u[1]
+    pga_g = CONFIG2_PGA_GRID[min(15, int(u[2] * 16))]
+    steps = int(round((2.0 + strong + 2.0 + 1.0) / dt))
+    steps = max(1000, min(3000, steps))
+    return {"seed": seed, "f_corner": f_corner, "strong": strong, "pga_g": pga_g, "steps": steps, "dt": dt}
+
+
+def config2_motion(seed: int, dt: float = 0.01, max_steps: int = 3000) -> dict:
+    """config2_motion_params plus the record (values at t = 0, dt, ...);
+    max_steps caps the record length for short runs."""
+    m = config2_motion_params(seed, dt)
+    m["steps"] = min(m["steps"], max_steps)
+    m["values"] = ground_motion(seed, dt, m["steps"], m["pga_g"], f_hi=m["f_corner"], strong=m["strong"])
+    return m
+
+
+def config2_frame_case(stories: int, bays: int, steps: int = 1000) -> dict:
+    """A frame variant of the uneven ensemble (sections as in config 0);
+    its own record is a placeholder: instances bring config2_motion records."""
+    return rc_frame2d_case(stories, bays, steps=steps, dt=0.01, seed=0, pga_g=0.4,
+                           name=f"config2_frame_{stories}x{bays}")
