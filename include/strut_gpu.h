@@ -250,6 +250,70 @@ int sgpu_frame2d_result(sgpu_ctx *ctx, int32_t instance, double *u_hist, int32_t
 int sgpu_frame2d_info(sgpu_ctx *ctx, int32_t frame_id, int32_t *out4);
 
 /* ------------------------------------------------------------------
+ * Learned Newton predictor.  No counterpart in the reference: its Newmark
+ * predictor (solver/run_case/analysis/transient_nonlinear.mojo:553-590,
+ * :1757-1766) is the plain method and stays the fallback.
+ *
+ * The network: 4 hidden layers of 256 SiLU units and a linear output,
+ * bf16 weights and activations on the gfx950 matrix cores, FP32
+ * accumulation, FP32 master weights trained with Adam.  For a frame with n
+ * free DOFs, the features of step k are the last three displacement
+ * increments, v_k and a_k at every free DOF and the ground acceleration at
+ * t_k and t_{k+1} (n_in = 5 n + 2); the output is u_{k+1} minus the Newmark
+ * predictor (n_out = n).  Features and targets are normalised per column
+ * (x_n = (x - x_mean) * x_scale; y = y_n * y_scale + y_mean).
+ *
+ * Device pointers (x, y below) are [count][n_in] / [count][n_out] FP32
+ * arrays in the context's device memory.  A network belongs to its
+ * context and is destroyed before it.
+ * ------------------------------------------------------------------ */
+typedef struct sgpu_mlp sgpu_mlp;
+int sgpu_mlp_create(sgpu_ctx *ctx, int32_t n_in, int32_t n_out, uint64_t seed, sgpu_mlp **out);
+void sgpu_mlp_destroy(sgpu_mlp *m);
+int64_t sgpu_mlp_param_count(const sgpu_mlp *m);
+/* out8 = {n_in, n_out, padded n_in, padded n_out, hidden width, layers, parameters, Adam steps}. */
+int sgpu_mlp_layout(const sgpu_mlp *m, int32_t *out8);
+/* FP32 master weights, flat: per layer W [N][K] (row-major, padded) then b [N]. */
+int sgpu_mlp_get_params(sgpu_mlp *m, float *out);
+int sgpu_mlp_set_params(sgpu_mlp *m, const float *in);
+int sgpu_mlp_set_normalization(sgpu_mlp *m, const float *x_mean, const float *x_scale, const float *y_mean,
+                               const float *y_scale);
+/* out: x_mean[n_in], x_scale[n_in], y_mean[n_out], y_scale[n_out]. */
+int sgpu_mlp_get_normalization(sgpu_mlp *m, float *out);
+/* Per-column mean and 1/std (features) or std (targets) of device samples. */
+int sgpu_mlp_fit_normalization(sgpu_mlp *m, const float *x, const float *y, int64_t count);
+/* Forward and backward pass over `rows` device samples without an update:
+ * gradients of the mean squared error of the normalised outputs, in the
+ * layout of sgpu_mlp_get_params (host buffer), and that loss. */
+int sgpu_mlp_gradients(sgpu_mlp *m, const float *x, const float *y, int32_t rows, float *grad_out, double *loss);
+/* Adam over `epochs` passes of shuffled minibatches (batch a multiple of
+ * 16; the remainder of each pass is skipped); learning rate lr with a
+ * linear warm-up and cosine decay.  epoch_loss (host, may be NULL) receives
+ * each pass's mean squared error. */
+int sgpu_mlp_train(sgpu_mlp *m, const float *x, const float *y, int64_t count, int32_t batch, int32_t epochs,
+                   float lr, uint64_t seed, double *epoch_loss);
+/* Batched inference: de-normalised outputs of `count` device samples. */
+int sgpu_mlp_predict(sgpu_mlp *m, const float *x, int64_t count, float *y);
+
+/* Record every instance's (features, target) pair of each converged step
+ * on the device.  Chosen before sgpu_frame2d_finalize. */
+int sgpu_frame2d_record_samples(sgpu_ctx *ctx, int32_t on);
+/* Copy the recorded pairs of a frame's instances into device arrays x
+ * ([capacity][5 n + 2]) and y ([capacity][n]); *count receives the number
+ * of pairs (x or y NULL: count only). */
+int sgpu_frame2d_samples(sgpu_ctx *ctx, int32_t frame_id, float *x, float *y, int64_t capacity, int64_t *count);
+/* Use network m (a snapshot of its current weights; NULL: none) as the
+ * predictor of every instance of a frame.  Each step then starts from the
+ * network's proposal; the unchanged FP64 residual test decides convergence,
+ * and the step restarts from the Newmark predictor when the proposal's
+ * residual exceeds the first-order estimate of the residual at the Newmark
+ * predictor, R(u_p) + K_eff (u_p - u_nm), or an element state fails there. */
+int sgpu_frame2d_set_predictor(sgpu_ctx *ctx, int32_t frame_id, const sgpu_mlp *m);
+/* out4 = {Newton iterations, steps started from a proposal, proposals
+ * replaced by the Newmark predictor, dense-LU fallbacks} of an instance. */
+int sgpu_frame2d_inst_stats(sgpu_ctx *ctx, int32_t instance, int32_t *out4);
+
+/* ------------------------------------------------------------------
  * Per-kernel timing.  Mirrors the phase timers of Strut's --profile output
  * (phase_times_us.json, solver/run_case/helpers.mojo), per device kernel:
  * while enabled, every launch on the context's stream is bracketed by HIP

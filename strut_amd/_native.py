@@ -59,6 +59,22 @@ EXPORTED = (
     "sgpu_frame2d_schedule_info",
     "sgpu_profile_enable",
     "sgpu_profile_read",
+    "sgpu_mlp_create",
+    "sgpu_mlp_destroy",
+    "sgpu_mlp_param_count",
+    "sgpu_mlp_layout",
+    "sgpu_mlp_get_params",
+    "sgpu_mlp_set_params",
+    "sgpu_mlp_set_normalization",
+    "sgpu_mlp_get_normalization",
+    "sgpu_mlp_fit_normalization",
+    "sgpu_mlp_gradients",
+    "sgpu_mlp_train",
+    "sgpu_mlp_predict",
+    "sgpu_frame2d_record_samples",
+    "sgpu_frame2d_samples",
+    "sgpu_frame2d_set_predictor",
+    "sgpu_frame2d_inst_stats",
 )
 
 # Kernel ids of sgpu_profile_read.

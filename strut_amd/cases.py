@@ -337,3 +337,17 @@ def config2_frame_case(stories: int, bays: int, steps: int = 1000) -> dict:
     its own record is a placeholder: instances bring config2_motion records."""
     return rc_frame2d_case(stories, bays, steps=steps, dt=0.01, seed=0, pga_g=0.4,
                            name=f"config2_frame_{stories}x{bays}")
+
+
+def peak_story_drift(u_hist, stories: int, bays: int, story_h: float = 3.0):
+    """Peak inter-story drift ratio of each story (max over time and column
+    lines of |u_x(floor k) - u_x(floor k - 1)| / h) from a displacement
+    history over every DOF of an rc_frame2d_case frame (node f * (bays + 1) + c,
+    ux = DOF 3 * node)."""
+    import numpy as np
+
+    u = np.asarray(u_hist)
+    cols = bays + 1
+    ux = u[:, 0::3].reshape(u.shape[0], stories + 1, cols)
+    drift = np.abs(np.diff(ux, axis=1)) / story_h
+    return drift.max(axis=(0, 2))

@@ -589,9 +589,71 @@ __device__ void start_step(const Frame2dArgs &A, const InstDev &I, const FrameDe
     }
 }
 
+// Learned predictor and training-sample recorder.  The features of step n
+// (u_{n+1} from u_n) are the last three displacement increments, v_n and a_n
+// at every free DOF, and the ground acceleration at t_n and t_{n+1};
+// the target is u_{n+1} minus the Newmark predictor.
+__device__ __forceinline__ float step_feature(const double *ring, const double *vn, const double *an, double ag0,
+                                              double ag1, int n, int f) {
+    if (f < 3 * n) return (float)ring[f];
+    if (f < 4 * n) return (float)vn[f - 3 * n];
+    if (f < 5 * n) return (float)an[f - 4 * n];
+    return (float)(f == 5 * n ? ag0 : ag1);
+}
+
+// After start_step: record the step's features, and/or replace the Newmark
+// predictor in uf by the network's proposal.  Called by every thread.
+__device__ void learned_step_start(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, int step, double *uf,
+                                   const double *vn, const double *an, unsigned char *scratch, InstState &st) {
+    const int n = F.n_free, tid = threadIdx.x, nt = blockDim.x;
+    const double *ring = A.vec + I.vec_base + 7 * n;
+    const double ag1 = A.ag[I.ag_base + step];
+    const double ag0 = step > 0 ? A.ag[I.ag_base + step - 1] : 0.0;
+    const int n_in = 5 * n + 2;
+    if (A.samples && I.sample_base >= 0) {
+        float *x = A.samples + I.sample_base + (int64_t)step * n_in;
+        for (int f = tid; f < n_in; f += nt) x[f] = step_feature(ring, vn, an, ag0, ag1, n, f);
+    }
+    const MlpDev &m = A.preds[I.frame];
+    if (m.n_in != n_in) return;
+    uint16_t *feat = reinterpret_cast<uint16_t *>(scratch);
+    uint16_t *h0 = feat + m.k0 + kMlpPad;
+    uint16_t *h1 = h0 + kMlpHidden + kMlpPad;
+    float *out = reinterpret_cast<float *>(h1 + kMlpHidden + kMlpPad);
+    for (int f = tid; f < m.k0; f += nt) {
+        float xn = 0.0f;
+        if (f < n_in) xn = (step_feature(ring, vn, an, ag0, ag1, n, f) - m.x_mean[f]) * m.x_scale[f];
+        feat[f] = f32_to_bf16(xn);
+    }
+    __syncthreads();
+    mlp_infer_one(m, feat, h0, h1, out);
+    for (int i = tid; i < n; i += nt) uf[i] = uf[i] + (double)out[i];
+    __syncthreads();
+    st.proposed = 1;
+    st.proposals += 1;
+}
+
+// At convergence, before the next start_step: record the step's target and
+// shift the increment history.
+__device__ void learned_step_end(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, int step,
+                                 const double *uf, const double *un, const double *vn, const double *an) {
+    const int n = F.n_free;
+    const bool rec = A.samples && I.sample_base >= 0;
+    if (!rec && A.preds[I.frame].n_in == 0) return;
+    double *ring = A.vec + I.vec_base + 7 * n;
+    float *y = rec ? A.samples + I.sample_base + (int64_t)I.n_steps * (5 * n + 2) + (int64_t)step * n : nullptr;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (y) y[i] = (float)(uf[i] - (un[i] + F.dt * vn[i] + F.coeff * an[i]));
+        ring[2 * n + i] = ring[n + i];
+        ring[n + i] = ring[i];
+        ring[i] = uf[i] - un[i];
+    }
+}
+
 struct SysShared {
     double norm, diag;
     int stat;
+    int judge;
 };
 
 // One Newton iteration of instance `bid` by the whole block, after its
@@ -610,7 +672,19 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
     double *R = P + n, *du = R + n, *vt = du + n, *at = vt + n, *fint = at + n, *work = fint + n;
     double *band = work + n;  // n * bw
     const double *mass = A.mass_ff + F.free_base;
+    unsigned char *pscratch = reinterpret_cast<unsigned char *>(band);  // free at a step's start
     if (A.eflag[bid]) {  // an element state failed (the reference aborts)
+        if (st.proposed) {  // at the learned proposal: restart the step from the Newmark predictor
+            for (int i = tid; i < n; i += nt) gv[i] = gv[3 * n + i] + F.dt * gv[4 * n + i] + F.coeff * gv[5 * n + i];
+            if (tid == 0) {
+                A.eflag[bid] = 0;
+                st.iter += 1;
+                st.proposed = 0;
+                st.fallbacks += 1;
+                A.state[bid] = st;
+            }
+            return;
+        }
         if (tid == 0) {
             st.status = kInstElementFail;
             A.state[bid] = st;
@@ -627,6 +701,7 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         // the revert of :1452-1453 needs no work: trial fiber states are never stored
         start_step(A, I, F, 0, uf, vf, af, un, vn, an, P);
         __syncthreads();
+        learned_step_start(A, I, F, 0, uf, vn, an, pscratch, st);
         for (int i = tid; i < 7 * n; i += nt) gv[i] = uf[i];
         if (tid == 0) {
             st.setup_done = 1;
@@ -731,13 +806,16 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
             const int fi = A.rec_fidx[F.rec_base + r];
             h[r] = fi >= 0 ? uf[fi] : 0.0;
         }
+        learned_step_end(A, I, F, step, uf, un, vn, an);
         __syncthreads();
         st.total_iters += st.iter;
         st.iter = 0;
+        st.proposed = 0;
         st.step = step + 1;
         if (st.step < I.n_steps) {
             start_step(A, I, F, st.step, uf, vf, af, un, vn, an, P);
             __syncthreads();
+            learned_step_start(A, I, F, st.step, uf, vn, an, pscratch, st);
         }
         for (int i = tid; i < 7 * n; i += nt) gv[i] = uf[i];
         if (tid == 0) A.state[bid] = st;
@@ -772,6 +850,38 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         band[sl] = v;
     }
     __syncthreads();
+    if (st.proposed) {
+        // Judge of the learned proposal u_p: keep it unless its residual is
+        // above the first-order estimate of the residual at the Newmark
+        // predictor, R(u_nm) ~= R(u_p) + K_eff (u_p - u_nm).
+        for (int i = tid; i < n; i += nt) work[i] = uf[i] - (un[i] + F.dt * vn[i] + F.coeff * an[i]);
+        __syncthreads();
+        for (int i = tid; i < n; i += nt) {
+            double s = R[i];
+            const int j0 = i - hbw < 0 ? 0 : i - hbw, j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+            for (int j = j0; j <= j1; ++j) s += band[i * bw + (j - i + hbw)] * work[j];
+            du[i] = s;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double rp = 0.0, rn = 0.0;
+            for (int i = 0; i < n; ++i) {
+                rp += R[i] * R[i];
+                rn += du[i] * du[i];
+            }
+            ss.judge = rp > rn;
+        }
+        __syncthreads();
+        st.proposed = 0;
+        if (ss.judge) {  // the proposal raised the residual: restart from the Newmark predictor
+            for (int i = tid; i < n; i += nt) gv[i] = un[i] + F.dt * vn[i] + F.coeff * an[i];
+            if (tid == 0) {
+                st.fallbacks += 1;
+                A.state[bid] = st;
+            }
+            return;
+        }
+    }
     // BandGeneral LU without pivoting (lsb.mojo:631-679); the band as
     // loaded is kept for the dense fallback of :652-657.
     double *dense = A.dense + I.dense_base;
@@ -984,7 +1094,12 @@ __global__ __launch_bounds__(kElemThreads) void frame2d_inst_kernel(Frame2dArgs 
     }
 }
 
-size_t frame2d_lds_bytes(int n_free, int bw) { return sizeof(double) * ((size_t)13 * n_free + (size_t)n_free * bw); }
+size_t frame2d_lds_bytes(int n_free, int bw) {
+    const int k0 = (5 * n_free + 2 + 15) / 16 * 16, n5 = (n_free + 15) / 16 * 16;
+    const size_t infer = 2 * (size_t)(k0 + kMlpPad) + 2 * 2 * (size_t)(kMlpHidden + kMlpPad) + 4 * (size_t)n5;
+    const size_t band = sizeof(double) * (size_t)n_free * bw;
+    return sizeof(double) * (size_t)13 * n_free + (band > infer ? band : infer);
+}
 
 hipError_t frame2d_set_lds(size_t lds_bytes) {
     hipError_t e = hipFuncSetAttribute((const void *)frame2d_sys_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -14,7 +14,7 @@
 
 namespace sgpu {
 
-
+const MlpDev *mlp_device_view(const sgpu_mlp *m);
 size_t frame2d_lds_bytes(int n_free, int bw);
 hipError_t frame2d_set_lds(size_t lds_bytes);
 hipError_t launch_frame2d_round(hipStream_t stream, const Frame2dArgs &args, int n_pairs, int n_inst,
@@ -62,6 +62,11 @@ struct FrameHost {
     int chosen = SGPU_SCHEDULE_ROUNDS;
     double rate_rounds = 0.0, rate_persistent = 0.0;
     int tunings = 0;
+    // learned predictor (per frame) and training-sample recorder
+    MlpDev *d_preds = nullptr;
+    bool record_samples = false;
+    float *d_samples = nullptr;
+    std::vector<int64_t> sample_base;
 };
 
 void frame2d_host_free(FrameHost *fh) {
@@ -321,8 +326,9 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         lds = std::max(lds, frame2d_lds_bytes(d.n_free, d.bw));
     }
     std::vector<InstDev> idev;
-    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0, dense = 0, piv = 0, last = 0;
+    int64_t q = 0, vec = 0, kinit = 0, scratch = 0, hist = 0, iters = 0, dense = 0, piv = 0, last = 0, samples = 0;
     fh.hist_base.clear();
+    fh.sample_base.clear();
     fh.iters_base.clear();
     fh.max_steps = 0;
     for (InstHost &in : fh.insts) {
@@ -337,7 +343,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         I.q_base = q;
         q += d.q_len;
         I.vec_base = vec;
-        vec += 7 * d.n_free;
+        vec += 10 * d.n_free;
         I.kinit_base = kinit;
         kinit += (int64_t)d.n_free * d.bw;
         I.scratch_base = scratch;
@@ -352,6 +358,12 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         hist += (int64_t)in.n_steps * d.n_record;
         I.iters_base = iters;
         iters += in.n_steps;
+        I.sample_base = -1;
+        if (fh.record_samples) {
+            I.sample_base = samples;
+            samples += (int64_t)in.n_steps * (6 * d.n_free + 2);
+        }
+        fh.sample_base.push_back(I.sample_base);
         fh.hist_base.push_back(I.hist_base);
         fh.iters_base.push_back(I.iters_base);
         I.n_ranges = (int32_t)in.range_def.size();
@@ -397,7 +409,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &d_pi, pair_inst))) return rc;
     if ((rc = upload(ctx, &d_pe, pair_elem))) return rc;
     if ((rc = upload(ctx, &d_ef, std::vector<int32_t>(fh.insts.size(), 0)))) return rc;
-    std::vector<InstState> st(fh.insts.size(), InstState{0, 0, 0, 0, 0, 0});
+    std::vector<InstState> st(fh.insts.size(), InstState{});
     if ((rc = upload(ctx, &fh.d_state, st))) return rc;
     std::vector<double> zq(q, 0.0), zv(vec, 0.0);
     double *d_q, *d_vec, *d_kinit, *d_scratch, *d_dense, *d_last;
@@ -445,6 +457,13 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     A.n_mats = (int32_t)ctx->mats.size();
     A.c = ctx->c;
     A.t = ctx->t;
+    if ((rc = upload(ctx, &fh.d_preds, std::vector<MlpDev>(fh.frames.size(), MlpDev{})))) return rc;
+    A.preds = fh.d_preds;
+    A.samples = nullptr;
+    if (fh.record_samples) {
+        if ((rc = dalloc(ctx, &fh.d_samples, (size_t)samples))) return rc;
+        A.samples = fh.d_samples;
+    }
     fh.lds = lds;
     HIP_OK(ctx, frame2d_set_lds(lds));
     fh.finalized = true;
@@ -617,6 +636,97 @@ int sgpu_frame2d_info(sgpu_ctx *ctx, int32_t frame_id, int32_t *out4) {
     out4[1] = f.dev.hbw;
     out4[2] = f.dev.n_record;
     out4[3] = f.total_dofs;
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_record_samples(sgpu_ctx *ctx, int32_t on) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (fh.finalized) return fail(ctx, SGPU_ERR_STATE, "the sample recorder is chosen before sgpu_frame2d_finalize");
+    fh.record_samples = on != 0;
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_samples(sgpu_ctx *ctx, int32_t frame_id, float *x, float *y, int64_t capacity, int64_t *count) {
+    if (!ctx || !count) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized || !fh.record_samples) return fail(ctx, SGPU_ERR_STATE, "no sample recorder");
+    if (frame_id < 0 || frame_id >= (int)fh.frames.size()) return fail(ctx, SGPU_ERR_ARG, "frame out of range");
+    const int n = fh.frames[frame_id].dev.n_free, n_in = 5 * n + 2;
+    std::vector<InstState> st(fh.insts.size());
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, hipMemcpy(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost));
+    int64_t total = 0;
+    for (size_t k = 0; k < fh.insts.size(); ++k) {
+        if (fh.insts[k].frame != frame_id) continue;
+        const int64_t done = st[k].step;
+        if (x && y) {
+            if (total + done > capacity) return fail(ctx, SGPU_ERR_ARG, "sample capacity %lld too small", (long long)capacity);
+            const float *src = fh.d_samples + fh.sample_base[k];
+            HIP_OK(ctx, hipMemcpyAsync(x + total * n_in, src, sizeof(float) * (size_t)(done * n_in),
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_OK(ctx, hipMemcpyAsync(y + total * n, src + (int64_t)fh.insts[k].n_steps * n_in,
+                                       sizeof(float) * (size_t)(done * n), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        total += done;
+    }
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    *count = total;
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_set_predictor(sgpu_ctx *ctx, int32_t frame_id, const sgpu_mlp *m) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (frame_id < 0 || frame_id >= (int)fh.frames.size()) return fail(ctx, SGPU_ERR_ARG, "frame out of range");
+    MlpDev snap{};
+    if (m) {
+        const MlpDev &src = *mlp_device_view(m);
+        const int n = fh.frames[frame_id].dev.n_free;
+        if (src.n_in != 5 * n + 2 || src.n_out != n)
+            return fail(ctx, SGPU_ERR_ARG, "network sizes %d -> %d do not match the frame (%d -> %d)", src.n_in,
+                        src.n_out, 5 * n + 2, n);
+        // a copy owned by this context: the network may be trained further or destroyed
+        snap = src;
+        int rc;
+        for (int l = 0; l < kMlpLayers; ++l) {
+            const size_t nk = (size_t)mlp_n(src, l) * mlp_k(src, l);
+            uint16_t *w;
+            float *b;
+            if ((rc = dalloc(ctx, &w, nk)) || (rc = dalloc(ctx, &b, (size_t)mlp_n(src, l)))) return rc;
+            HIP_OK(ctx, hipMemcpy(w, src.w[l], sizeof(uint16_t) * nk, hipMemcpyDeviceToDevice));
+            HIP_OK(ctx, hipMemcpy(b, src.b[l], sizeof(float) * (size_t)mlp_n(src, l), hipMemcpyDeviceToDevice));
+            snap.w[l] = w;
+            snap.wt[l] = nullptr;
+            snap.b[l] = b;
+        }
+        float *nrm;
+        const size_t nn = 2 * (size_t)src.n_in + 2 * (size_t)src.n_out;
+        if ((rc = dalloc(ctx, &nrm, nn))) return rc;
+        HIP_OK(ctx, hipMemcpy(nrm, src.x_mean, sizeof(float) * nn, hipMemcpyDeviceToDevice));
+        snap.x_mean = nrm;
+        snap.x_scale = nrm + src.n_in;
+        snap.y_mean = nrm + 2 * src.n_in;
+        snap.y_scale = nrm + 2 * src.n_in + src.n_out;
+    }
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, hipMemcpy(fh.d_preds + frame_id, &snap, sizeof(MlpDev), hipMemcpyHostToDevice));
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_inst_stats(sgpu_ctx *ctx, int32_t instance, int32_t *out4) {
+    if (!ctx || !out4) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (instance < 0 || instance >= (int)fh.insts.size()) return fail(ctx, SGPU_ERR_ARG, "instance out of range");
+    InstState st;
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, hipMemcpy(&st, fh.d_state + instance, sizeof(st), hipMemcpyDeviceToHost));
+    out4[0] = st.total_iters;
+    out4[1] = st.proposals;
+    out4[2] = st.fallbacks;
+    out4[3] = st.dense_fallbacks;
     return SGPU_OK;
 }
 

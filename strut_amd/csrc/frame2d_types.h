@@ -3,6 +3,7 @@
 
 #include <stdint.h>
 
+#include "mlp_device.h"
 #include "strut_gpu_types.h"
 
 namespace sgpu {
@@ -43,7 +44,9 @@ struct InstDev {
     int32_t n_ranges, pad_;
     int64_t ag_base;        // ground acceleration at t = (step + 1) dt
     int64_t elem_inst_base; // section instance of integration point 0, per element
-    int64_t q_base, vec_base, kinit_base, scratch_base, hist_base, iters_base;  // vec: 7 n (u v a un vn an P)
+    int64_t q_base, vec_base, kinit_base, scratch_base, hist_base, iters_base;
+    // vec: 10 n = u v a un vn an P, then the last three displacement increments
+    int64_t sample_base;    // training samples (features then targets per step) into samples, or -1
     int64_t dense_base;     // n*n dense fallback matrix followed by the n*bw band copy
     int64_t piv_base;       // n row permutation of the dense fallback
     int64_t last_base;      // per element and integration point: last evaluated (eps0, kappa)
@@ -57,6 +60,10 @@ struct InstState {
     int32_t total_iters;
     int32_t dense_fallbacks;  // BandGeneral factorizations that switched to the dense LU
     int32_t iter;       // assemblies so far in the current step
+    int32_t proposed;   // the current iterate is the learned predictor's proposal (not yet judged)
+    int32_t proposals;  // steps started from a learned proposal
+    int32_t fallbacks;  // proposals replaced by the Newmark predictor
+    int32_t pad_;
 };
 
 enum InstStatus : int32_t {
@@ -89,6 +96,8 @@ struct Frame2dArgs {
     const MatDev *mats;
     int32_t n_mats;
     StateArrays c, t;
+    const MlpDev *preds;  // per frame: learned predictor (n_in == 0: the plain Newmark predictor)
+    float *samples;       // training-sample recorder, or null
 };
 
 }  // namespace sgpu

@@ -38,6 +38,7 @@ class SectionRuntime:
 
     def __init__(self, materials: MaterialTable, device: int = 0):
         self.lib = _native.load()
+        self.device = device
         self.ctx = c_void_p()
         rc = self.lib.sgpu_ctx_create(device, ctypes.byref(self.ctx))
         if rc != 0:

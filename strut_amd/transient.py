@@ -58,6 +58,11 @@ def _declare(lib):
         "sgpu_frame2d_launch_persistent": (ctypes.c_int, [c_void_p, c_int32]),
         "sgpu_frame2d_set_schedule": (ctypes.c_int, [c_void_p, ctypes.c_int]),
         "sgpu_frame2d_schedule_info": (ctypes.c_int, [c_void_p, POINTER(c_double)]),
+        "sgpu_frame2d_record_samples": (ctypes.c_int, [c_void_p, c_int32]),
+        "sgpu_frame2d_samples": (ctypes.c_int, [c_void_p, c_int32, c_void_p, c_void_p, ctypes.c_int64,
+                                                POINTER(ctypes.c_int64)]),
+        "sgpu_frame2d_set_predictor": (ctypes.c_int, [c_void_p, c_int32, c_void_p]),
+        "sgpu_frame2d_inst_stats": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -174,6 +179,37 @@ class FrameBatch:
         out = np.zeros(4, dtype=np.int32)
         self._check(self.lib.sgpu_frame2d_info(self.rt.ctx, frame_id, iptr(out)))
         return {"free": int(out[0]), "hbw": int(out[1]), "record": int(out[2]), "total_dofs": int(out[3])}
+
+    def record_samples(self, on: bool = True) -> None:
+        """Record every converged step's (features, target) pair on the
+        device, as training data of the learned predictor (before finalize)."""
+        self._check(self.lib.sgpu_frame2d_record_samples(self.rt.ctx, 1 if on else 0))
+
+    def samples(self, frame_id: int):
+        """The recorded pairs of a frame's instances as device tensors
+        x [count, 5 n + 2] and y [count, n] (float32)."""
+        import torch
+        cnt = ctypes.c_int64()
+        self._check(self.lib.sgpu_frame2d_samples(self.rt.ctx, frame_id, None, None, 0, ctypes.byref(cnt)))
+        n = self.info(frame_id)["free"]
+        dev = torch.device("cuda", self.rt.device)
+        x = torch.empty((max(cnt.value, 1), 5 * n + 2), dtype=torch.float32, device=dev)
+        y = torch.empty((max(cnt.value, 1), n), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize(dev)
+        self._check(self.lib.sgpu_frame2d_samples(self.rt.ctx, frame_id, c_void_p(x.data_ptr()),
+                                                  c_void_p(y.data_ptr()), cnt.value, ctypes.byref(cnt)))
+        return x[:cnt.value], y[:cnt.value]
+
+    def set_predictor(self, frame_id: int, mlp) -> None:
+        """Start every step of this frame's instances from the network's
+        proposal (strut_amd.learned.Mlp; None: the plain Newmark predictor)."""
+        self._check(self.lib.sgpu_frame2d_set_predictor(self.rt.ctx, frame_id, None if mlp is None else mlp.h))
+
+    def inst_stats(self, inst: int) -> dict:
+        out = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.sgpu_frame2d_inst_stats(self.rt.ctx, inst, iptr(out)))
+        return {"iters": int(out[0]), "proposals": int(out[1]), "fallbacks": int(out[2]),
+                "dense_fallbacks": int(out[3])}
 
     def close(self) -> None:
         self.rt.close()
