@@ -22,6 +22,22 @@ Two legs, one JSON line.
    and N records batched in one launch on the same GPU, plus the oracle's
    single-core rate for one record.
 
+3. The uneven ensemble (``config.ensemble``): the three frame variants
+   (4 x 3, 8 x 4 and 12 x 6 bays, sections as above) x N ground motions from
+   the ensemble generator (corner frequency 2-15 Hz, strong-motion duration
+   5-20 s, PGA on the log grid 0.1-1.5 g), each record capped at
+   --ens-steps steps, all in one FrameBatch (autotuned schedule), against
+   the same records solved one by one with the plain path (one record of
+   each frame timed alone; the one-by-one time is the sum of each record's
+   completed steps over its frame's single-record rate).
+
+4. The learned Newton predictor (``config.learned``): on the 4 x 3 frame, a
+   network trained on the device from the pairs the solver records on one
+   seed range, then held-out records of another seed range solved with the
+   plain Newmark predictor and with the learned one: Newton iterations per
+   step, GM-steps/s and the peak inter-story drift error against the plain
+   FP64 solution.
+
 `--gpus N` with N > 1 exits: this build measures one GPU.
 """
 
@@ -245,6 +261,124 @@ def gm_cpu_baseline(steps: int) -> dict:
             "sample": f"one record (seed 0), {r['steps_done']} steps, {dt:.2f} s, oracle C restatement, 1 thread"}
 
 
+def _motion_case(frame: dict, seed: int, max_steps: int):
+    from strut_amd import cases
+    from strut_amd.frame2d import load_frame2d_case
+
+    m = cases.config2_motion(seed, max_steps=max_steps)
+    c = dict(frame)
+    c["time_series"] = [{"type": "Path", "tag": 1, "dt": m["dt"], "factor": 1.0,
+                         "values": [float(v) for v in m["values"]]}]
+    c["analysis"] = dict(frame["analysis"], steps=m["steps"])
+    return load_frame2d_case(c)
+
+
+def _run_batch(fcs_by_frame, record=False, predictor=None, record_all=False, schedule=None):
+    """One FrameBatch over {frame index: [cases]}; returns (batch, ids, seconds)."""
+    from strut_amd import _native
+    from strut_amd.transient import FrameBatch
+
+    first = next(iter(fcs_by_frame.values()))[0]
+    fb = FrameBatch(first.materials)
+    ids, fids = [], {}
+    for k, fcs in fcs_by_frame.items():
+        fids[k] = fb.add_frame(fcs[0], record_all=record_all)
+        ids += [(k, fb.add_case_motion(fids[k], fc)) for fc in fcs]
+    if record:
+        fb.record_samples(True)
+    fb.finalize()
+    if predictor is not None:
+        for k, net in predictor.items():
+            fb.set_predictor(fids[k], net)
+    fb.set_schedule(_native.SCHEDULE_AUTO if schedule is None else schedule)
+    t0 = time.perf_counter()
+    fb.run(rounds=16)
+    return fb, fids, ids, time.perf_counter() - t0
+
+
+def ensemble_leg(n_motions: int, max_steps: int) -> dict:
+    """The uneven three-frame ensemble batched vs its records one by one."""
+    from strut_amd import cases
+
+    frames = [cases.config2_frame_case(s, b, steps=100) for s, b in cases.CONFIG2_FRAMES]
+    by_frame = {k: [_motion_case(f, 100000 + 1000 * k + j, max_steps) for j in range(n_motions)]
+                for k, f in enumerate(frames)}
+    fb, _, ids, sec = _run_batch(by_frame)
+    res = [fb.result(i) for _, i in ids]
+    sched = fb.schedule_info()
+    fb.close()
+    done = [r["steps_done"] for r in res]
+    # plain path: one record of each frame alone (its first motion, up to 300 steps)
+    single = {}
+    for k, f in enumerate(frames):
+        one = {k: [_motion_case(f, 100000 + 1000 * k, min(max_steps, 300))]}
+        fb1, _, ids1, s1 = _run_batch(one)
+        r1 = fb1.result(ids1[0][1])
+        fb1.close()
+        single[k] = r1["steps_done"] / s1
+    one_by_one = sum(d / single[k] for (k, _), d in zip(ids, done))
+    return {"records": len(ids), "frames": [f"{s}x{b}" for s, b in cases.CONFIG2_FRAMES], "max_steps": max_steps,
+            "gm_steps": int(sum(done)), "seconds": sec, "gm_steps_per_s": sum(done) / sec,
+            "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0)),
+            "plain_single_record_gm_steps_per_s": {f"{s}x{b}": single[k]
+                                                   for k, (s, b) in enumerate(cases.CONFIG2_FRAMES)},
+            "one_by_one_seconds_est": one_by_one, "batched_vs_one_by_one": one_by_one / sec, "schedule": sched}
+
+
+def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
+    """Learned predictor on the 4 x 3 frame: train on one seed range, compare
+    with the plain predictor on held-out records."""
+    from strut_amd import cases
+    from strut_amd.learned import train_predictor
+
+    frame = cases.config2_frame_case(4, 3, steps=100)
+    train = {0: [_motion_case(frame, 200000 + j, max_steps) for j in range(n_train)]}
+    fb, fids, _, gen_s = _run_batch(train, record=True)
+    x, y = fb.samples(fids[0])
+    fb.close()
+    t0 = time.perf_counter()
+    net, losses = train_predictor(x, y, seed=7, epochs=epochs, batch=512, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    held = {0: [_motion_case(frame, 300000 + j, max_steps) for j in range(n_test)]}
+    out = {"frame": "4x3", "training_pairs": int(x.shape[0]), "training_records": n_train,
+           "pair_generation_seconds": gen_s, "epochs": epochs, "training_seconds": train_s,
+           "loss_first_last": [float(losses[0]), float(losses[-1])], "held_out_records": n_test}
+    del x, y
+    runs = {}
+    for label, pred in (("plain", None), ("learned", {0: net})):
+        fb, _, ids, sec = _run_batch(held, predictor=pred, record_all=True)
+        runs[label] = ([fb.result(i) for _, i in ids], [fb.inst_stats(i) for _, i in ids], sec)
+        fb.close()
+    net.close()
+    (rp, _, sp), (rl, stl, sl) = runs["plain"], runs["learned"]
+    same = [a["status"] == b["status"] and a["steps_done"] == b["steps_done"] for a, b in zip(rp, rl)]
+    it_p = sum(int(a["iters"][:a["steps_done"]].sum()) for a in rp)
+    it_l = sum(int(b["iters"][:b["steps_done"]].sum()) for b in rl)
+    steps_p = sum(a["steps_done"] for a in rp)
+    steps_l = sum(b["steps_done"] for b in rl)
+    errs, errs_stalled = [], []
+    for a, b, ok in zip(rp, rl, same):
+        n = a["steps_done"]
+        if not ok or n == 0:
+            continue
+        dp = cases.peak_story_drift(a["u"][:n], 4, 3)
+        dl = cases.peak_story_drift(b["u"][:n], 4, 3)
+        e = float(np.max(np.abs(dl - dp) / dp))
+        (errs_stalled if int(a["iters"][:n].max()) > 10 else errs).append(e)
+    out.update({
+        "newton_iters_per_step_plain": it_p / steps_p, "newton_iters_per_step_learned": it_l / steps_l,
+        "newton_iter_reduction": 1.0 - (it_l / steps_l) / (it_p / steps_p),
+        "gm_steps_per_s_plain": steps_p / sp, "gm_steps_per_s_learned": steps_l / sl,
+        "proposals": int(sum(s["proposals"] for s in stl)), "fallbacks": int(sum(s["fallbacks"] for s in stl)),
+        "records_same_stop": int(sum(same)),
+        "peak_drift_rel_err_max": max(errs) if errs else None,
+        "records_within_1e-6": int(sum(1 for e in errs if e <= 1e-6)), "records_compared": len(errs),
+        "records_with_stalled_step": len(errs_stalled),
+        "stalled_records_drift_rel_err_max": max(errs_stalled) if errs_stalled else None,
+    })
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +387,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gm-batch", type=int, default=256, help="records in the batched GM-steps leg (0 = skip)")
     ap.add_argument("--gm-steps", type=int, default=1000)
+    ap.add_argument("--ens-motions", type=int, default=32, help="motions per frame in the ensemble leg (0 = skip)")
+    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--learned-train", type=int, default=256, help="training records of the learned leg (0 = skip)")
+    ap.add_argument("--learned-test", type=int, default=64)
+    ap.add_argument("--learned-steps", type=int, default=800)
+    ap.add_argument("--learned-epochs", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -280,6 +420,11 @@ def main() -> int:
         config["gm_steps"] = {"workload": f"2D RC frame 3 stories x 1 bay, forceBeamColumn 5 IPs, {args.gm_steps} "
                                           "steps dt 0.01 s, PGA 0.4 g seeded ground motions (seeds 0..N-1)",
                               **gm}
+    if args.ens_motions > 0:
+        config["ensemble"] = ensemble_leg(args.ens_motions, args.ens_steps)
+    if args.learned_train > 0:
+        config["learned"] = learned_leg(args.learned_train, args.learned_test, args.learned_steps,
+                                        args.learned_epochs)
     out = {
         "metric": "fiber updates/s",
         "value": sec["value"],

@@ -228,14 +228,23 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds);
  * the two schedules can alternate within one run with identical results. */
 int sgpu_frame2d_launch_persistent(sgpu_ctx *ctx, int32_t max_steps);
 
+/* Work-queue schedule: persistent worker blocks (as many as fit on the
+ * device) take element and system tasks from a device ring, so each
+ * instance runs its own Newton loop with its elements in parallel and no
+ * instance waits for another; every unfinished instance advances up to
+ * max_steps steps.  Same per-instance arithmetic as the other schedules. */
+int sgpu_frame2d_launch_queue(sgpu_ctx *ctx, int32_t max_steps);
+
 /* Schedule used by sgpu_frame2d_run.  AUTO (the default) times a chunk of
- * each schedule on the live ensemble, keeps the faster one and measures
- * again whenever the number of active instances has halved.
- * sgpu_frame2d_schedule_info: out4 = {schedule in use, last measured
- * GM-steps/s of the rounds, of the persistent schedule, tunings}. */
-enum { SGPU_SCHEDULE_AUTO = 0, SGPU_SCHEDULE_ROUNDS = 1, SGPU_SCHEDULE_PERSISTENT = 2 };
+ * the rounds and of the work queue on the live ensemble, keeps the faster
+ * and measures again whenever the number of active instances has halved and
+ * every 64 chunks.
+ * sgpu_frame2d_schedule_info: out5 = {schedule in use, last measured
+ * GM-steps/s of the rounds, of the persistent schedule, tunings, of the
+ * work-queue schedule}. */
+enum { SGPU_SCHEDULE_AUTO = 0, SGPU_SCHEDULE_ROUNDS = 1, SGPU_SCHEDULE_PERSISTENT = 2, SGPU_SCHEDULE_QUEUE = 3 };
 int sgpu_frame2d_set_schedule(sgpu_ctx *ctx, int schedule);
-int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out4);
+int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out5);
 
 /* Per-step recorded displacements (n_steps x n_record), per-step Newton
  * iteration counts (assemblies, including the converged one), status

@@ -75,6 +75,7 @@ EXPORTED = (
     "sgpu_frame2d_samples",
     "sgpu_frame2d_set_predictor",
     "sgpu_frame2d_inst_stats",
+    "sgpu_frame2d_launch_queue",
 )
 
 # Kernel ids of sgpu_profile_read.
@@ -87,6 +88,7 @@ KERNEL_FRAME2D = 3
 SCHEDULE_AUTO = 0
 SCHEDULE_ROUNDS = 1
 SCHEDULE_PERSISTENT = 2
+SCHEDULE_QUEUE = 3
 
 
 class SectionPoint(ctypes.Structure):
@@ -127,6 +129,15 @@ def load() -> ctypes.CDLL:
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `make -C strut_amd/csrc` "
             "(or __graft_entry__.build()); the fiber-section path has no CPU fallback")
+    # torch ships its own HIP and HSA runtimes under the same sonames as the
+    # system ones.  Loading the library first would bring the system runtime
+    # in, and torch's later would be a second runtime in the process (its
+    # device initialisation then fails).  With torch loaded first, the
+    # library binds to the runtime already present.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     dp = POINTER(c_double)
     ip = POINTER(c_int32)

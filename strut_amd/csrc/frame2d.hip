@@ -530,11 +530,23 @@ __device__ __forceinline__ const MatDev *stage_materials(const Frame2dArgs &A, E
 }
 
 // One element of one instance by the whole block (kElemThreads threads).
-__device__ __forceinline__ void element_state(const Frame2dArgs &A, int inst, int e, ElemLds &L, const MatDev *mats) {
+// When the instance's last converged step is still to be committed, the
+// element first commits its own sections, one wavefront per integration
+// point (transient_nonlinear.mojo:3081-3101).
+__device__ __forceinline__ void element_state(const Frame2dArgs &A, int inst, int e, ElemLds &L, const MatDev *mats,
+                                              bool commit_first) {
     const InstDev &I = A.insts[inst];
     const FrameDev &F = A.frames[I.frame];
     const ElemDev &el = A.elems[F.elem_base + e];
     const int w = threadIdx.x >> 6;
+    if (commit_first) {
+        if (w < el.nip) {
+            const double *last = A.last + I.last_base + 2 * kMaxNipDev * e + 2 * w;
+            commit_section_2d(A.defs[el.def], A.tb, A.mats, A.c, A.elem_inst[I.elem_inst_base + e] + w, last[0],
+                              last[1]);
+        }
+        __syncthreads();
+    }
     const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
                                   A.scratch + I.scratch_base + 42 * e, &L.sh, L.qbuf[w], L.scr[w], mats, L.wred[w]);
     if (!ok && threadIdx.x == 0) A.eflag[inst] = 1;
@@ -549,7 +561,7 @@ __global__ __launch_bounds__(kElemThreads) void frame2d_elem_kernel(Frame2dArgs 
     const InstDev &I = A.insts[inst];
     if (st.status != 0 || st.step >= I.n_steps) return;  // finished or stopped (block-uniform)
     const MatDev *mats = stage_materials(A, L);
-    element_state(A, inst, e, L, mats);
+    element_state(A, inst, e, L, mats, st.commit_pending != 0);
 }
 
 // K_ff band (gather in element order) into `band`, F_int into `fint`.
@@ -601,21 +613,13 @@ __device__ __forceinline__ float step_feature(const double *ring, const double *
     return (float)(f == 5 * n ? ag0 : ag1);
 }
 
-// After start_step: record the step's features, and/or replace the Newmark
-// predictor in uf by the network's proposal.  Called by every thread.
-__device__ void learned_step_start(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, int step, double *uf,
-                                   const double *vn, const double *an, unsigned char *scratch, InstState &st) {
-    const int n = F.n_free, tid = threadIdx.x, nt = blockDim.x;
-    const double *ring = A.vec + I.vec_base + 7 * n;
-    const double ag1 = A.ag[I.ag_base + step];
-    const double ag0 = step > 0 ? A.ag[I.ag_base + step - 1] : 0.0;
-    const int n_in = 5 * n + 2;
-    if (A.samples && I.sample_base >= 0) {
-        float *x = A.samples + I.sample_base + (int64_t)step * n_in;
-        for (int f = tid; f < n_in; f += nt) x[f] = step_feature(ring, vn, an, ag0, ag1, n, f);
-    }
-    const MlpDev &m = A.preds[I.frame];
-    if (m.n_in != n_in) return;
+// The network's proposal u_nm + y in uf (uf holds u_nm).  Out of line: it
+// runs once per step, and inlined its matrix-core registers would add to the
+// element loop's in the kernels that contain both.
+__device__ __noinline__ void learned_propose(const MlpDev &m, int n, const double *ring, const double *vn,
+                                             const double *an, double ag0, double ag1, double *uf,
+                                             unsigned char *scratch) {
+    const int tid = threadIdx.x, nt = blockDim.x, n_in = 5 * n + 2;
     uint16_t *feat = reinterpret_cast<uint16_t *>(scratch);
     uint16_t *h0 = feat + m.k0 + kMlpPad;
     uint16_t *h1 = h0 + kMlpHidden + kMlpPad;
@@ -629,6 +633,23 @@ __device__ void learned_step_start(const Frame2dArgs &A, const InstDev &I, const
     mlp_infer_one(m, feat, h0, h1, out);
     for (int i = tid; i < n; i += nt) uf[i] = uf[i] + (double)out[i];
     __syncthreads();
+}
+
+// After start_step: record the step's features, and/or replace the Newmark
+// predictor in uf by the network's proposal.  Called by every thread.
+__device__ void learned_step_start(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, int step, double *uf,
+                                   const double *vn, const double *an, unsigned char *scratch, InstState &st) {
+    const int n = F.n_free, tid = threadIdx.x, nt = blockDim.x;
+    const double *ring = A.vec + I.vec_base + 7 * n;
+    const double ag1 = A.ag[I.ag_base + step];
+    const double ag0 = step > 0 ? A.ag[I.ag_base + step - 1] : 0.0;
+    const int n_in = 5 * n + 2;
+    if (A.samples && I.sample_base >= 0) {
+        float *x = A.samples + I.sample_base + (int64_t)step * n_in;
+        for (int f = tid; f < n_in; f += nt) x[f] = step_feature(ring, vn, an, ag0, ag1, n, f);
+    }
+    if (A.preds[I.frame].n_in != n_in) return;
+    learned_propose(A.preds[I.frame], n, ring, vn, an, ag0, ag1, uf, scratch);
     st.proposed = 1;
     st.proposals += 1;
 }
@@ -680,6 +701,7 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
                 A.eflag[bid] = 0;
                 st.iter += 1;
                 st.proposed = 0;
+                st.commit_pending = 0;
                 st.fallbacks += 1;
                 A.state[bid] = st;
             }
@@ -712,6 +734,7 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
     }
     const double *kinit = A.kinit + I.kinit_base;
     const int step = st.step;
+    st.commit_pending = 0;  // this round's element tasks have committed
     st.iter += 1;  // this assembly
     for (int i = tid; i < n; i += nt) {  // :496-531
         const double anext = F.a0 * (uf[i] - un[i]) - F.a2 * vn[i] - F.a3 * an[i];
@@ -777,7 +800,11 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         // converged: commit (:3081-3101), committed basic sync (helpers.mojo:1832-1883),
         // Newmark update (:594-647), recorder, then the next step's predictor
         if (tid == 0) A.iters[I.iters_base + step] = st.iter;
-        commit_instance(A, I, F);
+        // the commit of every section is left to the next round's element
+        // tasks (each commits its own sections first); after the last step
+        // it is done here
+        const bool last_step = step + 1 >= I.n_steps;
+        if (last_step) commit_instance(A, I, F);
         for (int e = tid; e < F.n_elem; e += nt) {
             const ElemDev &el = A.elems[F.elem_base + e];
             double ug[6];
@@ -811,6 +838,7 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         st.total_iters += st.iter;
         st.iter = 0;
         st.proposed = 0;
+        st.commit_pending = last_step ? 0 : 1;
         st.step = step + 1;
         if (st.step < I.n_steps) {
             start_step(A, I, F, st.step, uf, vf, af, un, vn, an, P);
@@ -887,59 +915,95 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
     double *dense = A.dense + I.dense_base;
     double *orig = dense + (int64_t)n * n;
     for (int sl = tid; sl < n * bw; sl += nt) orig[sl] = band[sl];
+    __syncthreads();
     int32_t *piv = A.piv + I.piv_base;
-    bool dense_mode = false;
-    for (int k = 0; k < n; ++k) {
-        const int i1 = k + hbw > n - 1 ? n - 1 : k + hbw;
-        if (k > 0) {  // store the multipliers of column k - 1
-            const int kp = k - 1;
-            const int ip1 = kp + hbw > n - 1 ? n - 1 : kp + hbw;
-            for (int i = kp + 1 + tid; i <= ip1; i += nt) {
-                const int sl = i * bw + (kp - i + hbw);
-                band[sl] = band[sl] / s_diag;
+    // One wavefront factors (no block barriers inside the loop); every
+    // entry sees the same operations in the same order as the block-wide
+    // form.  The multipliers of column k, a_ik / a_kk, are stored when column
+    // k is eliminated (the reference divides the same two values when it
+    // stores them at step k + 1, and the elimination uses the same quotient).
+    // `work` holds them for the update.
+    if (tid < 64) {
+        const int lane = tid;
+        int stat = 0;
+        for (int k = 0; k < n; ++k) {
+            const int i1 = k + hbw > n - 1 ? n - 1 : k + hbw;
+            const double diag = band[k * bw + hbw];
+            const double adiag = fabs(diag);
+            // partial-pivot test (lsb.mojo:640-657): the first row of the
+            // largest |a_rk| (scanned from a running maximum of 0, so a NaN is
+            // never taken) must be row k
+            const double acmp = adiag > 0.0 ? adiag : 0.0;
+            bool above = false;
+            for (int row = k + 1 + lane; row <= i1; row += 64)
+                if (fabs(band[row * bw + (k - row + hbw)]) > acmp) above = true;
+            if (__builtin_amdgcn_ballot_w64(above) != 0) {
+                double pabs = acmp;
+                for (int row = k + 1 + lane; row <= i1; row += 64) {
+                    const double cand = fabs(band[row * bw + (k - row + hbw)]);
+                    pabs = cand > pabs ? cand : pabs;
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double o = __shfl_xor(pabs, off);
+                    pabs = o > pabs ? o : pabs;
+                }
+                stat = pabs <= 1.0e-14 ? kInstSingular : kInstBandPivot;
+                break;
             }
-        }
-        if (tid == 0) {
-            int prow = k;
-            double pabs = 0.0;
-            for (int row = k; row <= i1; ++row) {
-                const double cand = fabs(band[row * bw + (k - row + hbw)]);
-                if (cand > pabs) {
-                    pabs = cand;
-                    prow = row;
+            if (!(adiag > 1.0e-14)) {  // largest magnitude (or, all zero, the pivot itself) too small
+                stat = kInstSingular;
+                break;
+            }
+            const int j1 = k + hbw > n - 1 ? n - 1 : k + hbw;
+            const int ni = i1 - k, nj = j1 - k;
+            for (int t = lane; t < ni; t += 64) {
+                const int sl = (k + 1 + t) * bw + (hbw - 1 - t);
+                const double mlt = band[sl] / diag;
+                band[sl] = mlt;
+                work[t] = mlt;
+            }
+            wave_sync();
+            // lane -> column jj of the pivot row (held in a register), rows in
+            // steps of `rstep`; row i, column j sits at band slot
+            // i * bw + (j - i + hbw) = k * bw + hbw + jj + 1 + (i - k) * (bw - 1)
+            for (int cb = 0; cb < nj; cb += 64) {
+                const int m = nj - cb < 64 ? nj - cb : 64;
+                const int rstep = 64 / m;
+                const int jj = cb + lane % m, r0 = lane / m;
+                if (r0 < rstep) {
+                    const double urow = band[k * bw + hbw + 1 + jj];
+                    const int base = k * bw + hbw + jj + 1;
+                    int ii = r0;
+                    for (; ii + 3 * rstep < ni; ii += 4 * rstep) {
+                        const double f0 = work[ii], f1 = work[ii + rstep], f2 = work[ii + 2 * rstep],
+                                     f3 = work[ii + 3 * rstep];
+                        const int s0 = base + (ii + 1) * (bw - 1), ds = rstep * (bw - 1);
+                        if (f0 != 0.0) band[s0] -= f0 * urow;
+                        if (f1 != 0.0) band[s0 + ds] -= f1 * urow;
+                        if (f2 != 0.0) band[s0 + 2 * ds] -= f2 * urow;
+                        if (f3 != 0.0) band[s0 + 3 * ds] -= f3 * urow;
+                    }
+                    for (; ii < ni; ii += rstep) {
+                        const double factor = work[ii];
+                        if (factor == 0.0) continue;
+                        band[base + (ii + 1) * (bw - 1)] -= factor * urow;
+                    }
                 }
             }
-            s_stat = 0;
-            if (pabs <= 1.0e-14) s_stat = kInstSingular;
-            else if (prow != k) s_stat = kInstBandPivot;
-            else {
-                const double diag = band[k * bw + hbw];
-                if (fabs(diag) <= 1.0e-14) s_stat = kInstSingular;
-            }
+            wave_sync();
         }
-        __syncthreads();
-        if (s_stat == kInstBandPivot) {
-            dense_mode = true;
-            break;
+        if (lane == 0) s_stat = stat;
+    }
+    __syncthreads();
+    bool dense_mode = false;
+    if (s_stat == kInstBandPivot) {
+        dense_mode = true;
+    } else if (s_stat != 0) {
+        if (tid == 0) {
+            st.status = s_stat;
+            A.state[bid] = st;
         }
-        if (s_stat != 0) {
-            if (tid == 0) {
-                st.status = s_stat;
-                A.state[bid] = st;
-            }
-            return;
-        }
-        const double diag = band[k * bw + hbw];
-        const int j1 = k + hbw > n - 1 ? n - 1 : k + hbw;
-        const int ni = i1 - k, nj = j1 - k;
-        for (int p = tid; p < ni * nj; p += nt) {
-            const int i = k + 1 + p / nj, j = k + 1 + p % nj;
-            const double factor = band[i * bw + (k - i + hbw)] / diag;
-            if (factor == 0.0) continue;
-            band[i * bw + (j - i + hbw)] -= factor * band[k * bw + (j - k + hbw)];
-        }
-        if (tid == 0) s_diag = diag;
-        __syncthreads();
+        return;
     }
     if (dense_mode) {
         // lsb.mojo:608-628 then :175-214: dense LU with partial pivoting
@@ -1031,11 +1095,6 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         }
         __syncthreads();
     } else {
-        {
-            const int kp = n - 1;
-            for (int i = kp + 1 + tid; i < n; i += nt) band[i * bw + (kp - i + hbw)] /= s_diag;
-        }
-        __syncthreads();
         // forward (column order, same per-row subtraction order as lsb.mojo:746-757), then backward
         if (tid < 64) {
             const int lane = tid;
@@ -1049,13 +1108,19 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
                 }
                 wave_sync();
             }
-            if (lane == 0) {
-                for (int i = n - 1; i >= 0; --i) {
+            // backward: the lanes form row i's products (in `vt`), lane 0
+            // subtracts them in column order (the serial order of lsb.mojo:758-769)
+            for (int i = n - 1; i >= 0; --i) {
+                const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
+                const int cnt = j1 - i;
+                for (int t = lane; t < cnt; t += 64) vt[t] = band[i * bw + (t + 1 + hbw)] * du[i + 1 + t];
+                wave_sync();
+                if (lane == 0) {
                     double sum = work[i];
-                    const int j1 = i + hbw > n - 1 ? n - 1 : i + hbw;
-                    for (int j = i + 1; j <= j1; ++j) sum -= band[i * bw + (j - i + hbw)] * du[j];
+                    for (int t = 0; t < cnt; ++t) sum -= vt[t];
                     du[i] = sum / band[i * bw + hbw];
                 }
+                wave_sync();
             }
         }
         __syncthreads();
@@ -1088,9 +1153,129 @@ __global__ __launch_bounds__(kElemThreads) void frame2d_inst_kernel(Frame2dArgs 
         const InstState st = A.state[bid];
         if (st.status != 0 || st.step >= I.n_steps) break;
         if (st.setup_done && st.step - step0 >= max_steps) break;
-        for (int e = 0; e < F.n_elem; ++e) element_state(A, bid, e, L, mats);
+        for (int e = 0; e < F.n_elem; ++e) element_state(A, bid, e, L, mats, st.commit_pending != 0);
         __syncthreads();
         sys_round(A, bid, lds, ss);
+    }
+}
+
+// Work-queue schedule.  Persistent worker blocks take tasks from a ring:
+// an element task runs one element's state determination; the last
+// element task of an instance's round publishes the instance's system task
+// (sys_round), which publishes the next round's element tasks unless the
+// instance has finished, stopped or used its step budget.  Instances never
+// wait for one another.  Each task's writes are made visible at device
+// scope before the task that reads them is published.
+constexpr long long kQueueSpinLimit = 1ll << 28;  // sleeps of a wait before the launch gives up (many seconds)
+
+__device__ __forceinline__ void queue_push(const QueueDev &Q, int32_t task, unsigned long long idx) {
+    int32_t *slot = Q.slots + (idx & (unsigned long long)Q.cap_mask);
+    long long spins = 0;
+    while (__hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != -1) {
+        // the ring holds every outstanding task by construction; this wait ends at once
+        if (++spins > kQueueSpinLimit || __hip_atomic_load(Q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __hip_atomic_store(Q.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(slot, task, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish the element tasks of an instance's next round (called by every
+// thread of the block; `base` is block-shared scratch).
+__device__ __forceinline__ void queue_round(const Frame2dArgs &A, const QueueDev &Q, int inst,
+                                            unsigned long long &base) {
+    const int n_elem = A.frames[A.insts[inst].frame].n_elem;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(Q.pending + inst, n_elem, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        base = atomicAdd(Q.tail, (unsigned long long)n_elem);
+    }
+    __syncthreads();
+    const unsigned long long t0 = base;
+    for (int e = threadIdx.x; e < n_elem; e += blockDim.x) queue_push(Q, (inst << 16) | (e + 1), t0 + e);
+}
+
+__global__ void frame2d_queue_reset_kernel(QueueDev Q) {
+    *Q.head = 0ull;
+    *Q.tail = 0ull;
+    *Q.live = 0;
+    *Q.abort = 0;
+}
+
+// One block per instance: the start step, the live count and the first
+// round of every running instance.
+__global__ void frame2d_queue_init_kernel(Frame2dArgs A, QueueDev Q) {
+    __shared__ unsigned long long base;
+    const int i = blockIdx.x;
+    const InstState st = A.state[i];
+    if (threadIdx.x == 0) Q.step0[i] = st.step;
+    if (st.status != 0 || st.step >= A.insts[i].n_steps) return;
+    if (threadIdx.x == 0) atomicAdd(Q.live, 1);
+    queue_round(A, Q, i, base);
+}
+
+__global__ __launch_bounds__(kElemThreads) void frame2d_queue_kernel(Frame2dArgs A, QueueDev Q, int max_steps) {
+    extern __shared__ double lds[];
+    __shared__ ElemLds L;
+    __shared__ SysShared ss;
+    __shared__ int32_t s_task, s_more;
+    __shared__ unsigned long long s_base;
+    const MatDev *mats = stage_materials(A, L);
+    while (true) {
+        if (threadIdx.x == 0) {
+            const unsigned long long h = atomicAdd(Q.head, 1ull);
+            int32_t *slot = Q.slots + (h & (unsigned long long)Q.cap_mask);
+            int32_t t;
+            long long spins = 0;
+            while ((t = __hip_atomic_load(slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) == -1) {
+                if (__hip_atomic_load(Q.live, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0 ||
+                    __hip_atomic_load(Q.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    t = -2;
+                    break;
+                }
+                if (++spins > kQueueSpinLimit) {
+                    __hip_atomic_store(Q.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    t = -2;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            if (t >= 0) __hip_atomic_store(slot, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_task = t;
+        }
+        // thread 0's device-scope acquire (the slot load) invalidated this
+        // compute unit's caches before the barrier; the other wavefronts read
+        // after it
+        __syncthreads();
+        const int32_t t = s_task;
+        if (t == -2) break;
+        const int inst = t >> 16, e = (t & 0xffff) - 1;
+        if (e >= 0) {
+            element_state(A, inst, e, L, mats, A.state[inst].commit_pending != 0);
+            // the barrier completes every wavefront's stores; thread 0's
+            // device-scope release then publishes them with the count
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __threadfence();
+                if (__hip_atomic_fetch_add(Q.pending + inst, -1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 1)
+                    queue_push(Q, inst << 16, atomicAdd(Q.tail, 1ull));
+            }
+        } else {
+            sys_round(A, inst, lds, ss);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                __threadfence();
+                const InstState st = A.state[inst];
+                s_more = st.status == 0 && st.step < A.insts[inst].n_steps &&
+                         !(st.setup_done && st.step - Q.step0[inst] >= max_steps);
+                if (!s_more) __hip_atomic_fetch_add(Q.live, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __syncthreads();
+            if (s_more) queue_round(A, Q, inst, s_base);
+        }
+        __syncthreads();  // s_task, s_base and the LDS of this task are free again
     }
 }
 
@@ -1107,6 +1292,23 @@ hipError_t frame2d_set_lds(size_t lds_bytes) {
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute((const void *)frame2d_inst_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds_bytes);
+}
+
+hipError_t frame2d_queue_set_lds(size_t lds_bytes, int *blocks_per_cu) {
+    hipError_t e = hipFuncSetAttribute((const void *)frame2d_queue_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void *)frame2d_queue_kernel,
+                                                        kElemThreads, lds_bytes);
+}
+
+// Work-queue schedule: every unfinished instance advances up to max_steps steps.
+hipError_t launch_frame2d_queue(hipStream_t stream, const Frame2dArgs &args, const QueueDev &Q, int workers,
+                                int max_steps, size_t lds_bytes) {
+    hipLaunchKernelGGL(frame2d_queue_reset_kernel, dim3(1), dim3(1), 0, stream, Q);
+    hipLaunchKernelGGL(frame2d_queue_init_kernel, dim3(Q.n_inst), dim3(256), 0, stream, args, Q);
+    hipLaunchKernelGGL(frame2d_queue_kernel, dim3(workers), dim3(kElemThreads), lds_bytes, stream, args, Q, max_steps);
+    return hipGetLastError();
 }
 
 // Persistent schedule: every unfinished instance advances up to max_steps steps.

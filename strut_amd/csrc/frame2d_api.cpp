@@ -21,6 +21,9 @@ hipError_t launch_frame2d_round(hipStream_t stream, const Frame2dArgs &args, int
                                 size_t lds_bytes);
 hipError_t launch_frame2d_persistent(hipStream_t stream, const Frame2dArgs &args, int n_inst, int max_steps,
                                      size_t lds_bytes);
+hipError_t frame2d_queue_set_lds(size_t lds_bytes, int *blocks_per_cu);
+hipError_t launch_frame2d_queue(hipStream_t stream, const Frame2dArgs &args, const QueueDev &Q, int workers,
+                                int max_steps, size_t lds_bytes);
 
 struct FrameDef {
     FrameDev dev{};
@@ -62,6 +65,10 @@ struct FrameHost {
     int chosen = SGPU_SCHEDULE_ROUNDS;
     double rate_rounds = 0.0, rate_persistent = 0.0;
     int tunings = 0;
+    // work-queue schedule
+    QueueDev queue{};
+    int workers = 0;
+    double rate_queue = 0.0;
     // learned predictor (per frame) and training-sample recorder
     MlpDev *d_preds = nullptr;
     bool record_samples = false;
@@ -466,16 +473,40 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     }
     fh.lds = lds;
     HIP_OK(ctx, frame2d_set_lds(lds));
+    {  // work queue: room for every instance's largest round plus one claim per worker
+        int per_cu = 0, cus = 0;
+        HIP_OK(ctx, frame2d_queue_set_lds(lds, &per_cu));
+        HIP_OK(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+        if (per_cu < 1) return fail(ctx, SGPU_ERR_STATE, "the work-queue kernel does not fit on a compute unit");
+        fh.workers = per_cu * cus;
+        int64_t need = fh.workers + 1024;
+        for (const InstHost &in : fh.insts) need += fdev[in.frame].n_elem + 1;
+        int64_t cap = 1;
+        while (cap < need) cap <<= 1;
+        if (fh.insts.size() >= 32768 || cap > (1ll << 30))
+            return fail(ctx, SGPU_ERR_ARG, "too many instances for the work queue");
+        QueueDev &Q = fh.queue;
+        unsigned long long *ht;
+        if ((rc = dalloc(ctx, &ht, 2))) return rc;
+        Q.head = ht;
+        Q.tail = ht + 1;
+        if ((rc = dalloc(ctx, &Q.slots, (size_t)cap))) return rc;
+        HIP_OK(ctx, hipMemset(Q.slots, 0xff, sizeof(int32_t) * (size_t)cap));
+        if ((rc = dalloc(ctx, &Q.pending, fh.insts.size())) || (rc = dalloc(ctx, &Q.live, 1)) ||
+            (rc = dalloc(ctx, &Q.step0, fh.insts.size())) || (rc = dalloc(ctx, &Q.abort, 1)))
+            return rc;
+        Q.cap_mask = (int32_t)(cap - 1);
+        Q.n_inst = (int32_t)fh.insts.size();
+    }
     fh.finalized = true;
     return SGPU_OK;
 }
 
-int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds) {
-    if (!ctx) return SGPU_ERR_ARG;
-    FrameHost &fh = frames_of(ctx);
-    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
-    if (rounds <= 0) return fail(ctx, SGPU_ERR_ARG, "rounds must be > 0");
-    if (fh.graph_rounds != rounds) {  // capture `rounds` rounds once, replay them afterwards
+namespace {
+
+// Capture `rounds` rounds in a HIP graph once; replays reuse it.
+int ensure_graph(sgpu_ctx *ctx, FrameHost &fh, int rounds) {
+    if (fh.graph_rounds != rounds) {
         if (fh.graph_exec) HIP_OK(ctx, hipGraphExecDestroy(fh.graph_exec));
         if (fh.graph) HIP_OK(ctx, hipGraphDestroy(fh.graph));
         fh.graph_exec = nullptr;
@@ -496,6 +527,18 @@ int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds) {
         HIP_OK(ctx, hipGraphInstantiate(&fh.graph_exec, g, nullptr, nullptr, 0));
         fh.graph_rounds = rounds;
     }
+    return SGPU_OK;
+}
+
+}  // namespace
+
+int sgpu_frame2d_launch(sgpu_ctx *ctx, int32_t rounds) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (rounds <= 0) return fail(ctx, SGPU_ERR_ARG, "rounds must be > 0");
+    int rc;
+    if ((rc = ensure_graph(ctx, fh, rounds))) return rc;
     {
         ProfScope ps(ctx, SGPU_KERNEL_FRAME2D);
         HIP_OK(ctx, hipGraphLaunch(fh.graph_exec, ctx->stream));
@@ -513,9 +556,26 @@ int sgpu_frame2d_launch_persistent(sgpu_ctx *ctx, int32_t max_steps) {
     return SGPU_OK;
 }
 
+int sgpu_frame2d_launch_queue(sgpu_ctx *ctx, int32_t max_steps) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (max_steps <= 0) return fail(ctx, SGPU_ERR_ARG, "max_steps must be > 0");
+    {
+        ProfScope ps(ctx, SGPU_KERNEL_FRAME2D);
+        HIP_OK(ctx, launch_frame2d_queue(ctx->stream, fh.args, fh.queue, fh.workers, max_steps, fh.lds));
+    }
+    int32_t abort_flag = 0;
+    HIP_OK(ctx, hipMemcpyAsync(&abort_flag, fh.queue.abort, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    if (abort_flag) return fail(ctx, SGPU_ERR_STATE, "work queue: a worker's wait exceeded its bound");
+    return SGPU_OK;
+}
+
 int sgpu_frame2d_set_schedule(sgpu_ctx *ctx, int schedule) {
     if (!ctx) return SGPU_ERR_ARG;
-    if (schedule != SGPU_SCHEDULE_AUTO && schedule != SGPU_SCHEDULE_ROUNDS && schedule != SGPU_SCHEDULE_PERSISTENT)
+    if (schedule != SGPU_SCHEDULE_AUTO && schedule != SGPU_SCHEDULE_ROUNDS && schedule != SGPU_SCHEDULE_PERSISTENT &&
+        schedule != SGPU_SCHEDULE_QUEUE)
         return fail(ctx, SGPU_ERR_ARG, "unknown schedule %d", schedule);
     FrameHost &fh = frames_of(ctx);
     fh.schedule = schedule;
@@ -523,13 +583,14 @@ int sgpu_frame2d_set_schedule(sgpu_ctx *ctx, int schedule) {
     return SGPU_OK;
 }
 
-int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out4) {
-    if (!ctx || !out4) return SGPU_ERR_ARG;
+int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out5) {
+    if (!ctx || !out5) return SGPU_ERR_ARG;
     FrameHost &fh = frames_of(ctx);
-    out4[0] = fh.chosen;
-    out4[1] = fh.rate_rounds;
-    out4[2] = fh.rate_persistent;
-    out4[3] = fh.tunings;
+    out5[0] = fh.chosen;
+    out5[1] = fh.rate_rounds;
+    out5[2] = fh.rate_persistent;
+    out5[3] = fh.tunings;
+    out5[4] = fh.rate_queue;
     return SGPU_OK;
 }
 
@@ -538,11 +599,16 @@ namespace {
 // Launch one chunk of the given schedule, wait, and report the GM-steps the
 // instances advanced and whether any instance is still active.
 int run_chunk(sgpu_ctx *ctx, FrameHost &fh, int schedule, int rounds, std::vector<InstState> &st, int64_t &advanced,
-              int &active, double &seconds) {
+              int &active, double &seconds, bool timing) {
     int64_t before = 0;
     for (const InstState &s : st) before += s.step;
     const auto t0 = std::chrono::steady_clock::now();
-    int rc = schedule == SGPU_SCHEDULE_PERSISTENT ? sgpu_frame2d_launch_persistent(ctx, rounds / 4 > 0 ? rounds / 4 : 1)
+    // a chunk is about rounds / 4 steps per instance for every schedule; the
+    // queue's regular (untimed-choice) chunks are longer, as its chunk end
+    // waits for the slowest instance
+    const int steps = rounds / 4 > 0 ? rounds / 4 : 1;
+    int rc = schedule == SGPU_SCHEDULE_PERSISTENT ? sgpu_frame2d_launch_persistent(ctx, steps)
+             : schedule == SGPU_SCHEDULE_QUEUE    ? sgpu_frame2d_launch_queue(ctx, timing ? 2 : 4 * rounds)
                                                   : sgpu_frame2d_launch(ctx, rounds);
     if (rc) return rc;
     HIP_OK(ctx, hipMemcpyAsync(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost,
@@ -576,6 +642,7 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
     // the faster; measure again when the active set has halved.
     bool tune = fh.schedule == SGPU_SCHEDULE_AUTO;
     int active_at_tune = active;
+    int since_tune = 0;
     int64_t chunks = 0;
     const int64_t max_chunks = 4 + 2 * fh.max_rounds / rounds;
     while (active > 0) {
@@ -585,22 +652,32 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
         double sec = 0.0;
         int rc;
         if (tune) {
+            // the rounds and the work queue (the persistent schedule, one
+            // block per instance with its elements in sequence, is slower than
+            // the queue wherever both were measured, and is not sampled)
             double rate[2] = {0.0, 0.0};
-            const int order[2] = {SGPU_SCHEDULE_ROUNDS, SGPU_SCHEDULE_PERSISTENT};
+            const int order[2] = {SGPU_SCHEDULE_ROUNDS, SGPU_SCHEDULE_QUEUE};
+            if ((rc = ensure_graph(ctx, fh, rounds))) return rc;
             for (int k = 0; k < 2 && active > 0; ++k) {
-                if ((rc = run_chunk(ctx, fh, order[k], rounds, st, adv, active, sec))) return rc;
+                if ((rc = run_chunk(ctx, fh, order[k], rounds, st, adv, active, sec, true))) return rc;
                 rate[k] = sec > 0.0 ? (double)adv / sec : 0.0;
             }
             fh.rate_rounds = rate[0];
-            fh.rate_persistent = rate[1];
-            fh.chosen = rate[1] > rate[0] ? SGPU_SCHEDULE_PERSISTENT : SGPU_SCHEDULE_ROUNDS;
+            fh.rate_queue = rate[1];
+            fh.chosen = rate[1] > rate[0] ? SGPU_SCHEDULE_QUEUE : SGPU_SCHEDULE_ROUNDS;
             fh.tunings += 1;
             tune = false;
             active_at_tune = active;
             continue;
         }
-        if ((rc = run_chunk(ctx, fh, fh.chosen, rounds, st, adv, active, sec))) return rc;
-        if (fh.schedule == SGPU_SCHEDULE_AUTO && active > 0 && 2 * active <= active_at_tune) tune = true;
+        if ((rc = run_chunk(ctx, fh, fh.chosen, rounds, st, adv, active, sec, false))) return rc;
+        // measure again when the active set has halved, and every 64 chunks
+        // (the cost of an instance's step changes as its records go nonlinear)
+        if (fh.schedule == SGPU_SCHEDULE_AUTO && active > 0 &&
+            (2 * active <= active_at_tune || ++since_tune >= 64)) {
+            tune = true;
+            since_tune = 0;
+        }
     }
     return SGPU_OK;
 }

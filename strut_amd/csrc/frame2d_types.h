@@ -63,7 +63,7 @@ struct InstState {
     int32_t proposed;   // the current iterate is the learned predictor's proposal (not yet judged)
     int32_t proposals;  // steps started from a learned proposal
     int32_t fallbacks;  // proposals replaced by the Newmark predictor
-    int32_t pad_;
+    int32_t commit_pending;  // the last converged step's commit is done by the next round's element tasks
 };
 
 enum InstStatus : int32_t {
@@ -98,6 +98,19 @@ struct Frame2dArgs {
     StateArrays c, t;
     const MlpDev *preds;  // per frame: learned predictor (n_in == 0: the plain Newmark predictor)
     float *samples;       // training-sample recorder, or null
+};
+
+// Work-queue schedule: a ring of tasks, (instance << 16) | (element + 1),
+// element + 1 == 0 being the instance's system step (sys_round).
+struct QueueDev {
+    unsigned long long *head, *tail;  // claimed / published task counts
+    int32_t *slots;                   // ring of cap_mask + 1 entries, -1 when empty
+    int32_t *pending;                 // per instance: element tasks of the current round not yet done
+    int32_t *live;                    // instances still running in this launch
+    int32_t *step0;                   // per instance: step at the launch
+    int32_t *abort;                   // set when a wait exceeds its bound (the launch then fails)
+    int32_t cap_mask;
+    int32_t n_inst;
 };
 
 }  // namespace sgpu

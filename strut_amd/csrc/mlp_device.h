@@ -97,6 +97,37 @@ __device__ __forceinline__ void mlp_tile_gemm(const uint16_t *A, int lda, int ro
     }
 }
 
+// One-row product for in-step inference (C[0][N] = A[0][K] W^T), the
+// latency-bound case: a wavefront issues all weight loads of a 256-wide K
+// chunk of its 16-column tile before the matrix operations, so a layer costs
+// a few memory latencies.  Lanes of row 0 read A from LDS (the other rows
+// are zero).
+template <class Epi>
+__device__ __forceinline__ void mlp_row_gemm(const uint16_t *A, const uint16_t *W, int N, int K, Epi epi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    for (int jt = wave; jt < N / 16; jt += nw) {
+        const uint16_t *wp = W + (size_t)(jt * 16 + r) * K + 4 * g;
+        f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int kc = 0; kc < K; kc += 256) {
+            const int nk = (K - kc) / 16 < 16 ? (K - kc) / 16 : 16;
+            bf16x4_t b[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                if (t < nk) b[t] = *reinterpret_cast<const bf16x4_t *>(wp + kc + 16 * t);
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                if (t < nk) {
+                    bf16x4_t a = {0, 0, 0, 0};
+                    if (r == 0) a = *reinterpret_cast<const bf16x4_t *>(A + 4 * g + kc + 16 * t);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc, 0, 0, 0);
+                }
+            }
+        }
+        if (g == 0) epi(jt * 16 + r, acc[0]);  // row 0 of D: lanes 0..15, element 0
+    }
+}
+
 // Inference of one sample by the whole block.  feat (LDS, k0 + kMlpPad bf16)
 // holds the normalised, padded features; h0/h1 (LDS, kMlpHidden + kMlpPad
 // bf16 each) are scratch; out (LDS, n5 floats) receives the de-normalised
@@ -104,21 +135,18 @@ __device__ __forceinline__ void mlp_tile_gemm(const uint16_t *A, int lda, int ro
 __device__ __forceinline__ void mlp_infer_one(const MlpDev &m, const uint16_t *feat, uint16_t *h0, uint16_t *h1,
                                               float *out) {
     const uint16_t *a = feat;
-    int lda = m.k0 + kMlpPad;
     uint16_t *hb[2] = {h0, h1};
     for (int l = 0; l < kMlpLayers - 1; ++l) {
         uint16_t *h = hb[l & 1];
         const float *bias = m.b[l];
-        mlp_tile_gemm(a, lda, 1, m.w[l], kMlpHidden, mlp_k(m, l), [&](int i, int j, float c) {
-            if (i == 0) h[j] = f32_to_bf16(silu_f(c + bias[j]));
-        });
+        mlp_row_gemm(a, m.w[l], kMlpHidden, mlp_k(m, l),
+                     [&](int j, float c) { h[j] = f32_to_bf16(silu_f(c + bias[j])); });
         __syncthreads();
         a = h;
-        lda = kMlpHidden + kMlpPad;
     }
     const float *bias = m.b[kMlpLayers - 1];
-    mlp_tile_gemm(a, lda, 1, m.w[kMlpLayers - 1], m.n5, kMlpHidden, [&](int i, int j, float c) {
-        if (i == 0 && j < m.n_out) out[j] = (c + bias[j]) * m.y_scale[j] + m.y_mean[j];
+    mlp_row_gemm(a, m.w[kMlpLayers - 1], m.n5, kMlpHidden, [&](int j, float c) {
+        if (j < m.n_out) out[j] = (c + bias[j]) * m.y_scale[j] + m.y_mean[j];
     });
     __syncthreads();
 }

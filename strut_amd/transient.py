@@ -63,6 +63,7 @@ def _declare(lib):
                                                 POINTER(ctypes.c_int64)]),
         "sgpu_frame2d_set_predictor": (ctypes.c_int, [c_void_p, c_int32, c_void_p]),
         "sgpu_frame2d_inst_stats": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_int32)]),
+        "sgpu_frame2d_launch_queue": (ctypes.c_int, [c_void_p, c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -146,16 +147,22 @@ class FrameBatch:
         up to max_steps steps on its own."""
         self._check(self.lib.sgpu_frame2d_launch_persistent(self.rt.ctx, max_steps))
 
+    def launch_queue(self, max_steps: int) -> None:
+        """Enqueue the work-queue schedule: every unfinished instance advances
+        up to max_steps steps, its elements taken as tasks by persistent
+        worker blocks."""
+        self._check(self.lib.sgpu_frame2d_launch_queue(self.rt.ctx, max_steps))
+
     def set_schedule(self, schedule: int) -> None:
-        """_native.SCHEDULE_AUTO (timed choice), SCHEDULE_ROUNDS or SCHEDULE_PERSISTENT."""
+        """_native.SCHEDULE_AUTO (timed choice), SCHEDULE_ROUNDS, SCHEDULE_PERSISTENT or SCHEDULE_QUEUE."""
         self._check(self.lib.sgpu_frame2d_set_schedule(self.rt.ctx, schedule))
 
     def schedule_info(self) -> dict:
-        out = np.zeros(4)
+        out = np.zeros(5)
         self._check(self.lib.sgpu_frame2d_schedule_info(self.rt.ctx, dptr(out)))
-        names = {0: "auto", 1: "rounds", 2: "persistent"}
+        names = {0: "auto", 1: "rounds", 2: "persistent", 3: "queue"}
         return {"schedule": names.get(int(out[0]), "?"), "rounds_gm_steps_per_s": out[1],
-                "persistent_gm_steps_per_s": out[2], "tunings": int(out[3])}
+                "persistent_gm_steps_per_s": out[2], "queue_gm_steps_per_s": out[4], "tunings": int(out[3])}
 
     def run(self, rounds: int = 64) -> None:
         """Run until every instance has finished or stopped, checking after

@@ -12,6 +12,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _device_runtime_first(request):
+    """In a session with GPU tests, bring up torch's device runtime before
+    any test creates a library context (torch tensors are the device-memory
+    plumbing of some GPU tests)."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    yield
+
+
 @pytest.fixture(scope="session")
 def gpu_available():
     import torch

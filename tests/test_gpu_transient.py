@@ -26,10 +26,10 @@ def _rel_err(u_gpu, u_ref):
     return float(np.max(np.abs(u_gpu - u_ref) / scale))
 
 
-SCHEDULES = [_native.SCHEDULE_ROUNDS, _native.SCHEDULE_PERSISTENT, _native.SCHEDULE_AUTO]
+SCHEDULES = [_native.SCHEDULE_ROUNDS, _native.SCHEDULE_PERSISTENT, _native.SCHEDULE_QUEUE, _native.SCHEDULE_AUTO]
 
 
-@pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "auto"])
+@pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "queue", "auto"])
 def test_gpu_config0_plain_matches_oracle(schedule):
     fc = load_frame2d_case(cases.config0_case(steps=400, seed=0))
     ref = oracle.run_transient2d(fc)
@@ -50,7 +50,7 @@ def test_gpu_config0_plain_matches_oracle(schedule):
     assert err <= REL
 
 
-@pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "auto"])
+@pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "queue", "auto"])
 def test_gpu_ensemble_of_uneven_instances_matches_oracle(schedule):
     """Two frames x several motions of different lengths in one launch."""
     specs = [(3, 1, 0, 0.4, 260), (3, 1, 1, 0.8, 180), (4, 3, 2, 0.6, 140), (4, 3, 3, 0.3, 220)]
