@@ -27,9 +27,9 @@ Two legs, one JSON line.
    the ensemble generator (corner frequency 2-15 Hz, strong-motion duration
    5-20 s, PGA on the log grid 0.1-1.5 g), each record capped at
    --ens-steps steps, all in one FrameBatch (autotuned schedule), against
-   the same records solved one by one with the plain path (one record of
-   each frame timed alone; the one-by-one time is the sum of each record's
-   completed steps over its frame's single-record rate).
+   the same records solved one by one with the plain path (four records of
+   each frame timed alone; the one-by-one time is each frame's completed
+   steps over the steps per second of its timed single records).
 
 4. The learned Newton predictor (``config.learned``): on the 4 x 3 frame, a
    network trained on the device from the pairs the solver records on one
@@ -296,9 +296,9 @@ def _run_batch(fcs_by_frame, record=False, predictor=None, record_all=False, sch
     return fb, fids, ids, time.perf_counter() - t0
 
 
-def ensemble_leg(n_motions: int, max_steps: int) -> dict:
+def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4) -> dict:
     """The uneven three-frame ensemble batched vs its records one by one."""
-    from strut_amd import cases
+    from strut_amd import _native, cases
 
     frames = [cases.config2_frame_case(s, b, steps=100) for s, b in cases.CONFIG2_FRAMES]
     by_frame = {k: [_motion_case(f, 100000 + 1000 * k + j, max_steps) for j in range(n_motions)]
@@ -308,27 +308,33 @@ def ensemble_leg(n_motions: int, max_steps: int) -> dict:
     sched = fb.schedule_info()
     fb.close()
     done = [r["steps_done"] for r in res]
-    # plain path: one record of each frame alone (its first motion, up to 300 steps)
+    # plain path, records one by one: n_single records of each frame alone
+    # (every n_motions / n_single-th motion, the same step cap); each frame's
+    # one-by-one time is its batched steps over the sample's steps per second
     single = {}
     for k, f in enumerate(frames):
-        one = {k: [_motion_case(f, 100000 + 1000 * k, min(max_steps, 300))]}
-        fb1, _, ids1, s1 = _run_batch(one)
-        r1 = fb1.result(ids1[0][1])
-        fb1.close()
-        single[k] = r1["steps_done"] / s1
+        sec1 = steps1 = 0.0
+        for j in range(0, n_motions, max(1, n_motions // n_single))[:n_single]:
+            fb1, _, ids1, s1 = _run_batch({k: [_motion_case(f, 100000 + 1000 * k + j, max_steps)]},
+                                          schedule=_native.SCHEDULE_ROUNDS)
+            steps1 += fb1.result(ids1[0][1])["steps_done"]
+            sec1 += s1
+            fb1.close()
+        single[k] = steps1 / sec1
     one_by_one = sum(d / single[k] for (k, _), d in zip(ids, done))
     return {"records": len(ids), "frames": [f"{s}x{b}" for s, b in cases.CONFIG2_FRAMES], "max_steps": max_steps,
             "gm_steps": int(sum(done)), "seconds": sec, "gm_steps_per_s": sum(done) / sec,
             "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0)),
             "plain_single_record_gm_steps_per_s": {f"{s}x{b}": single[k]
                                                    for k, (s, b) in enumerate(cases.CONFIG2_FRAMES)},
+            "single_records_timed_per_frame": n_single,
             "one_by_one_seconds_est": one_by_one, "batched_vs_one_by_one": one_by_one / sec, "schedule": sched}
 
 
 def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
     """Learned predictor on the 4 x 3 frame: train on one seed range, compare
     with the plain predictor on held-out records."""
-    from strut_amd import cases
+    from strut_amd import _native, cases
     from strut_amd.learned import train_predictor
 
     frame = cases.config2_frame_case(4, 3, steps=100)
@@ -345,8 +351,9 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
            "loss_first_last": [float(losses[0]), float(losses[-1])], "held_out_records": n_test}
     del x, y
     runs = {}
+    # both runs on the work queue, so that they differ only in the predictor
     for label, pred in (("plain", None), ("learned", {0: net})):
-        fb, _, ids, sec = _run_batch(held, predictor=pred, record_all=True)
+        fb, _, ids, sec = _run_batch(held, predictor=pred, record_all=True, schedule=_native.SCHEDULE_QUEUE)
         runs[label] = ([fb.result(i) for _, i in ids], [fb.inst_stats(i) for _, i in ids], sec)
         fb.close()
     net.close()
@@ -356,7 +363,7 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
     it_l = sum(int(b["iters"][:b["steps_done"]].sum()) for b in rl)
     steps_p = sum(a["steps_done"] for a in rp)
     steps_l = sum(b["steps_done"] for b in rl)
-    errs, errs_stalled = [], []
+    errs, errs_stalled, over = [], [], []
     for a, b, ok in zip(rp, rl, same):
         n = a["steps_done"]
         if not ok or n == 0:
@@ -364,8 +371,13 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
         dp = cases.peak_story_drift(a["u"][:n], 4, 3)
         dl = cases.peak_story_drift(b["u"][:n], 4, 3)
         e = float(np.max(np.abs(dl - dp) / dp))
-        (errs_stalled if int(a["iters"][:n].max()) > 10 else errs).append(e)
+        stalled = int(a["iters"][:n].max()) > 10
+        (errs_stalled if stalled else errs).append(e)
+        if e > 1e-6 and not stalled:
+            over.append({"status": int(a["status"]), "steps": int(n), "max_iters": int(a["iters"][:n].max()),
+                         "drift_rel_err": e})
     out.update({
+        "schedule": "queue",
         "newton_iters_per_step_plain": it_p / steps_p, "newton_iters_per_step_learned": it_l / steps_l,
         "newton_iter_reduction": 1.0 - (it_l / steps_l) / (it_p / steps_p),
         "gm_steps_per_s_plain": steps_p / sp, "gm_steps_per_s_learned": steps_l / sl,
@@ -373,6 +385,7 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
         "records_same_stop": int(sum(same)),
         "peak_drift_rel_err_max": max(errs) if errs else None,
         "records_within_1e-6": int(sum(1 for e in errs if e <= 1e-6)), "records_compared": len(errs),
+        "records_over_1e-6": over,
         "records_with_stalled_step": len(errs_stalled),
         "stalled_records_drift_rel_err_max": max(errs_stalled) if errs_stalled else None,
     })

@@ -160,7 +160,8 @@ int sgpu_fiber_totals(sgpu_ctx *ctx, int64_t *fibers, int64_t *slots);
 
 /* A 2D frame after loading (solver/run_case/loader.mojo:758-2460): nodes,
  * plain DOF numbering (3 per node), constraint flags, lumped masses
- * (M_total), forceBeamColumn2d elements with a fiber-section definition
+ * (M_total), forceBeamColumn2d / dispBeamColumn2d elements (elem_type 0 / 1;
+ * NULL: all forceBeamColumn2d) with a fiber-section definition
  * and integration rule each (10 slots per element in elem_xi / elem_w,
  * points on [0, 1], weights halved: beam_integration.mojo:224-241), and the
  * transient_nonlinear analysis record: Newmark gamma / beta, NormUnbalance
@@ -185,6 +186,7 @@ typedef struct {
     int32_t exc_dir;
     int32_t n_record;
     const int32_t *record_dofs;
+    const int32_t *elem_type;
 } sgpu_frame2d_desc;
 
 /* Path time series with a constant dt (time_series.mojo:71-96). */

@@ -234,6 +234,7 @@ class _Tr2dCase(ctypes.Structure):
         ("max_iters", c_int), ("alpha_m", c_double), ("beta_k", c_double), ("beta_k_init", c_double),
         ("exc_dir", c_int), ("ts_n", c_int), ("ts_values", POINTER(c_double)), ("ts_dt", c_double),
         ("ts_factor", c_double), ("ts_start", c_double), ("ts_use_last", c_int),
+        ("elem_type", POINTER(c_int)),
     ]
 
 
@@ -268,7 +269,9 @@ def run_transient2d(fc, record_history: bool = True) -> dict:
                 en=np.ascontiguousarray(fc.elem_nodes.reshape(-1), dtype=np.int32), first=first,
                 nip=np.ascontiguousarray(fc.elem_nip, dtype=np.int32),
                 xi=np.ascontiguousarray(fc.elem_xi.reshape(-1)), w=np.ascontiguousarray(fc.elem_w.reshape(-1)),
-                qo=q_off, ts=np.ascontiguousarray(fc.ts_values, dtype=np.float64))
+                qo=q_off, ts=np.ascontiguousarray(fc.ts_values, dtype=np.float64),
+                et=np.ascontiguousarray(fc.elem_type if len(fc.elem_type) else np.zeros(len(fc.elem_nip)),
+                                        dtype=np.int32))
     c = _Tr2dCase()
     c.n_nodes = fc.n_nodes
     c.x, c.y = _d_out(keep["x"]), _d_out(keep["y"])
@@ -287,6 +290,7 @@ def run_transient2d(fc, record_history: bool = True) -> dict:
     c.ts_n = len(keep["ts"])
     c.ts_values = _d_out(keep["ts"]) if len(keep["ts"]) else None
     c.ts_dt, c.ts_factor, c.ts_start, c.ts_use_last = fc.ts_dt, fc.ts_factor, fc.ts_start, fc.ts_use_last
+    c.elem_type = keep["et"].ctypes.data_as(POINTER(c_int))
     o = _Tr2dOut()
     u_hist = np.zeros((fc.steps, 3 * fc.n_nodes)) if record_history else None
     iters = np.zeros(fc.steps, dtype=np.int32)
@@ -297,6 +301,24 @@ def run_transient2d(fc, record_history: bool = True) -> dict:
             "free_count": o.free_count, "section_evals": o.section_evals, "local_iters": o.local_iters,
             "subdivisions": o.subdivisions, "fallbacks": o.fallbacks, "dense_fallbacks": o.dense_fallbacks,
             "q_state": q_state, "q_offset": q_off, "elem_first_inst": first, "_sections": secs}
+
+
+def dbc2d_eval(section: "Section2d", first_inst: int, xi, w, x1: float, y1: float, x2: float, y2: float, u_global):
+    """dispBeamColumn2d state determination of the oracle
+    (disp_beam_column2d.mojo:114-254, Linear, no element loads): global
+    6x6 tangent and 6 resisting forces at element displacements u_global."""
+    lib = load()
+    lib.or_dbc2d_eval.restype = c_int
+    lib.or_dbc2d_eval.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), POINTER(c_double), c_double, c_double,
+                                  c_double, c_double, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    u = np.ascontiguousarray(u_global, dtype=np.float64)
+    k = np.zeros(36)
+    f = np.zeros(6)
+    lib.or_dbc2d_eval(section.h, first_inst, len(xi), _d_out(xi), _d_out(w), x1, y1, x2, y2, _d_out(u), _d_out(k),
+                      _d_out(f))
+    return k.reshape(6, 6), f
 
 
 def band_solve(band: np.ndarray, hbw: int, rhs: np.ndarray):

@@ -21,10 +21,13 @@ typedef struct {
     int nip;
     double xi[OR_MAX_NIP], w[OR_MAX_NIP]; /* on [0,1], weights halved */
     double L, inv_L, c, s; /* cached geometry (force_beam_column2d.mojo:383-408) */
+    int type;             /* 0 forceBeamColumn2d, 1 dispBeamColumn2d */
+    double dc, ds;        /* dx / L, dy / L (disp_beam_column2d.mojo:137-140) */
     or_stats *stats;
 } or_elem2d;
 
 int or_sec2d_all_elastic(const or_sec2d *s);
+int or_dbc2d_state(const or_elem2d *el, const double u_global[6], double k_out[36], double f_out[6]);
 int or_fbc2d_state(const or_elem2d *el, double *q_state, const double u_global[6], double k_out[36],
                    double f_out[6]);
 

@@ -264,7 +264,8 @@ static int assemble(const or_tr2d_case *c, or_elem2d *els, double *q, const doub
         }
         double ue[6], ke[36], fe[6];
         for (int k = 0; k < 6; ++k) ue[k] = u[dof[k]];
-        int rc = or_fbc2d_state(&els[e], q + (size_t)c->q_offset[e], ue, ke, fe);
+        int rc = els[e].type == 1 ? or_dbc2d_state(&els[e], ue, ke, fe)
+                                  : or_fbc2d_state(&els[e], q + (size_t)c->q_offset[e], ue, ke, fe);
         if (rc < 0) return -1;
         unconverged += rc;
         for (int a = 0; a < 6; ++a)
@@ -277,6 +278,7 @@ static int assemble(const or_tr2d_case *c, or_elem2d *els, double *q, const doub
 /* helpers.mojo:1832-1883 */
 static void sync_basic(const or_tr2d_case *c, double *q, const double *u) {
     for (int e = 0; e < c->n_elem; ++e) {
+        if (c->elem_type && c->elem_type[e] != 0) continue; /* forceBeamColumn2d only (:1844-1845) */
         int nip = c->elem_nip[e];
         int ni = c->elem_nodes[2 * e], nj = c->elem_nodes[2 * e + 1];
         double dx = c->x[nj] - c->x[ni], dy = c->y[nj] - c->y[ni];
@@ -329,6 +331,9 @@ int or_tr2d_run(const or_tr2d_case *c, or_tr2d_out *out) {
         el->inv_L = 1.0 / el->L;
         el->c = dx * el->inv_L;
         el->s = dy * el->inv_L;
+        el->type = c->elem_type ? c->elem_type[e] : 0;
+        el->dc = dx / el->L;
+        el->ds = dy / el->L;
         el->stats = &stats;
         int fd[6];
         for (int k = 0; k < 3; ++k) {

@@ -27,6 +27,7 @@ typedef struct {
     const double *ts_values;
     double ts_dt, ts_factor, ts_start;
     int ts_use_last;
+    const int *elem_type;    /* 0 forceBeamColumn2d, 1 dispBeamColumn2d per element; NULL: all 0 */
 } or_tr2d_case;
 
 typedef struct {

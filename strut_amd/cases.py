@@ -192,7 +192,8 @@ def _init_tangent(d):
 
 def rc_frame2d_case(stories: int, bays: int, steps: int, dt: float, seed: int, pga_g: float,
                     story_h: float = 3.0, bay_w: float = 6.0, floor_mass: float = 50.0, nip: int = 5,
-                    strong: float = 6.0, tol: float = 1.0e-8, max_iters: int = 25, name: str = "") -> dict:
+                    strong: float = 6.0, tol: float = 1.0e-8, max_iters: int = 25, name: str = "",
+                    column_type: str = "forceBeamColumn2d", beam_type: str = "forceBeamColumn2d") -> dict:
     """2D RC moment frame in Strut's JSON schema: forceBeamColumn2d columns
     and beams (Lobatto, nip points, Linear), fixed bases, lumped floor masses
     on ux and uy of every floor node, Rayleigh 5 % on modes 1 and 3
@@ -217,12 +218,12 @@ def rc_frame2d_case(stories: int, bays: int, steps: int, dt: float, seed: int, p
     eid = 1
     for f in range(stories):
         for c in range(bays + 1):
-            elements.append({"id": eid, "type": "forceBeamColumn2d", "nodes": [node_at[(f, c)], node_at[(f + 1, c)]],
+            elements.append({"id": eid, "type": column_type, "nodes": [node_at[(f, c)], node_at[(f + 1, c)]],
                              "section": 1, "geomTransf": "Linear", "integration": "Lobatto", "num_int_pts": nip})
             eid += 1
     for f in range(1, stories + 1):
         for c in range(bays):
-            elements.append({"id": eid, "type": "forceBeamColumn2d", "nodes": [node_at[(f, c)], node_at[(f, c + 1)]],
+            elements.append({"id": eid, "type": beam_type, "nodes": [node_at[(f, c)], node_at[(f, c + 1)]],
                              "section": 2, "geomTransf": "Linear", "integration": "Lobatto", "num_int_pts": nip})
             eid += 1
     masses = []

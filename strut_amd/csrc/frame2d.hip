@@ -491,6 +491,107 @@ __device__ __forceinline__ double tree8_seq(const double *v) {
     return x0 + x1;
 }
 
+// dispBeamColumn2d (disp_beam_column2d.mojo:114-254, Linear, no element
+// loads) by the whole block: wavefront w evaluates integration point w at
+// the section deformation of the Hermite field; threads 0..41 then sum the
+// local tangent and force over the points in order, and wavefront 0 turns
+// them to global axes with the reference's in-place sequence (columns 0, 1,
+// 3, 4 of every row, then rows 0, 1 and 3, 4; elements/utils.mojo:156-240).
+__device__ __forceinline__ double dbc_axial_shape(int idx, double inv_L) {
+    return idx == 0 ? -inv_L : (idx == 3 ? inv_L : 0.0);
+}
+
+__device__ __forceinline__ double dbc_curvature_shape(int idx, double inv_L, double inv_L_sq, double xi) {
+    if (idx == 1) return (-6.0 + 12.0 * xi) * inv_L_sq;
+    if (idx == 2) return (-4.0 + 6.0 * xi) * inv_L;
+    if (idx == 4) return (6.0 - 12.0 * xi) * inv_L_sq;
+    if (idx == 5) return (-2.0 + 6.0 * xi) * inv_L;
+    return 0.0;
+}
+
+static_assert(sizeof(FbcScratch) >= 42 * sizeof(double), "dispBeamColumn2d sums its outputs in an FbcScratch");
+
+__device__ void dbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, const double *uf,
+                              double *out42, ElemShared *sh, const MatDev *mats, double *red, double *tmp) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nip = el.nip;
+    const double L = el.L, c = el.c, s = el.s;
+    const double inv_L = 1.0 / L;
+    const double inv_L_sq = inv_L * inv_L;
+    if (w < nip) {
+        double ug[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ug[k] = el.fmap[k] >= 0 ? uf[el.fmap[k]] : 0.0;
+        const double ul0 = c * ug[0] + s * ug[1], ul1 = -s * ug[0] + c * ug[1], ul2 = ug[2];
+        const double ul3 = c * ug[3] + s * ug[4], ul4 = -s * ug[3] + c * ug[4], ul5 = ug[5];
+        const double xi = el.xi[w];
+        const double b1 = dbc_curvature_shape(1, inv_L, inv_L_sq, xi), b2 = dbc_curvature_shape(2, inv_L, inv_L_sq, xi);
+        const double b4 = dbc_curvature_shape(4, inv_L, inv_L_sq, xi), b5 = dbc_curvature_shape(5, inv_L, inv_L_sq, xi);
+        const double eps0 = (-inv_L) * ul0 + inv_L * ul3;
+        const double kappa = b1 * ul1 + b2 * ul2 + b4 * ul4 + b5 * ul5;
+        const Sec2dOut r = wave_section_2d<false>(A.defs[el.def], A.tb, mats, A.c, A.t,
+                                                  A.elem_inst[I.elem_inst_base + e] + w, eps0, kappa, red);
+        if (lane == 0) {
+            double *last = A.last + I.last_base + 2 * kMaxNipDev * e + 2 * w;
+            last[0] = eps0;
+            last[1] = kappa;
+            double *o = sh->res[w];
+            o[0] = r.N;
+            o[1] = r.Mz;
+            o[2] = r.k11;
+            o[3] = r.k12;
+            o[4] = r.k22;
+        }
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 42) {  // local tangent entry (a, b) = (t / 6, t % 6), or force entry t - 36
+        const int a = t < 36 ? t / 6 : t - 36, b = t % 6;
+        double acc = 0.0;
+        for (int ip = 0; ip < nip; ++ip) {
+            const double *r = sh->res[ip];
+            const double xi = el.xi[ip], wL = el.w[ip] * L;
+            const double ba_n = dbc_axial_shape(a, inv_L), ba_m = dbc_curvature_shape(a, inv_L, inv_L_sq, xi);
+            if (t < 36) {
+                const double bb_n = dbc_axial_shape(b, inv_L), bb_m = dbc_curvature_shape(b, inv_L, inv_L_sq, xi);
+                const double dN = r[2] * bb_n + r[3] * bb_m;
+                const double dM = r[3] * bb_n + r[4] * bb_m;
+                acc += wL * (ba_n * dN + ba_m * dM);
+            } else {
+                acc += wL * (ba_n * r[0] + ba_m * r[1]);
+            }
+        }
+        tmp[t] = acc;
+    }
+    __syncthreads();
+    if (t < 36) {
+        const int a = t / 6, b = t % 6;
+        // column pass of row `row` at column b
+        auto colT = [&](int row) {
+            const double *k = tmp + 6 * row;
+            if (b == 0) return c * k[0] - s * k[1];
+            if (b == 1) return s * k[0] + c * k[1];
+            if (b == 3) return c * k[3] - s * k[4];
+            if (b == 4) return s * k[3] + c * k[4];
+            return k[b];
+        };
+        double v;
+        if (a == 0 || a == 3) v = c * colT(a) - s * colT(a + 1);
+        else if (a == 1 || a == 4) v = s * colT(a - 1) + c * colT(a);
+        else v = colT(a);
+        out42[t] = v;
+    } else if (t < 42) {
+        const double *f = tmp + 36;
+        const int a = t - 36;
+        double v;
+        if (a == 0) v = c * f[0] - s * f[1];
+        else if (a == 1) v = s * f[0] + c * f[1];
+        else if (a == 3) v = c * f[3] - s * f[4];
+        else if (a == 4) v = s * f[3] + c * f[4];
+        else v = f[a];
+        out42[t] = v;
+    }
+}
+
 // Commit every fiber section of the instance (transient_nonlinear.mojo:3081-3101):
 // one wavefront per (element, integration point) rebuilds the trial state of
 // the section's last evaluation into the committed arrays.
@@ -546,6 +647,11 @@ __device__ __forceinline__ void element_state(const Frame2dArgs &A, int inst, in
                               last[1]);
         }
         __syncthreads();
+    }
+    if (el.type == 1) {
+        dbc2d_element(A, I, el, e, A.vec + I.vec_base, A.scratch + I.scratch_base + 42 * e, &L.sh, mats, L.wred[w],
+                      reinterpret_cast<double *>(&L.scr[0]));
+        return;
     }
     const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
                                   A.scratch + I.scratch_base + 42 * e, &L.sh, L.qbuf[w], L.scr[w], mats, L.wred[w]);
@@ -807,6 +913,7 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         if (last_step) commit_instance(A, I, F);
         for (int e = tid; e < F.n_elem; e += nt) {
             const ElemDev &el = A.elems[F.elem_base + e];
+            if (el.type != 0) continue;  // forceBeamColumn2d only (helpers.mojo:1844-1845)
             double ug[6];
             for (int k = 0; k < 6; ++k) ug[k] = el.fmap[k] >= 0 ? uf[el.fmap[k]] : 0.0;
             const double cs = el.c2, sn = el.s2;
@@ -1166,7 +1273,7 @@ __global__ __launch_bounds__(kElemThreads) void frame2d_inst_kernel(Frame2dArgs 
 // instance has finished, stopped or used its step budget.  Instances never
 // wait for one another.  Each task's writes are made visible at device
 // scope before the task that reads them is published.
-constexpr long long kQueueSpinLimit = 1ll << 28;  // sleeps of a wait before the launch gives up (many seconds)
+constexpr long long kQueueSpinLimit = 1ll << 32;  // sleeps of a wait before the launch gives up (minutes)
 
 __device__ __forceinline__ void queue_push(const QueueDev &Q, int32_t task, unsigned long long idx) {
     int32_t *slot = Q.slots + (idx & (unsigned long long)Q.cap_mask);
@@ -1202,6 +1309,7 @@ __global__ void frame2d_queue_reset_kernel(QueueDev Q) {
     *Q.tail = 0ull;
     *Q.live = 0;
     *Q.abort = 0;
+    *Q.start = __builtin_amdgcn_s_memrealtime();
 }
 
 // One block per instance: the start step, the live count and the first
@@ -1268,8 +1376,11 @@ __global__ __launch_bounds__(kElemThreads) void frame2d_queue_kernel(Frame2dArgs
             if (threadIdx.x == 0) {
                 __threadfence();
                 const InstState st = A.state[inst];
+                // an instance parks after max_steps steps, or at its next
+                // system task once the launch's time budget has run out
                 s_more = st.status == 0 && st.step < A.insts[inst].n_steps &&
-                         !(st.setup_done && st.step - Q.step0[inst] >= max_steps);
+                         !(st.setup_done && st.step - Q.step0[inst] >= max_steps) &&
+                         !(Q.budget && __builtin_amdgcn_s_memrealtime() - *Q.start > Q.budget);
                 if (!s_more) __hip_atomic_fetch_add(Q.live, -1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
             __syncthreads();

@@ -141,7 +141,9 @@ int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *fram
         if (def < 0 || def >= (int)ctx->defs.size() || ctx->defs[def].dim != 2)
             return fail(ctx, SGPU_ERR_ARG, "element %d: needs a 2D fiber-section definition", e);
         const SecDefHost &sd = ctx->defs[def];
-        if (!sd.init_flex_valid)
+        const int type = d->elem_type ? d->elem_type[e] : 0;
+        if (type != 0 && type != 1) return fail(ctx, SGPU_ERR_ARG, "element %d: unknown element type %d", e, type);
+        if (type == 0 && !sd.init_flex_valid)
             return fail(ctx, SGPU_ERR_ARG, "element %d: section initial flexibility is singular", e);
         if ((int)sd.batch_chunk_begin.size() - 1 > 63)
             return fail(ctx, SGPU_ERR_ARG, "element %d: sections of more than 63 x 64 slots are not on the frame path", e);
@@ -155,12 +157,13 @@ int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *fram
             return fail(ctx, SGPU_ERR_ARG, "element %d: 1..%d integration points on the device path", e, kMaxNipDev);
         el.nip = nip;
         el.def = def;
+        el.type = type;
         const double dx = d->x[nj] - d->x[ni], dy = d->y[nj] - d->y[ni];
         el.L = std::hypot(dx, dy);
         if (el.L == 0.0) return fail(ctx, SGPU_ERR_ARG, "element %d: zero length", e);
         el.inv_L = 1.0 / el.L;
-        el.c = dx * el.inv_L;
-        el.s = dy * el.inv_L;
+        el.c = type == 1 ? dx / el.L : dx * el.inv_L;
+        el.s = type == 1 ? dy / el.L : dy * el.inv_L;
         el.L2 = std::sqrt(dx * dx + dy * dy);
         el.c2 = dx / el.L2;
         el.s2 = dy / el.L2;
@@ -171,7 +174,7 @@ int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *fram
         el.if00 = sd.init_flex[0];
         el.if01 = sd.init_flex[1];
         el.if11 = sd.init_flex[2];
-        el.if_valid = 1;
+        el.if_valid = sd.init_flex_valid;
         for (int k = 0; k < 3; ++k) {
             el.fmap[k] = free_idx[3 * ni + k];
             el.fmap[3 + k] = free_idx[3 * nj + k];
@@ -493,7 +496,8 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
         if ((rc = dalloc(ctx, &Q.slots, (size_t)cap))) return rc;
         HIP_OK(ctx, hipMemset(Q.slots, 0xff, sizeof(int32_t) * (size_t)cap));
         if ((rc = dalloc(ctx, &Q.pending, fh.insts.size())) || (rc = dalloc(ctx, &Q.live, 1)) ||
-            (rc = dalloc(ctx, &Q.step0, fh.insts.size())) || (rc = dalloc(ctx, &Q.abort, 1)))
+            (rc = dalloc(ctx, &Q.step0, fh.insts.size())) || (rc = dalloc(ctx, &Q.abort, 1)) ||
+            (rc = dalloc(ctx, &Q.start, 1)))
             return rc;
         Q.cap_mask = (int32_t)(cap - 1);
         Q.n_inst = (int32_t)fh.insts.size();
@@ -556,20 +560,32 @@ int sgpu_frame2d_launch_persistent(sgpu_ctx *ctx, int32_t max_steps) {
     return SGPU_OK;
 }
 
-int sgpu_frame2d_launch_queue(sgpu_ctx *ctx, int32_t max_steps) {
-    if (!ctx) return SGPU_ERR_ARG;
-    FrameHost &fh = frames_of(ctx);
-    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
-    if (max_steps <= 0) return fail(ctx, SGPU_ERR_ARG, "max_steps must be > 0");
+namespace {
+
+// A work-queue launch; budget_s > 0 parks every instance at its first
+// system task after that many seconds.
+int queue_launch(sgpu_ctx *ctx, FrameHost &fh, int32_t max_steps, double budget_s) {
+    QueueDev Q = fh.queue;
+    Q.budget = budget_s > 0.0 ? (unsigned long long)(budget_s * 1.0e8) + 1ull : 0ull;  // 100 MHz counter
     {
         ProfScope ps(ctx, SGPU_KERNEL_FRAME2D);
-        HIP_OK(ctx, launch_frame2d_queue(ctx->stream, fh.args, fh.queue, fh.workers, max_steps, fh.lds));
+        HIP_OK(ctx, launch_frame2d_queue(ctx->stream, fh.args, Q, fh.workers, max_steps, fh.lds));
     }
     int32_t abort_flag = 0;
     HIP_OK(ctx, hipMemcpyAsync(&abort_flag, fh.queue.abort, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
     if (abort_flag) return fail(ctx, SGPU_ERR_STATE, "work queue: a worker's wait exceeded its bound");
     return SGPU_OK;
+}
+
+}  // namespace
+
+int sgpu_frame2d_launch_queue(sgpu_ctx *ctx, int32_t max_steps) {
+    if (!ctx) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (max_steps <= 0) return fail(ctx, SGPU_ERR_ARG, "max_steps must be > 0");
+    return queue_launch(ctx, fh, max_steps, 0.0);
 }
 
 int sgpu_frame2d_set_schedule(sgpu_ctx *ctx, int schedule) {
@@ -599,16 +615,18 @@ namespace {
 // Launch one chunk of the given schedule, wait, and report the GM-steps the
 // instances advanced and whether any instance is still active.
 int run_chunk(sgpu_ctx *ctx, FrameHost &fh, int schedule, int rounds, std::vector<InstState> &st, int64_t &advanced,
-              int &active, double &seconds, bool timing) {
+              int &active, double &seconds, double queue_budget) {
     int64_t before = 0;
     for (const InstState &s : st) before += s.step;
     const auto t0 = std::chrono::steady_clock::now();
-    // a chunk is about rounds / 4 steps per instance for every schedule; the
-    // queue's regular (untimed-choice) chunks are longer, as its chunk end
-    // waits for the slowest instance
+    // a timed chunk is a few steps per instance for every schedule
     const int steps = rounds / 4 > 0 ? rounds / 4 : 1;
+    // a queue chunk is either timed (a wall-clock budget, so that a slow
+    // instance's tail does not set the measured rate) or runs the instances
+    // to their ends: a chunk end waits for the slowest instance while the
+    // others' workers idle
     int rc = schedule == SGPU_SCHEDULE_PERSISTENT ? sgpu_frame2d_launch_persistent(ctx, steps)
-             : schedule == SGPU_SCHEDULE_QUEUE    ? sgpu_frame2d_launch_queue(ctx, timing ? 2 : 4 * rounds)
+             : schedule == SGPU_SCHEDULE_QUEUE    ? queue_launch(ctx, fh, 1 << 30, queue_budget)
                                                   : sgpu_frame2d_launch(ctx, rounds);
     if (rc) return rc;
     HIP_OK(ctx, hipMemcpyAsync(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost,
@@ -658,9 +676,11 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
             double rate[2] = {0.0, 0.0};
             const int order[2] = {SGPU_SCHEDULE_ROUNDS, SGPU_SCHEDULE_QUEUE};
             if ((rc = ensure_graph(ctx, fh, rounds))) return rc;
+            double budget = 0.002;  // the queue's sample gets the rounds sample's time
             for (int k = 0; k < 2 && active > 0; ++k) {
-                if ((rc = run_chunk(ctx, fh, order[k], rounds, st, adv, active, sec, true))) return rc;
+                if ((rc = run_chunk(ctx, fh, order[k], rounds, st, adv, active, sec, budget))) return rc;
                 rate[k] = sec > 0.0 ? (double)adv / sec : 0.0;
+                if (k == 0 && sec > budget) budget = sec;
             }
             fh.rate_rounds = rate[0];
             fh.rate_queue = rate[1];
@@ -670,7 +690,7 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
             active_at_tune = active;
             continue;
         }
-        if ((rc = run_chunk(ctx, fh, fh.chosen, rounds, st, adv, active, sec, false))) return rc;
+        if ((rc = run_chunk(ctx, fh, fh.chosen, rounds, st, adv, active, sec, 0.0))) return rc;
         // measure again when the active set has halved, and every 64 chunks
         // (the cost of an instance's step changes as its records go nonlinear)
         if (fh.schedule == SGPU_SCHEDULE_AUTO && active > 0 &&

@@ -10,12 +10,16 @@ namespace sgpu {
 
 constexpr int kMaxNipDev = 5;  // integration points per element on the device path
 
-// One forceBeamColumn2d element (Linear transformation, no element loads).
+// One forceBeamColumn2d or dispBeamColumn2d element (Linear transformation,
+// no element loads).
 struct ElemDev {
     int32_t nip;
     int32_t def;        // fiber-section definition of its integration points
+    int32_t type;       // 0 forceBeamColumn2d, 1 dispBeamColumn2d
+    int32_t pad_;
     int32_t fmap[6];    // free-DOF index of the element DOFs, -1 if constrained
-    double L, inv_L, c, s;  // hypot length, 1/L, dx/L as dx*(1/L) (force_beam_column2d.mojo:383-408)
+    double L, inv_L, c, s;  // hypot length, 1/L, and dx/L as dx*(1/L) (force_beam_column2d.mojo:383-408)
+                            // or, for dispBeamColumn2d, as dx/L (disp_beam_column2d.mojo:137-140)
     double L2, c2, s2;      // sqrt length and dx/L2 of the committed sync (helpers.mojo:1855-1866)
     double xi[kMaxNipDev], w[kMaxNipDev];
     double if00, if01, if11;  // initial section flexibility (fiber2d.mojo:3448-3454)
@@ -109,6 +113,8 @@ struct QueueDev {
     int32_t *live;                    // instances still running in this launch
     int32_t *step0;                   // per instance: step at the launch
     int32_t *abort;                   // set when a wait exceeds its bound (the launch then fails)
+    unsigned long long *start;        // realtime counter at the launch
+    unsigned long long budget;        // 100 MHz ticks after which instances park (0: none)
     int32_t cap_mask;
     int32_t n_inst;
 };

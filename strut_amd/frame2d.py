@@ -51,10 +51,15 @@ class Frame2dCase:
     ts_start: float
     ts_use_last: int
     record_dofs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    elem_type: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))  # 0 FBC, 1 DBC
 
     @property
     def total_dofs(self) -> int:
         return 3 * self.n_nodes
+
+
+# Element types of the fiber-frame path and their codes in the device tables.
+ELEMENT_TYPES = {"forceBeamColumn2d": 0, "dispBeamColumn2d": 1}
 
 
 def load_frame2d_case(case: dict, steps: int | None = None) -> Frame2dCase:
@@ -77,13 +82,13 @@ def load_frame2d_case(case: dict, steps: int | None = None) -> Frame2dCase:
     elems = case["elements"]
     used = []
     for e in elems:
-        if e["type"] != "forceBeamColumn2d":
+        if e["type"] not in ELEMENT_TYPES:
             raise ValueError(f"element type {e['type']} is outside the fiber-frame path")
         if e.get("geomTransf", "Linear") != "Linear":
             raise ValueError("only the Linear geometric transformation is on the fiber-frame path")
         sid = int(e["section"])
         if sections[sid]["type"] != "FiberSection2d":
-            raise ValueError("forceBeamColumn2d needs a FiberSection2d on the fiber-frame path")
+            raise ValueError(f"{e['type']} needs a FiberSection2d on the fiber-frame path")
         if sid not in used:
             used.append(sid)
     cells = {sid: discretize_section(sections[sid], mats) for sid in used}
@@ -91,6 +96,7 @@ def load_frame2d_case(case: dict, steps: int | None = None) -> Frame2dCase:
     elem_nodes = np.array([[idx[int(e["nodes"][0])], idx[int(e["nodes"][1])]] for e in elems], dtype=np.int32)
     elem_section = np.array([int(e["section"]) for e in elems], dtype=np.int32)
     elem_nip = np.array([int(e.get("num_int_pts", 5)) for e in elems], dtype=np.int32)
+    elem_type = np.array([ELEMENT_TYPES[e["type"]] for e in elems], dtype=np.int32)
     elem_xi = np.zeros((ne, MAX_NIP))
     elem_w = np.zeros((ne, MAX_NIP))
     for k, e in enumerate(elems):
@@ -138,4 +144,4 @@ def load_frame2d_case(case: dict, steps: int | None = None) -> Frame2dCase:
         max_iters=int(an.get("max_iters", 20)), alpha_m=float(ray.get("alphaM", 0.0)),
         beta_k=float(ray.get("betaK", 0.0)), beta_k_init=float(ray.get("betaKInit", 0.0)), exc_dir=exc_dir,
         ts_values=ts_values, ts_dt=ts_dt, ts_factor=ts_factor, ts_start=ts_start, ts_use_last=ts_use_last,
-        record_dofs=np.array(rec, dtype=np.int32))
+        record_dofs=np.array(rec, dtype=np.int32), elem_type=elem_type)

@@ -33,7 +33,7 @@ class FrameDesc(ctypes.Structure):
         ("max_iters", c_int32),
         ("alpha_m", c_double), ("beta_k", c_double), ("beta_k_init", c_double),
         ("exc_dir", c_int32),
-        ("n_record", c_int32), ("record_dofs", POINTER(c_int32)),
+        ("n_record", c_int32), ("record_dofs", POINTER(c_int32)), ("elem_type", POINTER(c_int32)),
     ]
 
 
@@ -98,7 +98,9 @@ class FrameBatch:
                  ed=np.array([defs[int(s)] for s in fc.elem_section], dtype=np.int32),
                  nip=np.ascontiguousarray(fc.elem_nip, dtype=np.int32),
                  xi=np.ascontiguousarray(fc.elem_xi.reshape(-1)), w=np.ascontiguousarray(fc.elem_w.reshape(-1)),
-                 rec=np.ascontiguousarray(rec))
+                 rec=np.ascontiguousarray(rec),
+                 et=np.ascontiguousarray(fc.elem_type if len(fc.elem_type) else np.zeros(len(fc.elem_nip)),
+                                         dtype=np.int32))
         assert fc.elem_xi.shape[1] == MAX_NIP
         d = FrameDesc()
         d.n_nodes = fc.n_nodes
@@ -112,6 +114,7 @@ class FrameBatch:
         d.alpha_m, d.beta_k, d.beta_k_init, d.exc_dir = fc.alpha_m, fc.beta_k, fc.beta_k_init, fc.exc_dir
         d.n_record = len(rec)
         d.record_dofs = iptr(k["rec"])
+        d.elem_type = iptr(k["et"])
         fid = c_int32()
         self._check(self.lib.sgpu_frame2d_create(self.rt.ctx, ctypes.byref(d), ctypes.byref(fid)))
         self.frames.append(fc)

@@ -73,3 +73,32 @@ def test_gpu_ensemble_of_uneven_instances_matches_oracle(schedule):
         np.testing.assert_array_equal(got["iters"], ref["iters"])
         assert _rel_err(got["u"], ref["u"]) <= REL
     fb.close()
+
+
+@pytest.mark.parametrize("column_type,beam_type", [("forceBeamColumn2d", "dispBeamColumn2d"),
+                                                   ("dispBeamColumn2d", "dispBeamColumn2d")],
+                         ids=["fbc-columns-dbc-beams", "dbc-all"])
+def test_gpu_disp_beam_column_frames_match_oracle(column_type, beam_type):
+    """dispBeamColumn2d elements (disp_beam_column2d.mojo:114-254) in the
+    3-story frame, alone or with forceBeamColumn2d columns: identical Newton
+    counts and displacements within 1e-8 of the oracle."""
+    case = cases.rc_frame2d_case(3, 1, steps=300, dt=0.01, seed=0, pga_g=0.4,
+                                 column_type=column_type, beam_type=beam_type)
+    fc = load_frame2d_case(case)
+    assert set(fc.elem_type.tolist()) == ({0, 1} if column_type != beam_type else {1})
+    ref = oracle.run_transient2d(fc)
+    fb = FrameBatch(fc.materials)
+    f = fb.add_frame(fc, record_all=True)
+    i = fb.add_case_motion(f, fc)
+    fb.finalize()
+    fb.run(rounds=32)
+    got = fb.result(i)
+    fb.close()
+    n = ref["steps_done"]
+    print(f"{column_type}/{beam_type}: status {got['status']} vs {ref['status']}, "
+          f"iterations {got['iters'][:n].sum()} vs {ref['iters'][:n].sum()}")
+    assert got["status"] == ref["status"] and got["steps_done"] == n
+    np.testing.assert_array_equal(got["iters"][:n], ref["iters"][:n])
+    err = _rel_err(got["u"][:n], ref["u"][:n])
+    print(f"max relative displacement error {err:.3e}")
+    assert err <= REL

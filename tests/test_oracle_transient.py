@@ -214,3 +214,64 @@ def test_force_beam_column_state_satisfies_equilibrium_and_compatibility():
         # compatibility: v = sum_i w_i L b_i^T e_i
         assert np.abs(vr - vb).max() <= 1e-6 * max(np.abs(vb).max(), 1e-12)
     assert checked == 5 * len(fc.elem_nip)
+
+
+def test_oracle_dbc2d_elastic_element_closed_form():
+    """dispBeamColumn2d on an elastic section: Lobatto-5 integrates the
+    Hermite curvature field exactly, so the tangent is the Euler-Bernoulli
+    stiffness (EA/L, 12EI/L^3, 6EI/L^2, 4EI/L, 2EI/L) rotated to the
+    element's axis, and the resisting force is that stiffness times u."""
+    from strut_amd.model import MaterialTable, beam_integration_rule, discretize_section
+
+    mats = MaterialTable([{"id": 1, "type": "Elastic", "params": {"E": 30.0e6}}])
+    sec = {"id": 1, "type": "FiberSection2d", "params": {"patches": [
+        {"type": "rect", "material": 1, "num_subdiv_y": 8, "num_subdiv_z": 2, "y_i": -0.25, "z_i": -0.15,
+         "y_j": 0.25, "z_j": 0.15}]}}
+    cells = discretize_section(sec, mats)
+    t, p = mats.arrays()
+    s = oracle.Section2d(cells, t, p)
+    first = s.alloc(5)
+    xi, w = beam_integration_rule("Lobatto", 5)
+    x1, y1, x2, y2 = 1.0, 2.0, 1.0 + 4.0 * np.cos(0.5), 2.0 + 4.0 * np.sin(0.5)
+    L = 4.0
+    E, A = 30.0e6, 0.5 * 0.3
+    I = 0.3 * 0.5 ** 3 / 12.0 * (1.0 - 1.0 / 64.0)  # 8 layers: sum of A_i y_i^2
+    kl = np.zeros((6, 6))
+    kl[np.ix_([0, 3], [0, 3])] = E * A / L * np.array([[1, -1], [-1, 1]])
+    kb = E * I / L ** 3 * np.array([[12, 6 * L, -12, 6 * L], [6 * L, 4 * L * L, -6 * L, 2 * L * L],
+                                    [-12, -6 * L, 12, -6 * L], [6 * L, 2 * L * L, -6 * L, 4 * L * L]])
+    kl[np.ix_([1, 2, 4, 5], [1, 2, 4, 5])] = kb
+    c, sn = np.cos(0.5), np.sin(0.5)
+    T = np.zeros((6, 6))
+    for b in (0, 3):
+        T[b:b + 2, b:b + 2] = [[c, sn], [-sn, c]]
+        T[b + 2, b + 2] = 1.0
+    want = T.T @ kl @ T
+    u = np.array([1e-4, -2e-4, 3e-4, -1e-4, 2e-4, -4e-4])
+    k, f = oracle.dbc2d_eval(s, first, xi, w, x1, y1, x2, y2, u)
+    np.testing.assert_allclose(k, want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    np.testing.assert_allclose(f, want @ u, rtol=1e-9, atol=1e-10 * np.abs(want @ u).max())
+    # a rigid-body translation and rotation produce no force
+    rb = np.array([1e-3, 2e-3, 1e-4, 1e-3 - 1e-4 * (y2 - y1), 2e-3 + 1e-4 * (x2 - x1), 1e-4])
+    _, f0 = oracle.dbc2d_eval(s, first, xi, w, x1, y1, x2, y2, rb)
+    assert np.max(np.abs(f0)) <= 1e-9 * np.abs(want).max() * 1e-3
+
+
+def test_oracle_elastic_frame_dbc_equals_fbc():
+    """With elastic fiber sections both element formulations are exact for
+    the Euler-Bernoulli frame (Lobatto-5 integrates the cubic Hermite field
+    exactly; the force-based element's equilibrium field is exact), so the
+    transient histories of the two agree to rounding."""
+    def elastic(case):
+        case = dict(case)
+        case["materials"] = [{"id": m["id"], "type": "Elastic", "params": {"E": 30.0e6}} for m in case["materials"]]
+        return case
+    runs = {}
+    for et in ("forceBeamColumn2d", "dispBeamColumn2d"):
+        case = elastic(cases.rc_frame2d_case(3, 1, steps=150, dt=0.01, seed=3, pga_g=0.3,
+                                             column_type=et, beam_type=et))
+        fc = load_frame2d_case(case)
+        runs[et] = oracle.run_transient2d(fc)
+        assert runs[et]["status"] == 0
+    a, b = runs["forceBeamColumn2d"]["u"], runs["dispBeamColumn2d"]["u"]
+    assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(a))
