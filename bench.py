@@ -158,7 +158,7 @@ def section_leg(args) -> dict:
     rt.synchronize()
     torch.cuda.synchronize()
 
-    rt.profile_enable(True)
+    # timed steps: the launches alone (no per-kernel events on the stream)
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     t_start.record(stream)
@@ -168,10 +168,16 @@ def section_leg(args) -> dict:
     rt.synchronize()
     torch.cuda.synchronize()
     elapsed_ms = t_start.elapsed_time(t_end)
-    main_ms, main_n = rt.profile_read(_native.KERNEL_SECTION_EVAL)
-    aux_ms, aux_n = rt.profile_read(_native.KERNEL_AUX_EVAL)
     res = d_res.cpu().numpy().view(RESULT_DTYPE)
     assert np.all(res["valid"] == 1), "a section flexibility was singular"
+    # the same steps again with HIP events around each kernel (library
+    # profiling on the context's stream): per-kernel launch durations
+    rt.profile_enable(True)
+    for k in range(args.steps):
+        rt.eval_device(n_pts, d_pts[args.warmup + k].data_ptr(), d_res.data_ptr())
+    rt.synchronize()
+    main_ms, main_n = rt.profile_read(_native.KERNEL_SECTION_EVAL)
+    aux_ms, aux_n = rt.profile_read(_native.KERNEL_AUX_EVAL)
 
     rt.profile_enable(True)
     for _ in range(args.steps):
@@ -392,6 +398,10 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
     return out
 
 
+def _progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +424,7 @@ def main() -> int:
         return 2
 
     sec = section_leg(args)
+    _progress(f"section leg: {sec['value']:.4g} fiber updates/s")
     config = {
         "workload": "section population of the 3D RC frame (20 stories x 5x5 bays): "
                     f"{sec['pop']['columns']} columns + {sec['pop']['beams']} beams x 5 IPs = {sec['n_pts']} "
@@ -428,6 +439,7 @@ def main() -> int:
     }
     if args.gm_batch > 0:
         gm = gm_leg(args.gm_batch, args.gm_steps)
+        _progress(f"GM-steps leg: batched/plain {gm['batched_vs_plain']:.3g}")
         if not args.no_cpu_baseline:
             gm["cpu_baseline"] = gm_cpu_baseline(args.gm_steps)
         config["gm_steps"] = {"workload": f"2D RC frame 3 stories x 1 bay, forceBeamColumn 5 IPs, {args.gm_steps} "
@@ -435,9 +447,11 @@ def main() -> int:
                               **gm}
     if args.ens_motions > 0:
         config["ensemble"] = ensemble_leg(args.ens_motions, args.ens_steps)
+        _progress(f"ensemble leg: batched/one-by-one {config['ensemble']['batched_vs_one_by_one']:.3g}")
     if args.learned_train > 0:
         config["learned"] = learned_leg(args.learned_train, args.learned_test, args.learned_steps,
                                         args.learned_epochs)
+        _progress(f"learned leg: iteration reduction {config['learned']['newton_iter_reduction']:.3g}")
     out = {
         "metric": "fiber updates/s",
         "value": sec["value"],

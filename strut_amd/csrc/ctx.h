@@ -60,7 +60,8 @@ struct sgpu_ctx {
     SecDefDev *d_defs = nullptr;
     double *d_tab_y = nullptr, *d_tab_z = nullptr, *d_tab_area = nullptr;
     int32_t *d_tab_mat = nullptr, *d_tab_aux = nullptr, *d_tab_chunk = nullptr, *d_tab_bcb = nullptr,
-            *d_tab_cslot = nullptr, *d_tab_auxslot = nullptr;
+            *d_tab_cslot = nullptr, *d_tab_auxslot = nullptr, *d_tab_auxmat = nullptr;
+    double *d_tab_auxy = nullptr, *d_tab_auxz = nullptr, *d_tab_auxarea = nullptr;
     int max_aux = 0;  // most aux slots of any definition
     StateArrays c{}, t{};
     std::vector<void *> allocations;
@@ -91,6 +92,10 @@ struct sgpu_ctx {
         tb.batch_chunk_begin = d_tab_bcb;
         tb.chunk_slot = d_tab_cslot;
         tb.aux_slot = d_tab_auxslot;
+        tb.aux_mat = d_tab_auxmat;
+        tb.aux_y = d_tab_auxy;
+        tb.aux_z = d_tab_auxz;
+        tb.aux_area = d_tab_auxarea;
         return tb;
     }
 };

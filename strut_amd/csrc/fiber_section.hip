@@ -24,7 +24,8 @@ namespace sgpu {
 
 constexpr int kPointsPerBlock = 4;         // one wavefront per section point
 constexpr int kRedStride = kBatch + 1;     // LDS row stride (odd, so the NQ rows sit in different banks)
-constexpr bool kPrefetch = true;           // load batch b + 1 while batch b is reduced
+constexpr int kLdsMaxMats = 128;           // material tables up to this size are staged in LDS
+constexpr int kLdsMaxAux = 128;            // aux results of a point staged in LDS up to this many slots
 
 // Section term q of one slot from its fiber force fs = sigma A and stiffness
 // ks = E_t A (fiber2d.mojo:2065-2082, fiber3d.mojo:804-822).
@@ -60,24 +61,19 @@ struct SlotIn {
     double eps, sig, tan, h0, h1;
 };
 
-__device__ __forceinline__ void load_slot_in(const SecDefDev &d, const SlotTables &tb, const StateArrays &c,
-                                             int64_t inst_state, int s, bool dim3, SlotIn &in) {
-    in.mi = -1;
-    in.code = -1;
-    in.aux = -1;
-    in.y = in.z = in.area = 0.0;
-    in.eps = in.sig = in.tan = in.h0 = in.h1 = 0.0;
-    if (s >= d.n_slots) return;
-    const int64_t gs = d.slot_base + s;
-    const int64_t gi = inst_state + s;
+// gs = table index of the slot, gi = its state index.  Every field is loaded
+// (z only in 3D): a lane past the section's slots reads slot 0 of its
+// instance and is masked by slot_in_ready, so no register of a set is left
+// with a load in flight when the set is loaded again.
+template <int DIM>
+__device__ __forceinline__ void load_slot_in(const SlotTables &tb, const StateArrays &c, int64_t gs, int64_t gi,
+                                             SlotIn &in) {
     in.mi = tb.mat[gs];
     in.code = tb.chunk[gs];
     in.aux = tb.aux[gs];
     in.y = tb.y[gs];
     in.area = tb.area[gs];
-    if (dim3) in.z = tb.z[gs];
-    // The committed history is read for every slot (the address does not
-    // depend on the table), so the loads of a batch issue together.
+    in.z = DIM == 3 ? tb.z[gs] : 0.0;
     in.eps = c.eps[gi];
     in.sig = c.sig[gi];
     in.tan = c.tan[gi];
@@ -85,101 +81,91 @@ __device__ __forceinline__ void load_slot_in(const SecDefDev &d, const SlotTable
     in.h1 = c.h1[gi];
 }
 
-// One section point, evaluated by one wavefront.  Lanes walk the slots 64 at
-// a time; each batch leaves its chunk partials in the wave's LDS rows, and
-// lanes 0..NQ-1 add them one after another onto their running sums, so the
-// sums see the chunks in exactly the sequential order of the reference.
-template <int DIM>
-__device__ void section_point(const SecDefDev &d, const SlotTables &tb, const MatDev *__restrict__ mats,
-                              const StateArrays &t, int inst, double eps0, double kz, double ky,
-                              const double *__restrict__ fsks, double *red, const StateArrays &c,
-                              sgpu_section_result *out) {
-    constexpr int NQ = DIM == 2 ? 5 : 9;
-    const int lane = threadIdx.x & (kBatch - 1);
-    const int64_t inst_state = d.state_base + (int64_t)inst * d.n_slots;
-    const int32_t *bcb = tb.batch_chunk_begin + d.batch_base;
-    const bool zero_lane = d.zero_k12 && (DIM == 2 ? lane == 3 : (lane == 4 || lane == 5));
-    double acc = 0.0;
-    SlotIn cur;
-    load_slot_in(d, tb, c, inst_state, lane, DIM == 3, cur);
-    for (int b = 0; b < d.n_batches; ++b) {
-        SlotIn nxt;
-        if (kPrefetch && b + 1 < d.n_batches) load_slot_in(d, tb, c, inst_state, (b + 1) * kBatch + lane, DIM == 3, nxt);
-        double fs = 0.0, ks = 0.0;
-        if (cur.mi >= 0) {
-            if (cur.aux >= 0) {
-                fs = fsks[2 * cur.aux];
-                ks = fsks[2 * cur.aux + 1];
-            } else {
-                const MatDev &m = mats[cur.mi];
-                const double strain = DIM == 2 ? eps0 - cur.y * kz : eps0 + cur.z * ky - cur.y * kz;
-                if (m.type == kElastic) {
-                    ks = m.p[0] * cur.area;
-                    fs = ks * strain;
-                } else {
-                    FiberHist hc, ht;
-                    hc.eps = cur.eps;
-                    hc.sig = cur.sig;
-                    hc.tan = cur.tan;
-                    hc.h0 = cur.h0;
-                    hc.h1 = cur.h1;
-                    hc.a0 = hc.a1 = hc.a2 = hc.a3 = hc.a4 = 0.0;
-                    hc.kon = 0;
-                    ht = hc;
-                    if (m.type == kConcrete02)
-                        concrete02_trial(m, hc, strain, ht);
-                    else
-                        steel01_trial(m, hc, strain, ht);
-                    const int64_t gi = inst_state + b * kBatch + lane;
-                    t.eps[gi] = ht.eps;
-                    t.sig[gi] = ht.sig;
-                    t.tan[gi] = ht.tan;
-                    t.h0[gi] = ht.h0;
-                    t.h1[gi] = ht.h1;
-                    fs = ht.sig * cur.area;
-                    ks = ht.tan * cur.area;
-                }
-            }
-        }
-        const int cbase = bcb[b];
-        const int dst = cur.code >= 0 ? (cur.code >> 1) : -1;
-        const bool w8 = cur.code >= 0 && (cur.code & 1);
-        if (__builtin_amdgcn_ballot_w64(w8) != 0) {
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const double term = slot_term<DIM>(q, fs, ks, cur.y, cur.z);
-                const double tr = tree8_dpp(term);
-                if (dst >= 0) red[q * kRedStride + dst] = w8 ? tr : term;
-            }
-        } else {  // every chunk of the batch is a single slot
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                if (dst >= 0) red[q * kRedStride + dst] = slot_term<DIM>(q, fs, ks, cur.y, cur.z);
-        }
-        wave_sync();
-        if (lane < NQ) {
-            const int c1 = bcb[b + 1];
-            const double *row = red + lane * kRedStride - cbase;
-            int cc = cbase;
-            if (zero_lane && d.n_elastic_chunks >= cbase && d.n_elastic_chunks < c1) {
-                for (; cc < d.n_elastic_chunks; ++cc) acc += row[cc];
-                acc = 0.0;  // uniform elastic modulus: k12 (k13) restarts with the nonlinear families
-            }
-#pragma unroll 8
-            for (; cc < c1; ++cc) acc += row[cc];
-        }
-        wave_sync();
-        if (kPrefetch) {
-            cur = nxt;
-        } else if (b + 1 < d.n_batches) {
-            load_slot_in(d, tb, c, inst_state, (b + 1) * kBatch + lane, DIM == 3, cur);
-        }
+// Wait for every load of a set before it is used (an empty asm that reads
+// the registers), then blank the lanes past the section's slots.
+__device__ __forceinline__ void slot_in_ready(SlotIn &in, bool valid) {
+    asm volatile("" ::"v"(in.mi), "v"(in.code), "v"(in.aux), "v"(in.y), "v"(in.z), "v"(in.area), "v"(in.eps),
+                 "v"(in.sig), "v"(in.tan), "v"(in.h0), "v"(in.h1)
+                 : "memory");
+    if (!valid) {
+        in.mi = -1;
+        in.code = -1;
+        in.aux = -1;
+        in.y = in.z = in.area = 0.0;
     }
-    if (zero_lane && bcb[d.n_batches] <= d.n_elastic_chunks) acc = 0.0;
-    double fin[9];
+}
+
+// (fs, ks) = (sigma A, E_t A) of one slot of a batch, from its loaded
+// committed history; the trial history of a material slot is stored at gi.
+// Aux slots (Steel02, Concrete01) take the aux pre-pass result.
+template <int DIM, typename MatPtr, typename AuxPtr>
+__device__ __forceinline__ void slot_eval(const SlotIn &cur, MatPtr mats, const StateArrays &t, int64_t gi,
+                                          double eps0, double kz, double ky, AuxPtr fsks, double &fs, double &ks) {
+    fs = 0.0;
+    ks = 0.0;
+    if (cur.mi < 0) return;
+    if (cur.aux >= 0) {
+        fs = fsks[2 * cur.aux];
+        ks = fsks[2 * cur.aux + 1];
+        return;
+    }
+    const MatDev &m = mats[cur.mi];
+    const double strain = DIM == 2 ? eps0 - cur.y * kz : eps0 + cur.z * ky - cur.y * kz;
+    if (m.type == kElastic) {
+        ks = m.p[0] * cur.area;
+        fs = ks * strain;
+        return;
+    }
+    FiberHist hc, ht;
+    hc.eps = cur.eps;
+    hc.sig = cur.sig;
+    hc.tan = cur.tan;
+    hc.h0 = cur.h0;
+    hc.h1 = cur.h1;
+    hc.a0 = hc.a1 = hc.a2 = hc.a3 = hc.a4 = 0.0;
+    hc.kon = 0;
+    ht = hc;
+    if (m.type == kConcrete02)
+        concrete02_trial(m, hc, strain, ht);
+    else
+        steel01_trial(m, hc, strain, ht);
+    t.eps[gi] = ht.eps;
+    t.sig[gi] = ht.sig;
+    t.tan[gi] = ht.tan;
+    t.h0[gi] = ht.h0;
+    t.h1[gi] = ht.h1;
+    fs = ht.sig * cur.area;
+    ks = ht.tan * cur.area;
+}
+
+// The chunk partials of one batch: W-chunks folded by the halving tree,
+// single-slot chunks as they are.  Chunk k of the batch (its code >> 1)
+// lands in red[q * stride + cbase + k]; the whole wave must be active.
+template <int DIM>
+__device__ __forceinline__ void batch_partials(const SlotIn &cur, double fs, double ks, double *red, int stride,
+                                               int cbase) {
+    constexpr int NQ = DIM == 2 ? 5 : 9;
+    const int dst = cur.code >= 0 ? cbase + (cur.code >> 1) : -1;
+    const bool w8 = cur.code >= 0 && (cur.code & 1);
+    if (__builtin_amdgcn_ballot_w64(w8) != 0) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) fin[q] = readlane_d(acc, q);
-    if (lane != 0) return;
+        for (int q = 0; q < NQ; ++q) {
+            const double term = slot_term<DIM>(q, fs, ks, cur.y, cur.z);
+            const double tr = tree8_dpp(term);
+            if (dst >= 0) red[q * stride + dst] = w8 ? tr : term;
+        }
+    } else {  // every chunk of the batch is a single slot
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            if (dst >= 0) red[q * stride + dst] = slot_term<DIM>(q, fs, ks, cur.y, cur.z);
+    }
+}
+
+// Section resultants, tangent and flexibility of one point from its reduced
+// sums fin[0..NQ) (fiber2d.mojo:3074-3085; force_beam_column3d.mojo:679-727
+// applied as :1093-1117).
+template <int DIM>
+__device__ __forceinline__ void finish_point(const double *fin, sgpu_section_result *out) {
     sgpu_section_result r;
     for (int q = 0; q < 3; ++q) r.s[q] = 0.0;
     for (int q = 0; q < 6; ++q) {
@@ -243,6 +229,96 @@ __device__ void section_point(const SecDefDev &d, const SlotTables &tb, const Ma
     *out = r;
 }
 
+// Batch b of one section point: evaluate the lanes' slots, fold the chunk
+// partials into the wave's LDS rows, and let lanes 0..NQ-1 add them one after
+// another onto their running sums acc (chunks [cbase, c1) of the section).
+template <int DIM, typename MatPtr, typename AuxPtr>
+__device__ __forceinline__ void section_batch(SlotIn cur, int b, int cbase, int c1, int n_slots, int ne,
+                                              MatPtr mats, const StateArrays &t, int64_t inst_state, double eps0,
+                                              double kz, double ky, AuxPtr fsks, double *red, bool zero_lane,
+                                              double &acc) {
+    constexpr int NQ = DIM == 2 ? 5 : 9;
+    const int lane = threadIdx.x & (kBatch - 1);
+    slot_in_ready(cur, b * kBatch + lane < n_slots);
+    double fs, ks;
+    slot_eval<DIM>(cur, mats, t, inst_state + b * kBatch + lane, eps0, kz, ky, fsks, fs, ks);
+    batch_partials<DIM>(cur, fs, ks, red, kRedStride, 0);
+    wave_sync();
+    if (lane < NQ) {
+        const double *row = red + lane * kRedStride - cbase;
+        int cc = cbase;
+        if (zero_lane && ne >= cbase && ne < c1) {
+            for (; cc < ne; ++cc) acc += row[cc];
+            acc = 0.0;  // uniform elastic modulus: k12 (k13) restarts with the nonlinear families
+        }
+#pragma unroll 8
+        for (; cc < c1; ++cc) acc += row[cc];
+    }
+    wave_sync();
+}
+
+// One section point, evaluated by one wavefront.  Lanes walk the slots 64 at
+// a time; each batch leaves its chunk partials in the wave's LDS rows, and
+// lanes 0..NQ-1 add them one after another onto their running sums, so the
+// sums see the chunks in exactly the sequential order of the reference.
+// Two register sets A and B alternate: while batch b is evaluated the loads
+// of batch b + 1 are in flight, and batch b + 2's loads issue as soon as b
+// is done, with no copy between the sets (a copy would wait for the loads).
+// Batches run in pages of 62 whose chunk ranges sit in one register.
+// Every other read inside the loop comes from registers or LDS (material
+// table, aux results, chunk ranges): vector-memory reads return in issue
+// order, so waiting for one there would wait for the batches in flight too.
+template <int DIM, typename MatPtr, typename AuxPtr>
+__device__ __forceinline__ void section_point(const SecDefDev &d, const SlotTables &tb, MatPtr mats,
+                                              const StateArrays &t, int inst, double eps0, double kz, double ky,
+                                              AuxPtr fsks, double *red, const StateArrays &c,
+                                              sgpu_section_result *out) {
+    constexpr int NQ = DIM == 2 ? 5 : 9;
+    const int lane = threadIdx.x & (kBatch - 1);
+    const int n_slots = d.n_slots, n_batches = d.n_batches;
+    const int64_t slot_base = d.slot_base;
+    const int64_t inst_state = d.state_base + (int64_t)inst * n_slots;
+    const int32_t *bcb = tb.batch_chunk_begin + d.batch_base;
+    const int ne = d.n_elastic_chunks;
+    const bool zero_lane = d.zero_k12 && (DIM == 2 ? lane == 3 : (lane == 4 || lane == 5));
+    // slot index of a lane in batch b, or 0 past the section's slots
+    auto slot_of = [&](int b) { const int sl = b * kBatch + lane; return sl < n_slots ? sl : 0; };
+    double acc = 0.0;
+    int n_chunks = 0;
+    // pages of up to kPage batches; a page's chunk range starts are held one
+    // per lane (bcb[pb + l] in lane l), read once before its batch loads issue
+    constexpr int kPage = kBatch - 2;
+    for (int pb = 0; pb < n_batches; pb += kPage) {
+        const int pn = n_batches - pb < kPage ? n_batches - pb : kPage;
+        const int lane_bcb = lane <= pn ? bcb[pb + lane] : 0;
+        asm volatile("" ::"v"(lane_bcb) : "memory");
+        n_chunks = __builtin_amdgcn_readlane(lane_bcb, pn);
+        // The set loads are unconditional (past the section a lane reads slot
+        // 0), so that every path issues the same loads and the wait for a set
+        // leaves the other set's loads in flight.
+        SlotIn A, B;
+        load_slot_in<DIM>(tb, c, slot_base + slot_of(pb), inst_state + slot_of(pb), A);
+        load_slot_in<DIM>(tb, c, slot_base + slot_of(pb + 1), inst_state + slot_of(pb + 1), B);
+        for (int k = 0; k < pn; k += 2) {
+            section_batch<DIM>(A, pb + k, __builtin_amdgcn_readlane(lane_bcb, k),
+                               __builtin_amdgcn_readlane(lane_bcb, k + 1), n_slots, ne, mats, t, inst_state, eps0, kz,
+                               ky, fsks, red, zero_lane, acc);
+            load_slot_in<DIM>(tb, c, slot_base + slot_of(pb + k + 2), inst_state + slot_of(pb + k + 2), A);
+            if (k + 1 < pn)
+                section_batch<DIM>(B, pb + k + 1, __builtin_amdgcn_readlane(lane_bcb, k + 1),
+                                   __builtin_amdgcn_readlane(lane_bcb, k + 2), n_slots, ne, mats, t, inst_state, eps0,
+                                   kz, ky, fsks, red, zero_lane, acc);
+            load_slot_in<DIM>(tb, c, slot_base + slot_of(pb + k + 3), inst_state + slot_of(pb + k + 3), B);
+        }
+    }
+    if (zero_lane && n_chunks <= ne) acc = 0.0;
+    double fin[9];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) fin[q] = readlane_d(acc, q);
+    if (lane != 0) return;
+    finish_point<DIM>(fin, out);
+}
+
 __device__ __forceinline__ bool point_ok(const SecDefDev *__restrict__ defs, int n_defs,
                                          const sgpu_section_point &pt) {
     return pt.def_id >= 0 && pt.def_id < n_defs && pt.instance >= 0 && pt.instance < defs[pt.def_id].n_inst;
@@ -270,12 +346,12 @@ __global__ __launch_bounds__(256) void aux_eval_kernel(const SecDefDev *__restri
     const int64_t inst_state = d.state_base + (int64_t)pt.instance * d.n_slots;
     const int64_t inst_aux = d.aux_base + (int64_t)pt.instance * d.n_aux;
     for (int j = threadIdx.x - k * per; j < d.n_aux; j += per) {
-        const int s = tb.aux_slot[d.aux_slot_base + j];
-        const int64_t gs = d.slot_base + s;
-        const int mi = tb.mat[gs];
-        const double y = tb.y[gs];
-        const double area = tb.area[gs];
-        const double strain = d.dim == 2 ? pt.eps0 - y * pt.kappa_z : pt.eps0 + tb.z[gs] * pt.kappa_y - y * pt.kappa_z;
+        const int64_t ja = d.aux_slot_base + j;
+        const int s = tb.aux_slot[ja];
+        const int mi = tb.aux_mat[ja];
+        const double y = tb.aux_y[ja];
+        const double area = tb.aux_area[ja];
+        const double strain = d.dim == 2 ? pt.eps0 - y * pt.kappa_z : pt.eps0 + tb.aux_z[ja] * pt.kappa_y - y * pt.kappa_z;
         double fs, ks;
         eval_slot(mats, mi, strain, area, c, t, inst_state + s, inst_aux + j, fs, ks);
         double *o = fsks + ((int64_t)p * max_aux + j) * 2;
@@ -286,18 +362,31 @@ __global__ __launch_bounds__(256) void aux_eval_kernel(const SecDefDev *__restri
 
 // Main pass: every point's slots without aux state, and the section
 // reduction.  Wave w of a block evaluates point blockIdx.x * kPointsPerBlock + w.
+// LDS variant (kLds): the material table and each point's aux results are
+// staged in LDS first (dynamic LDS: n_mats MatDev, then max_aux (fs, ks)
+// pairs per wave).
+template <bool kLds>
 __global__ __launch_bounds__(kPointsPerBlock * kBatch) void section_eval_kernel(
-    const SecDefDev *__restrict__ defs, int n_defs, SlotTables tb, const MatDev *__restrict__ mats, StateArrays c,
-    StateArrays t, const sgpu_section_point *__restrict__ pts, sgpu_section_result *__restrict__ res, int n,
-    const double *__restrict__ fsks, int max_aux) {
+    const SecDefDev *__restrict__ defs, int n_defs, SlotTables tb, const MatDev *__restrict__ mats, int n_mats,
+    StateArrays c, StateArrays t, const sgpu_section_point *__restrict__ pts, sgpu_section_result *__restrict__ res,
+    int n, const double *__restrict__ fsks, int max_aux) {
     __shared__ double red[kPointsPerBlock][9 * kRedStride];
+    extern __shared__ double lds_dyn[];
+    MatDev *smat = reinterpret_cast<MatDev *>(lds_dyn);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kBatch - 1);
+    if (kLds) {
+        constexpr int per = sizeof(MatDev) / sizeof(double);
+        const double *src = reinterpret_cast<const double *>(mats);
+        for (int i = threadIdx.x; i < n_mats * per; i += blockDim.x) lds_dyn[i] = src[i];
+        __syncthreads();
+    }
     // wave-uniform point index: point and definition loads go through the scalar unit
     const int p = blockIdx.x * kPointsPerBlock + wave;
     if (p >= n) return;
     const sgpu_section_point pt = pts[p];
     if (!point_ok(defs, n_defs, pt)) {
-        if ((threadIdx.x & (kBatch - 1)) == 0) {
+        if (lane == 0) {
             sgpu_section_result r = {};
             r.valid = -1;
             res[p] = r;
@@ -306,10 +395,22 @@ __global__ __launch_bounds__(kPointsPerBlock * kBatch) void section_eval_kernel(
     }
     const SecDefDev &d = defs[pt.def_id];
     const double *pf = fsks + (int64_t)p * max_aux * 2;
-    if (d.dim == 2)
-        section_point<2>(d, tb, mats, t, pt.instance, pt.eps0, pt.kappa_z, 0.0, pf, red[wave], c, res + p);
-    else
-        section_point<3>(d, tb, mats, t, pt.instance, pt.eps0, pt.kappa_z, pt.kappa_y, pf, red[wave], c, res + p);
+    if (kLds) {
+        double *saux = lds_dyn + (size_t)n_mats * (sizeof(MatDev) / sizeof(double)) + (size_t)wave * max_aux * 2;
+        for (int j = lane; j < 2 * d.n_aux; j += kBatch) saux[j] = pf[j];
+        wave_sync();
+        const MatDev *ms = smat;
+        const double *as = saux;
+        if (d.dim == 2)
+            section_point<2>(d, tb, ms, t, pt.instance, pt.eps0, pt.kappa_z, 0.0, as, red[wave], c, res + p);
+        else
+            section_point<3>(d, tb, ms, t, pt.instance, pt.eps0, pt.kappa_z, pt.kappa_y, as, red[wave], c, res + p);
+    } else {
+        if (d.dim == 2)
+            section_point<2>(d, tb, mats, t, pt.instance, pt.eps0, pt.kappa_z, 0.0, pf, red[wave], c, res + p);
+        else
+            section_point<3>(d, tb, mats, t, pt.instance, pt.eps0, pt.kappa_z, pt.kappa_y, pf, red[wave], c, res + p);
+    }
 }
 
 // Plain copies between the committed and trial arrays (commit: src = trial).
@@ -370,13 +471,19 @@ void launch_aux_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, cons
 }
 
 void launch_section_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
-                         const MatDev *mats, const StateArrays &c, const StateArrays &t,
+                         const MatDev *mats, int n_mats, const StateArrays &c, const StateArrays &t,
                          const sgpu_section_point *pts, sgpu_section_result *res, int n, const double *fsks,
                          int max_aux) {
     if (n <= 0) return;
     const int blocks = (n + kPointsPerBlock - 1) / kPointsPerBlock;
-    hipLaunchKernelGGL(section_eval_kernel, dim3(blocks), dim3(kPointsPerBlock * kBatch), 0, stream, defs, n_defs,
-                       tb, mats, c, t, pts, res, n, fsks, max_aux);
+    if (n_mats <= kLdsMaxMats && max_aux <= kLdsMaxAux) {
+        const size_t lds = sizeof(MatDev) * (size_t)n_mats + sizeof(double) * 2 * (size_t)max_aux * kPointsPerBlock;
+        hipLaunchKernelGGL(section_eval_kernel<true>, dim3(blocks), dim3(kPointsPerBlock * kBatch), lds, stream,
+                           defs, n_defs, tb, mats, n_mats, c, t, pts, res, n, fsks, max_aux);
+    } else {
+        hipLaunchKernelGGL(section_eval_kernel<false>, dim3(blocks), dim3(kPointsPerBlock * kBatch), 0, stream,
+                           defs, n_defs, tb, mats, n_mats, c, t, pts, res, n, fsks, max_aux);
+    }
 }
 
 void launch_copy_state(hipStream_t stream, const StateArrays &dst, const StateArrays &src, int64_t n_slots,

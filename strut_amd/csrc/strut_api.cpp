@@ -21,7 +21,7 @@ void launch_aux_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, cons
                      const MatDev *mats, const StateArrays &c, const StateArrays &t,
                      const sgpu_section_point *pts, int n, double *fsks, int max_aux);
 void launch_section_eval(hipStream_t stream, const SecDefDev *defs, int n_defs, const SlotTables &tb,
-                         const MatDev *mats, const StateArrays &c, const StateArrays &t,
+                         const MatDev *mats, int n_mats, const StateArrays &c, const StateArrays &t,
                          const sgpu_section_point *pts, sgpu_section_result *res, int n, const double *fsks,
                          int max_aux);
 void launch_copy_state(hipStream_t stream, const StateArrays &dst, const StateArrays &src, int64_t n_slots,
@@ -407,7 +407,8 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
     HIP_OK(ctx, hipSetDevice(ctx->device));
     std::vector<SecDefDev> ddefs;
     std::vector<double> ty, tz, ta;
-    std::vector<int32_t> tm, tx, tc, tb, tcs, tas;
+    std::vector<int32_t> tm, tx, tc, tb, tcs, tas, tam;
+    std::vector<double> tay, taz, taa;
     ctx->max_aux = 0;
     int64_t state = 0, aux = 0, fibers = 0;
     for (SecDefHost &d : ctx->defs) {
@@ -417,6 +418,12 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
         tcs.insert(tcs.end(), d.chunk_slot.begin(), d.chunk_slot.end());
         d.aux_slot_base = (int64_t)tas.size();
         tas.insert(tas.end(), d.aux_slot.begin(), d.aux_slot.end());
+        for (int32_t sl : d.aux_slot) {
+            tam.push_back(d.mat[sl]);
+            tay.push_back(d.y[sl]);
+            taz.push_back(d.z[sl]);
+            taa.push_back(d.area[sl]);
+        }
         if (d.n_aux > ctx->max_aux) ctx->max_aux = d.n_aux;
         d.state_base = state;
         d.aux_base = aux;
@@ -462,6 +469,10 @@ int sgpu_sections_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &ctx->d_tab_mat, tm))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_aux, tx))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_auxslot, tas))) return rc;
+    if ((rc = upload(ctx, &ctx->d_tab_auxmat, tam))) return rc;
+    if ((rc = upload(ctx, &ctx->d_tab_auxy, tay))) return rc;
+    if ((rc = upload(ctx, &ctx->d_tab_auxz, taz))) return rc;
+    if ((rc = upload(ctx, &ctx->d_tab_auxarea, taa))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_chunk, tc))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_bcb, tb))) return rc;
     if ((rc = upload(ctx, &ctx->d_tab_cslot, tcs))) return rc;
@@ -539,8 +550,9 @@ int sgpu_section_eval_device(sgpu_ctx *ctx, int n, const sgpu_section_point *d_p
     }
     {
         ProfScope ps(ctx, SGPU_KERNEL_SECTION_EVAL);
-        launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c,
-                            ctx->t, d_points, d_results, n, ctx->d_fsks, ctx->max_aux);
+        launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats,
+                            (int)ctx->mats.size(), ctx->c, ctx->t, d_points, d_results, n, ctx->d_fsks,
+                            ctx->max_aux);
     }
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;

@@ -68,6 +68,10 @@ struct SlotTables {
     const int32_t *batch_chunk_begin;
     const int32_t *chunk_slot;  // first slot of each chunk
     const int32_t *aux_slot;    // slot of each aux index
+    // the slot tables again, indexed like aux_slot (aux_slot_base + j), so
+    // that the aux pre-pass reads them without waiting for aux_slot
+    const int32_t *aux_mat;
+    const double *aux_y, *aux_z, *aux_area;
 };
 
 // Fiber state, structure of arrays over all instances of all definitions.
