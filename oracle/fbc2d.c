@@ -485,3 +485,30 @@ int or_fbc2d_state(const or_elem2d *el, double *q_state, const double u_global[6
     f_out[4] = s * f3 + c * f4;
     return converged ? 0 : 1;
 }
+
+/* One forceBeamColumn2d element between (x1, y1) and (x2, y2) on the
+ * section instances first_inst .. first_inst + nip - 1, with its basic-force
+ * state q_state (3 + 2 nip + 3 values), for the checks in tests/. */
+int or_fbc2d_eval(or_sec2d *sec, int first_inst, int nip, const double *xi, const double *w, double x1, double y1,
+                  double x2, double y2, double *q_state, const double u_global[6], double k_out[36],
+                  double f_out[6]) {
+    or_stats stats = {0, 0, 0, 0, 0};
+    or_elem2d el;
+    el.sec = sec;
+    el.first_inst = first_inst;
+    el.nip = nip;
+    for (int ip = 0; ip < nip; ++ip) {
+        el.xi[ip] = xi[ip];
+        el.w[ip] = w[ip];
+    }
+    const double dx = x2 - x1, dy = y2 - y1;
+    el.L = hypot(dx, dy);
+    el.inv_L = 1.0 / el.L;
+    el.c = dx * el.inv_L;
+    el.s = dy * el.inv_L;
+    el.type = 0;
+    el.dc = dx / el.L;
+    el.ds = dy / el.L;
+    el.stats = &stats;
+    return or_fbc2d_state(&el, q_state, u_global, k_out, f_out);
+}

@@ -321,6 +321,121 @@ def dbc2d_eval(section: "Section2d", first_inst: int, xi, w, x1: float, y1: floa
     return k.reshape(6, 6), f
 
 
+def fbc2d_eval(section: "Section2d", first_inst: int, xi, w, x1: float, y1: float, x2: float, y2: float, q_state,
+               u_global):
+    """forceBeamColumn2d state determination of the oracle
+    (force_beam_column2d.mojo:1418-1919, Linear, no element loads) with its
+    basic-force state q_state [q0..q2 | eps0[nip] | kappa[nip] | v[3]]
+    (updated in place).  An element left unconverged after its
+    initial-tangent fallback returns normally, as in the reference."""
+    lib = load()
+    lib.or_fbc2d_eval.restype = c_int
+    lib.or_fbc2d_eval.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), POINTER(c_double), c_double, c_double,
+                                  c_double, c_double, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                  POINTER(c_double)]
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    u = np.ascontiguousarray(u_global, dtype=np.float64)
+    k = np.zeros(36)
+    f = np.zeros(6)
+    rc = lib.or_fbc2d_eval(section.h, first_inst, len(xi), _d_out(xi), _d_out(w), x1, y1, x2, y2, _d_out(q_state),
+                           _d_out(u), _d_out(k), _d_out(f))
+    if rc < 0:
+        raise RuntimeError(f"forceBeamColumn2d state failed ({rc})")
+    return k.reshape(6, 6), f
+
+
+class _Elem3d(ctypes.Structure):
+    _fields_ = [("sec", c_void_p), ("first_inst", c_int), ("nip", c_int), ("xi", c_double * MAX_NIP),
+                ("w", c_double * MAX_NIP), ("x1", c_double), ("y1", c_double), ("z1", c_double), ("x2", c_double),
+                ("y2", c_double), ("z2", c_double), ("has_vecxz", c_int), ("vecxz", c_double * 3), ("G", c_double),
+                ("J", c_double), ("transf", c_int), ("stats", c_void_p)]
+
+
+TRANSF = {"Linear": 0, "PDelta": 1}
+
+
+class Elem3d:
+    """A 3D fiber beam-column element of the oracle on an oracle Section3d:
+    forceBeamColumn3d (force_beam_column3d.mojo:2549-2990, kind "fbc") or
+    dispBeamColumn3d (beam_column3d_nonlinear.mojo:357-575, kind "dbc"),
+    Linear or PDelta transformation, no element loads.  The section
+    instances first_inst .. first_inst + nip - 1 hold its fibers; a
+    force-based element keeps its basic-force state
+    [q0..q4 | eps0[nip] | kappa_y[nip] | kappa_z[nip]] here (trial, as the
+    reference's force_basic_q)."""
+
+    def __init__(self, kind: str, section: "Section3d", first_inst: int, xi, w, xyz1, xyz2, vecxz=None,
+                 G: float = 0.0, J: float = 0.0, transf: str = "Linear"):
+        self.lib = load()
+        for fn in ("or_fbc3d_state", "or_dbc3d_state"):
+            getattr(self.lib, fn).restype = c_int
+        self.lib.or_fbc3d_state.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                                            POINTER(c_double)]
+        self.lib.or_dbc3d_state.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+        self.kind = kind
+        self.section = section
+        e = _Elem3d()
+        e.sec = section.h
+        e.first_inst = first_inst
+        e.nip = len(xi)
+        for k, (a, b) in enumerate(zip(xi, w)):
+            e.xi[k] = a
+            e.w[k] = b
+        e.x1, e.y1, e.z1 = (float(v) for v in xyz1)
+        e.x2, e.y2, e.z2 = (float(v) for v in xyz2)
+        e.has_vecxz = 0 if vecxz is None else 1
+        if vecxz is not None:
+            for k in range(3):
+                e.vecxz[k] = float(vecxz[k])
+        e.G, e.J = G, J
+        e.transf = TRANSF[transf]
+        self.e = e
+        self.q = np.zeros(5 + 3 * len(xi))
+
+    def state(self, u_global):
+        """Global 12x12 tangent and 12 resisting forces at element
+        displacements u_global; raises when the force-based element's
+        compatibility does not converge (the reference aborts)."""
+        u = np.ascontiguousarray(u_global, dtype=np.float64)
+        k = np.zeros(144)
+        f = np.zeros(12)
+        if self.kind == "fbc":
+            rc = self.lib.or_fbc3d_state(ctypes.byref(self.e), _d_out(self.q), _d_out(u), _d_out(k), _d_out(f))
+        else:
+            rc = self.lib.or_dbc3d_state(ctypes.byref(self.e), _d_out(u), _d_out(k), _d_out(f))
+        if rc != 0:
+            raise RuntimeError(f"3D element state failed ({rc})")
+        return k.reshape(12, 12), f
+
+    def commit(self) -> None:
+        for ip in range(self.e.nip):
+            self.section.commit(self.e.first_inst + ip)
+
+
+def beam3d_rotation(xyz1, xyz2, vecxz=None) -> np.ndarray:
+    lib = load()
+    lib.or_beam3d_rotation.restype = c_int
+    lib.or_beam3d_rotation.argtypes = [c_double] * 6 + [c_int, POINTER(c_double), POINTER(c_double)]
+    R = np.zeros(9)
+    v = np.zeros(3) if vecxz is None else np.ascontiguousarray(vecxz, dtype=np.float64)
+    rc = lib.or_beam3d_rotation(*(float(a) for a in xyz1), *(float(a) for a in xyz2), 0 if vecxz is None else 1,
+                                _d_out(v), _d_out(R))
+    if rc != 0:
+        raise ValueError("degenerate element axes")
+    return R.reshape(3, 3)
+
+
+def invert5(a) -> tuple:
+    lib = load()
+    lib.or_invert5.restype = c_int
+    lib.or_invert5.argtypes = [POINTER(c_double), POINTER(c_double)]
+    m = np.ascontiguousarray(a, dtype=np.float64).reshape(25)
+    inv = np.zeros(25)
+    ok = lib.or_invert5(_d_out(m.copy()), _d_out(inv))
+    return bool(ok), inv.reshape(5, 5)
+
+
 def band_solve(band: np.ndarray, hbw: int, rhs: np.ndarray):
     """BandGeneral factor + solve of the oracle (band: n x (2 hbw + 1)).
     Returns (ok, x, dense_fallback)."""

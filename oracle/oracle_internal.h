@@ -26,6 +26,29 @@ typedef struct {
     or_stats *stats;
 } or_elem2d;
 
+/* 3D fiber beam-column element (forceBeamColumn3d / dispBeamColumn3d) */
+enum { OR_TRANSF_LINEAR = 0, OR_TRANSF_PDELTA = 1 };
+
+typedef struct {
+    or_sec3d *sec;
+    int first_inst;       /* section instance of integration point 0 */
+    int nip;
+    double xi[OR_MAX_NIP], w[OR_MAX_NIP]; /* on [0,1], weights halved */
+    double x1, y1, z1, x2, y2, z2;
+    int has_vecxz;
+    double vecxz[3];
+    double G, J;          /* elastic torsion GJ (both > 0 to apply) */
+    int transf;           /* OR_TRANSF_* */
+    or_stats *stats;      /* may be NULL */
+} or_elem3d;
+
+int or_beam3d_rotation(double x1, double y1, double z1, double x2, double y2, double z2, int has_vecxz,
+                       const double *vecxz, double R[9]);
+int or_invert5(const double a[25], double inv[25]);
+int or_dbc3d_state(const or_elem3d *el, const double u_global[12], double k_out[144], double f_out[12]);
+int or_fbc3d_state(const or_elem3d *el, double *q_state, const double u_global[12], double k_out[144],
+                   double f_out[12]);
+
 int or_sec2d_all_elastic(const or_sec2d *s);
 int or_dbc2d_state(const or_elem2d *el, const double u_global[6], double k_out[36], double f_out[6]);
 int or_fbc2d_state(const or_elem2d *el, double *q_state, const double u_global[6], double k_out[36],
