@@ -32,7 +32,9 @@ typedef enum {
     SGPU_ERR_HIP = -2,       /* HIP runtime error                          */
     SGPU_ERR_NODEV = -3,     /* no usable gfx950 device                    */
     SGPU_ERR_STATE = -4,     /* call out of order (e.g. before finalize)   */
-    SGPU_SINGULAR_SECTION = 1 /* section |det k| <= 1e-40 (fiber2d.mojo:3074-3077) */
+    SGPU_SINGULAR_SECTION = 1, /* section |det k| <= 1e-40 (fiber2d.mojo:3074-3077) */
+    SGPU_ERR_LOCAL_NONCONV = 2 /* forceBeamColumn3d element compatibility did not
+                                  converge (force_beam_column3d.mojo:2846 aborts) */
 } sgpu_status;
 
 /* Material type tags: tag_types.mojo:136-142 (UniMaterialTypeTag). */
@@ -151,6 +153,68 @@ int sgpu_fiber_state_read(sgpu_ctx *ctx, int32_t def_id, int32_t instance, int w
 /* Total real fibers over all instances (the unit of the fiber-update
  * count, benchmarks_micro.mojo:250) and total slots. */
 int sgpu_fiber_totals(sgpu_ctx *ctx, int64_t *fibers, int64_t *slots);
+
+/* ------------------------------------------------------------------
+ * 3D fiber beam-column elements: forceBeamColumn3d and dispBeamColumn3d
+ * on 3D fiber sections, Linear or PDelta transformation, elastic torsion,
+ * no element loads.  The element state (the force-based basic forces and
+ * section deformations, the fibers' trial states) is device-resident.
+ * ------------------------------------------------------------------ */
+
+#define SGPU_MAX_NIP3D 5
+enum { SGPU_ELEM3D_FORCE = 0, SGPU_ELEM3D_DISP = 1 };
+enum { SGPU_TRANSF_LINEAR = 0, SGPU_TRANSF_PDELTA = 1 };
+
+/* One element as the loader hands it to the element routines: type, 3D
+ * fiber-section definition, integration rule (points on [0, 1], weights
+ * halved: beam_integration.mojo:224-241), geomTransf, end coordinates,
+ * optional vecxz (utils.mojo:96-139) and the elastic torsion G, J
+ * (applied when both are > 0). */
+typedef struct {
+    int32_t type;
+    int32_t def_id;
+    int32_t nip;
+    int32_t transf;
+    double xyz1[3], xyz2[3];
+    int32_t has_vecxz;
+    int32_t pad_;
+    double vecxz[3];
+    double G, J;
+    double xi[SGPU_MAX_NIP3D], w[SGPU_MAX_NIP3D];
+} sgpu_elem3d_desc;
+
+/* Register an element before sgpu_sections_finalize: allocates its nip
+ * section instances of def_id (loader.mojo:1821-1850; the force-based
+ * basic state [q0..q4 | eps0[nip] | kappa_y[nip] | kappa_z[nip]] starts at
+ * zero, :1826-1841) and caches its length and rotation
+ * (force_beam_column3d.mojo:895-953). */
+int sgpu_elem3d_add(sgpu_ctx *ctx, const sgpu_elem3d_desc *desc, int32_t *elem_id);
+
+/* State determination of n listed elements at their global end
+ * displacements u12[12 n]: global tangents k144[144 n] (row major) and
+ * resisting forces f12[12 n].  Replaces
+ * force_beam_column3d_fiber_global_tangent_and_internal
+ * (force_beam_column3d.mojo:2549-2990) and
+ * beam_column3d_fiber_global_tangent_and_internal
+ * (beam_column3d_nonlinear.mojo:357-575) in batch.  The force-based
+ * elements keep their trial basic state across calls (transient Newton
+ * iterations).  status[i] = SGPU_ERR_LOCAL_NONCONV where the reference
+ * aborts; the call then returns that code.  Host buffers; synchronous. */
+int sgpu_elem3d_state(sgpu_ctx *ctx, int n, const int32_t *elem_ids, const double *u12, double *k144, double *f12,
+                      int32_t *status);
+
+/* Same with device buffers; asynchronous on the context's stream, status
+ * per element left in d_status. */
+int sgpu_elem3d_state_device(sgpu_ctx *ctx, int n, const int32_t *d_elem_ids, const double *d_u12, double *d_k144,
+                             double *d_f12, int32_t *d_status);
+
+/* Commit the listed elements' sections (trial -> committed,
+ * fiber3d.mojo:873-926, as transient_nonlinear.mojo:3081-3101 does after a
+ * converged step). */
+int sgpu_elem3d_commit(sgpu_ctx *ctx, int n, const int32_t *elem_ids);
+
+/* The force-based element's basic state (5 + 3 nip values). */
+int sgpu_elem3d_basic_state(sgpu_ctx *ctx, int32_t elem_id, double *q);
 
 /* ------------------------------------------------------------------
  * Transient nonlinear analysis of 2D forceBeamColumn fiber frames.

@@ -24,6 +24,7 @@ STATUS_NAMES = {
     -3: "SGPU_ERR_NODEV",
     -4: "SGPU_ERR_STATE",
     1: "SGPU_SINGULAR_SECTION",
+    2: "SGPU_ERR_LOCAL_NONCONV",
 }
 
 # Every symbol include/strut_gpu.h declares, checked by the CPU test suite.
@@ -76,6 +77,11 @@ EXPORTED = (
     "sgpu_frame2d_set_predictor",
     "sgpu_frame2d_inst_stats",
     "sgpu_frame2d_launch_queue",
+    "sgpu_elem3d_add",
+    "sgpu_elem3d_state",
+    "sgpu_elem3d_state_device",
+    "sgpu_elem3d_commit",
+    "sgpu_elem3d_basic_state",
 )
 
 # Kernel ids of sgpu_profile_read.

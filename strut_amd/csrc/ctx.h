@@ -15,6 +15,8 @@
 namespace sgpu {
 struct FrameHost;
 void frame2d_host_free(FrameHost *fh);
+struct Elem3dHost;
+void elem3d_host_free(Elem3dHost *h);
 }
 
 namespace sgpu {
@@ -81,6 +83,8 @@ struct sgpu_ctx {
     int64_t prof_n[SGPU_KERNEL_COUNT] = {};
     // 2D frame transient instances (frame2d_api.cpp)
     sgpu::FrameHost *frames = nullptr;
+    // 3D fiber beam-column elements (elem3d_api.cpp)
+    sgpu::Elem3dHost *elems3d = nullptr;
     SlotTables tables() const {
         SlotTables tb;
         tb.y = d_tab_y;

@@ -320,6 +320,7 @@ int sgpu_ctx_destroy(sgpu_ctx *ctx) {
     if (ctx->d_list) (void)hipFree(ctx->d_list);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     sgpu::frame2d_host_free(ctx->frames);
+    sgpu::elem3d_host_free(ctx->elems3d);
     delete ctx;
     return SGPU_OK;
 }
