@@ -217,6 +217,33 @@ int sgpu_elem3d_commit(sgpu_ctx *ctx, int n, const int32_t *elem_ids);
 int sgpu_elem3d_basic_state(sgpu_ctx *ctx, int32_t elem_id, double *q);
 
 /* ------------------------------------------------------------------
+ * Multi-point constraints of the Transformation handler (rigidDiaphragm,
+ * equalDOF).  DOF i of n expands as u_i = sum_t coeff_pool[t] u[dof_pool[t]]
+ * over t in [row_offsets[i], row_offsets[i + 1]) -- the tables
+ * solver/run_case/loader.mojo:2230-2372 builds (identity rows for ordinary
+ * DOFs, the retained DOFs for a constrained one, none for an SP-fixed one).
+ * ------------------------------------------------------------------ */
+
+/* Upload the tables (replaces earlier ones); constrained / slave: 0 or 1
+ * per DOF (loader.mojo:2245-2345).  A row may not name a DOF twice. */
+int sgpu_mpc_set(sgpu_ctx *ctx, int n, const int32_t *row_offsets, const int32_t *dof_pool, const double *coeff_pool,
+                 const int32_t *constrained, const int32_t *slave);
+
+/* out = T^T K T on dense n x n row-major matrices: _collapse_matrix_by_mpc
+ * (solver/run_case/helpers.mojo:3345-3377), the same additions in the same
+ * order (entries of K equal to 0 contribute nothing).  Host buffers,
+ * synchronous; the _device form takes device buffers and is asynchronous. */
+int sgpu_mpc_collapse_matrix(sgpu_ctx *ctx, const double *K, double *out);
+int sgpu_mpc_collapse_matrix_device(sgpu_ctx *ctx, const double *d_K, double *d_out);
+
+/* out = T^T F: _collapse_vector_by_mpc (helpers.mojo:3288-3301). */
+int sgpu_mpc_collapse_vector(sgpu_ctx *ctx, const double *F, double *out);
+
+/* In place: slave DOFs from their retained DOFs, SP-fixed DOFs to 0 --
+ * _enforce_mpc_values (helpers.mojo:3380-3397). */
+int sgpu_mpc_enforce(sgpu_ctx *ctx, double *u);
+
+/* ------------------------------------------------------------------
  * Transient nonlinear analysis of 2D forceBeamColumn fiber frames.
  * One device workgroup runs one (frame, ground motion) instance's whole
  * Newmark / Newton loop; many instances run in one launch.
@@ -400,7 +427,8 @@ enum {
     SGPU_KERNEL_SECTION_EVAL = 1, /* section_eval_kernel: other slots + reduction  */
     SGPU_KERNEL_COMMIT = 2,       /* commit / revert copies                        */
     SGPU_KERNEL_FRAME2D = 3,      /* frame2d_kernel                                */
-    SGPU_KERNEL_COUNT = 4
+    SGPU_KERNEL_MPC = 4,          /* mpc_collapse_matrix_kernel                    */
+    SGPU_KERNEL_COUNT = 5
 };
 int sgpu_profile_enable(sgpu_ctx *ctx, int on);
 int sgpu_profile_read(sgpu_ctx *ctx, int kernel, double *total_ms, int64_t *launches);

@@ -436,6 +436,56 @@ def invert5(a) -> tuple:
     return bool(ok), inv.reshape(5, 5)
 
 
+def _mpc_args(tables):
+    ro, dp, cp = (np.ascontiguousarray(tables[0], dtype=np.int32), np.ascontiguousarray(tables[1], dtype=np.int32),
+                  np.ascontiguousarray(tables[2], dtype=np.float64))
+    return ro, dp, cp
+
+
+def mpc_collapse_matrix(K, tables) -> np.ndarray:
+    """_collapse_matrix_by_mpc (helpers.mojo:3345-3377) of the oracle;
+    tables = (row_offsets, dof_pool, coeff_pool)."""
+    lib = load()
+    ip = POINTER(c_int)
+    lib.or_mpc_collapse_matrix.restype = None
+    lib.or_mpc_collapse_matrix.argtypes = [c_int, POINTER(c_double), ip, ip, POINTER(c_double), POINTER(c_double)]
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    ro, dp, cp = _mpc_args(tables)
+    out = np.zeros_like(K)
+    lib.or_mpc_collapse_matrix(K.shape[0], _d_out(K), ro.ctypes.data_as(ip), dp.ctypes.data_as(ip), _d_out(cp),
+                               _d_out(out))
+    return out
+
+
+def mpc_collapse_vector(F, tables) -> np.ndarray:
+    """_collapse_vector_by_mpc (helpers.mojo:3288-3301) of the oracle."""
+    lib = load()
+    ip = POINTER(c_int)
+    lib.or_mpc_collapse_vector.restype = None
+    lib.or_mpc_collapse_vector.argtypes = [c_int, POINTER(c_double), ip, ip, POINTER(c_double), POINTER(c_double)]
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    ro, dp, cp = _mpc_args(tables)
+    out = np.zeros_like(F)
+    lib.or_mpc_collapse_vector(len(F), _d_out(F), ro.ctypes.data_as(ip), dp.ctypes.data_as(ip), _d_out(cp),
+                               _d_out(out))
+    return out
+
+
+def mpc_enforce(u, tables, constrained, slave) -> np.ndarray:
+    """_enforce_mpc_values (helpers.mojo:3380-3397) of the oracle."""
+    lib = load()
+    ip = POINTER(c_int)
+    lib.or_mpc_enforce.restype = None
+    lib.or_mpc_enforce.argtypes = [c_int, POINTER(c_double), ip, ip, ip, ip, POINTER(c_double)]
+    u = np.array(u, dtype=np.float64)
+    ro, dp, cp = _mpc_args(tables)
+    con = np.ascontiguousarray(constrained, dtype=np.int32)
+    sl = np.ascontiguousarray(slave, dtype=np.int32)
+    lib.or_mpc_enforce(len(u), _d_out(u), con.ctypes.data_as(ip), sl.ctypes.data_as(ip), ro.ctypes.data_as(ip),
+                       dp.ctypes.data_as(ip), _d_out(cp))
+    return u
+
+
 def band_solve(band: np.ndarray, hbw: int, rhs: np.ndarray):
     """BandGeneral factor + solve of the oracle (band: n x (2 hbw + 1)).
     Returns (ok, x, dense_fallback)."""

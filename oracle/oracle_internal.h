@@ -49,6 +49,14 @@ int or_dbc3d_state(const or_elem3d *el, const double u_global[12], double k_out[
 int or_fbc3d_state(const or_elem3d *el, double *q_state, const double u_global[12], double k_out[144],
                    double f_out[12]);
 
+/* Transformation-handler collapses (mpc.c) */
+void or_mpc_collapse_matrix(int n, const double *K, const int *row_offsets, const int *dof_pool,
+                            const double *coeff_pool, double *out);
+void or_mpc_collapse_vector(int n, const double *F, const int *row_offsets, const int *dof_pool,
+                            const double *coeff_pool, double *out);
+void or_mpc_enforce(int n, double *u, const int *constrained, const int *slave, const int *row_offsets,
+                    const int *dof_pool, const double *coeff_pool);
+
 int or_sec2d_all_elastic(const or_sec2d *s);
 int or_dbc2d_state(const or_elem2d *el, const double u_global[6], double k_out[36], double f_out[6]);
 int or_fbc2d_state(const or_elem2d *el, double *q_state, const double u_global[6], double k_out[36],

@@ -82,6 +82,11 @@ EXPORTED = (
     "sgpu_elem3d_state_device",
     "sgpu_elem3d_commit",
     "sgpu_elem3d_basic_state",
+    "sgpu_mpc_set",
+    "sgpu_mpc_collapse_matrix",
+    "sgpu_mpc_collapse_matrix_device",
+    "sgpu_mpc_collapse_vector",
+    "sgpu_mpc_enforce",
 )
 
 # Kernel ids of sgpu_profile_read.
@@ -89,6 +94,7 @@ KERNEL_AUX_EVAL = 0
 KERNEL_SECTION_EVAL = 1
 KERNEL_COMMIT = 2
 KERNEL_FRAME2D = 3
+KERNEL_MPC = 4
 
 # Schedules of sgpu_frame2d_run.
 SCHEDULE_AUTO = 0

@@ -19,6 +19,8 @@ from __future__ import annotations
 import math
 from typing import Dict, List, Tuple
 
+import numpy as np
+
 MPA = 1000.0  # kPa per MPa
 
 CORE_CONCRETE = {"fpc": -34.0 * MPA, "epsc0": -0.0025, "fpcu": -24.0 * MPA, "epscu": -0.02}
@@ -352,3 +354,72 @@ def peak_story_drift(u_hist, stories: int, bays: int, story_h: float = 3.0):
     ux = u[:, 0::3].reshape(u.shape[0], stories + 1, cols)
     drift = np.abs(np.diff(ux, axis=1)) / story_h
     return drift.max(axis=(0, 2))
+
+
+# ------------------------------------- 3D frame with rigid floor diaphragms
+def frame3d_diaphragm_model(stories: int = 2, bays_x: int = 2, bays_y: int = 1, h: float = 3.0, lx: float = 6.0,
+                            ly: float = 5.0) -> dict:
+    """Node grid of a 3D frame (6 DOFs per node) with a retained node at each
+    floor's centre joined to the floor's nodes by a rigidDiaphragm
+    (perpDirn 3), the base fixed and each retained node's uz, rx, ry fixed, as
+    OpenSees-style models set them up.  Also lists the column and beam
+    elements (node index pairs) for assembling a stiffness."""
+    from .constraints import mpc_tables, rigid_diaphragm
+
+    coords, grid = {}, {}
+    nid = 1
+    for k in range(stories + 1):
+        for j in range(bays_y + 1):
+            for i in range(bays_x + 1):
+                coords[nid] = (i * lx, j * ly, k * h)
+                grid[(i, j, k)] = nid
+                nid += 1
+    masters = {}
+    for k in range(1, stories + 1):
+        coords[nid] = (bays_x * lx / 2.0, bays_y * ly / 2.0, k * h)
+        masters[k] = nid
+        nid += 1
+    ids = sorted(coords)
+    node_index = {n: idx for idx, n in enumerate(ids)}
+    ndf = 6
+    sp = np.zeros(len(ids) * ndf, dtype=bool)
+    for j in range(bays_y + 1):
+        for i in range(bays_x + 1):
+            sp[node_index[grid[(i, j, 0)]] * ndf:node_index[grid[(i, j, 0)]] * ndf + 6] = True
+    mps = []
+    for k, m in masters.items():
+        for d in (3, 4, 5):
+            sp[node_index[m] * ndf + d - 1] = True
+        floor = [grid[(i, j, k)] for j in range(bays_y + 1) for i in range(bays_x + 1)]
+        mps += rigid_diaphragm(coords, m, floor, 3)
+    elems = []
+    for k in range(stories):
+        for j in range(bays_y + 1):
+            for i in range(bays_x + 1):
+                elems.append((node_index[grid[(i, j, k)]], node_index[grid[(i, j, k + 1)]]))
+    for k in range(1, stories + 1):
+        for j in range(bays_y + 1):
+            for i in range(bays_x):
+                elems.append((node_index[grid[(i, j, k)]], node_index[grid[(i + 1, j, k)]]))
+        for j in range(bays_y):
+            for i in range(bays_x + 1):
+                elems.append((node_index[grid[(i, j, k)]], node_index[grid[(i, j + 1, k)]]))
+    tables = mpc_tables(len(ids), ndf, sp, mps, node_index)
+    return {"coords": coords, "node_index": node_index, "ndf": ndf, "sp": sp, "mp_constraints": mps,
+            "elements": elems, "masters": masters, "tables": tables}
+
+
+def synthetic_frame_stiffness(model: dict, seed: int = 0) -> np.ndarray:
+    """A dense symmetric stiffness with the frame's element sparsity: each
+    element adds a seeded symmetric positive definite 12 x 12 block on its
+    two nodes' DOFs (a stand-in for the assembled K in the collapse tests and
+    timing)."""
+    n = len(model["node_index"]) * model["ndf"]
+    rng = np.random.default_rng(seed)
+    K = np.zeros((n, n))
+    for a, b in model["elements"]:
+        m = rng.normal(size=(12, 12))
+        ke = m @ m.T + 12.0 * np.eye(12)
+        dofs = list(range(6 * a, 6 * a + 6)) + list(range(6 * b, 6 * b + 6))
+        K[np.ix_(dofs, dofs)] += ke
+    return K

@@ -17,6 +17,8 @@ struct FrameHost;
 void frame2d_host_free(FrameHost *fh);
 struct Elem3dHost;
 void elem3d_host_free(Elem3dHost *h);
+struct MpcHost;
+void mpc_host_free(MpcHost *h);
 }
 
 namespace sgpu {
@@ -85,6 +87,8 @@ struct sgpu_ctx {
     sgpu::FrameHost *frames = nullptr;
     // 3D fiber beam-column elements (elem3d_api.cpp)
     sgpu::Elem3dHost *elems3d = nullptr;
+    // multi-point constraint tables (mpc.hip)
+    sgpu::MpcHost *mpc = nullptr;
     SlotTables tables() const {
         SlotTables tb;
         tb.y = d_tab_y;

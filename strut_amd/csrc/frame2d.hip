@@ -83,9 +83,9 @@ struct ElemCtx {
 __device__ bool eval_sections(const ElemCtx &E, const double *pe, const double *pk) {
     const SecDefDev &d = E.A->defs[E.el->def];
     const int nip = E.el->nip;
-    const int w = threadIdx.x >> 6;
+    const int n_waves = blockDim.x >> 6;
     __syncthreads();  // every wavefront has read the previous results
-    if (w < nip) {
+    for (int w = threadIdx.x >> 6; w < nip; w += n_waves) {
         const double e0 = pe[w], k0 = pk[w];
         const Sec2dOut r = wave_section_2d<false>(d, E.A->tb, E.mats, E.A->c, E.A->t, E.inst0 + w, e0, k0, E.red);
         if ((threadIdx.x & 63) == 0) {
@@ -513,11 +513,11 @@ static_assert(sizeof(FbcScratch) >= 42 * sizeof(double), "dispBeamColumn2d sums 
 
 __device__ void dbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, const double *uf,
                               double *out42, ElemShared *sh, const MatDev *mats, double *red, double *tmp) {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, nip = el.nip;
+    const int lane = threadIdx.x & 63, nip = el.nip, n_waves = blockDim.x >> 6;
     const double L = el.L, c = el.c, s = el.s;
     const double inv_L = 1.0 / L;
     const double inv_L_sq = inv_L * inv_L;
-    if (w < nip) {
+    for (int w = threadIdx.x >> 6; w < nip; w += n_waves) {
         double ug[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) ug[k] = el.fmap[k] >= 0 ? uf[el.fmap[k]] : 0.0;
@@ -639,11 +639,11 @@ __device__ __forceinline__ void element_state(const Frame2dArgs &A, int inst, in
     const InstDev &I = A.insts[inst];
     const FrameDev &F = A.frames[I.frame];
     const ElemDev &el = A.elems[F.elem_base + e];
-    const int w = threadIdx.x >> 6;
+    const int w = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
     if (commit_first) {
-        if (w < el.nip) {
-            const double *last = A.last + I.last_base + 2 * kMaxNipDev * e + 2 * w;
-            commit_section_2d(A.defs[el.def], A.tb, A.mats, A.c, A.elem_inst[I.elem_inst_base + e] + w, last[0],
+        for (int ip = w; ip < el.nip; ip += n_waves) {
+            const double *last = A.last + I.last_base + 2 * kMaxNipDev * e + 2 * ip;
+            commit_section_2d(A.defs[el.def], A.tb, A.mats, A.c, A.elem_inst[I.elem_inst_base + e] + ip, last[0],
                               last[1]);
         }
         __syncthreads();
