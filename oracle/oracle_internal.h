@@ -57,6 +57,25 @@ void or_mpc_collapse_vector(int n, const double *F, const int *row_offsets, cons
 void or_mpc_enforce(int n, double *u, const int *constrained, const int *slave, const int *row_offsets,
                     const int *dof_pool, const double *coeff_pool);
 
+/* BandGeneral storage and no-pivot LU with the dense fallback
+ * (transient2d.c: lsb.mojo:175-240, :377-381, :631-679, :735-769) and the
+ * chunked dot product (simd_contiguous.mojo:197-225), shared by the
+ * transient drivers */
+typedef struct {
+    int n, hbw, w;
+    double *band;       /* n x (2 hbw + 1) */
+    double *dense;      /* n x n dense fallback storage */
+    int *piv;
+    double *work;
+    int dense_fallback;
+    long dense_fallbacks;
+} band_t;
+int band_slot(int hbw, int row, int col);
+int band_factor(band_t *b);
+void band_solve(band_t *b, const double *rhs, double *x);
+double or_tree8(const double *v);
+double or_dot_w(const double *a, const double *b, int n);
+
 int or_sec2d_all_elastic(const or_sec2d *s);
 int or_dbc2d_state(const or_elem2d *el, const double u_global[6], double k_out[36], double f_out[6]);
 int or_fbc2d_state(const or_elem2d *el, double *q_state, const double u_global[6], double k_out[36],

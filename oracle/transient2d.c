@@ -22,20 +22,20 @@
 #include "oracle_internal.h"
 #include "transient2d.h"
 
-static double tree8(const double *v) {
+double or_tree8(const double *v) {
     double w0 = v[0] + v[4], w1 = v[1] + v[5], w2 = v[2] + v[6], w3 = v[3] + v[7];
     double x0 = w0 + w2, x1 = w1 + w3;
     return x0 + x1;
 }
 
 /* simd_contiguous.mojo:197-225 with factor 1.0 */
-static double dot_w(const double *a, const double *b, int n) {
+double or_dot_w(const double *a, const double *b, int n) {
     double sum = 0.0;
     int i = 0;
     for (; i + OR_SIMD_W <= n; i += OR_SIMD_W) {
         double v[OR_SIMD_W];
         for (int l = 0; l < OR_SIMD_W; ++l) v[l] = (a[i + l] * b[i + l]) * 1.0;
-        sum += tree8(v);
+        sum += or_tree8(v);
     }
     for (; i < n; ++i) sum += (a[i] * b[i]) * 1.0;
     return sum;
@@ -57,17 +57,7 @@ double or_path_eval(const or_tr2d_case *c, double t) {
     return c->ts_factor * (v1 + (v2 - v1) * (incr - (double)incr1));
 }
 
-typedef struct {
-    int n, hbw, w;
-    double *band;       /* n x (2 hbw + 1) */
-    double *dense;      /* n x n dense fallback storage */
-    int *piv;
-    double *work;
-    int dense_fallback;
-    long dense_fallbacks;
-} band_t;
-
-static int band_slot(int hbw, int row, int col) { /* lsb.mojo:377-381 */
+int band_slot(int hbw, int row, int col) { /* lsb.mojo:377-381 */
     int off = col - row + hbw;
     if (off < 0 || off > 2 * hbw) return -1;
     return row * (2 * hbw + 1) + off;
@@ -113,7 +103,7 @@ static int dense_factor(band_t *b) {
 }
 
 /* lsb.mojo:631-679 */
-static int band_factor(band_t *b) {
+int band_factor(band_t *b) {
     int n = b->n, hbw = b->hbw;
     size_t bsz = (size_t)n * (2 * hbw + 1);
     double *orig = malloc(sizeof(double) * bsz);
@@ -172,7 +162,7 @@ static int band_factor(band_t *b) {
 }
 
 /* lsb.mojo:735-769 and :216-232 */
-static void band_solve(band_t *b, const double *rhs, double *x) {
+void band_solve(band_t *b, const double *rhs, double *x) {
     int n = b->n, hbw = b->hbw;
     if (b->dense_fallback) {
         double *A = b->dense;
@@ -422,11 +412,11 @@ int or_tr2d_run(const or_tr2d_case *c, or_tr2d_out *out) {
             }
             for (int i = 0; i < n; ++i) { /* :2337-2356 */
                 double damping_force = 0.0;
-                if (has_damping) damping_force = dot_w(Cff + (size_t)i * n, vt, n);
-                double inertia_force = dot_w(Mff + (size_t)i * n, at, n);
+                if (has_damping) damping_force = or_dot_w(Cff + (size_t)i * n, vt, n);
+                double inertia_force = or_dot_w(Mff + (size_t)i * n, at, n);
                 R[i] = P[i] - F[free_list[i]] - 0.0 - damping_force - inertia_force;
             }
-            double norm = sqrt(dot_w(R, R, n)); /* :2357-2371 */
+            double norm = sqrt(or_dot_w(R, R, n)); /* :2357-2371 */
             if (norm <= c->tol) {
                 converged = 1;
                 break;

@@ -423,3 +423,114 @@ def synthetic_frame_stiffness(model: dict, seed: int = 0) -> np.ndarray:
         dofs = list(range(6 * a, 6 * a + 6)) + list(range(6 * b, 6 * b + 6))
         K[np.ix_(dofs, dofs)] += ke
     return K
+
+
+# ------------------------------------- 3D RC frame transient (rigid floors)
+def rc_frame3d_case(stories: int, bays_x: int, bays_y: int, steps: int, dt: float = 0.005,
+                    seeds: Tuple[int, int] = (0, 1), pgas_g: Tuple[float, float] = (0.5, 0.35),
+                    story_h: float = 3.0, bay_w: float = 6.0, floor_mass: float = 50.0, nip: int = 5,
+                    n_core: int = 12, n_bars: int = 12, strong: float = 6.0, tol: float = 1.0e-8,
+                    max_iters: int = 25, diaphragms: bool = True, bidirectional: bool = True,
+                    column_type: str = "forceBeamColumn3d", beam_type: str = "dispBeamColumn3d",
+                    column_transf: str = "PDelta", name: str = "") -> dict:
+    """3D RC moment frame in Strut's JSON schema: forceBeamColumn3d columns
+    (400 x 400 mm, PDelta) and dispBeamColumn3d beams (300 x 500 mm, depth
+    vertical, Linear), nip Lobatto points, n_core x n_core confined core,
+    two-fiber cover and n_bars Steel02 bars; fixed bases; lumped masses on
+    ux, uy, uz of every floor node; per floor a retained node at the plan
+    centre (uz, rx, ry fixed) tied to the floor's nodes by rigidDiaphragm
+    (perpDirn 3) under the Transformation handler; two UniformExcitation
+    records (x, y; seeded band-limited noise at pgas_g) as the ``patterns``
+    list; Newmark 1/2-1/4, Newton / NormUnbalance / BandGeneral.
+
+    Rayleigh 5 % on modes 1 and 3 from the estimates T1 = 0.1 s x stories
+    and T3 = T1 / 5 (alphaM + betaKInit)."""
+    from .constraints import rigid_diaphragm
+
+    nodes, node_at, nid = [], {}, 1
+    for k in range(stories + 1):
+        for j in range(bays_y + 1):
+            for i in range(bays_x + 1):
+                node = {"id": nid, "x": i * bay_w, "y": j * bay_w, "z": k * story_h}
+                if k == 0:
+                    node["constraints"] = [1, 2, 3, 4, 5, 6]
+                nodes.append(node)
+                node_at[(i, j, k)] = nid
+                nid += 1
+    masters = {}
+    if diaphragms:
+        for k in range(1, stories + 1):
+            nodes.append({"id": nid, "x": bays_x * bay_w / 2.0, "y": bays_y * bay_w / 2.0, "z": k * story_h,
+                          "constraints": [3, 4, 5]})
+            masters[k] = nid
+            nid += 1
+    elements, eid = [], 1
+    for k in range(stories):
+        for j in range(bays_y + 1):
+            for i in range(bays_x + 1):
+                elements.append({"id": eid, "type": column_type, "nodes": [node_at[(i, j, k)], node_at[(i, j, k + 1)]],
+                                 "section": 1, "geomTransf": column_transf, "vecxz": [1.0, 0.0, 0.0],
+                                 "integration": "Lobatto", "num_int_pts": nip})
+                eid += 1
+    for k in range(1, stories + 1):
+        for j in range(bays_y + 1):
+            for i in range(bays_x):
+                elements.append({"id": eid, "type": beam_type, "nodes": [node_at[(i, j, k)], node_at[(i + 1, j, k)]],
+                                 "section": 2, "geomTransf": "Linear", "vecxz": [0.0, 1.0, 0.0],
+                                 "integration": "Lobatto", "num_int_pts": nip})
+                eid += 1
+        for j in range(bays_y):
+            for i in range(bays_x + 1):
+                elements.append({"id": eid, "type": beam_type, "nodes": [node_at[(i, j, k)], node_at[(i, j + 1, k)]],
+                                 "section": 2, "geomTransf": "Linear", "vecxz": [1.0, 0.0, 0.0],
+                                 "integration": "Lobatto", "num_int_pts": nip})
+                eid += 1
+    masses = [{"node": node_at[(i, j, k)], "dof": d, "value": floor_mass}
+              for k in range(1, stories + 1) for j in range(bays_y + 1) for i in range(bays_x + 1) for d in (1, 2, 3)]
+    coords = {n["id"]: (n["x"], n["y"], n["z"]) for n in nodes}
+    mps = []
+    for k, m in masters.items():
+        floor = [node_at[(i, j, k)] for j in range(bays_y + 1) for i in range(bays_x + 1)]
+        mps += rigid_diaphragm(coords, m, floor, 3)
+    sections = [rc_section(1, dim=3, b=0.4, h=0.4, n_core=n_core, n_bars=n_bars),
+                rc_section(2, dim=3, b=0.3, h=0.5, n_core=n_core, n_bars=n_bars)]
+    t1 = 0.1 * stories
+    w1, w3 = 2.0 * math.pi / t1, 2.0 * math.pi / (t1 / 5.0)
+    zeta = 0.05
+    series, patterns = [], []
+    for tag, (seed, pga, direction) in enumerate(zip(seeds, pgas_g, (1, 2)), start=1):
+        if not bidirectional and tag > 1:
+            break
+        acc = ground_motion(seed, dt, steps, pga, strong=strong)
+        series.append({"type": "Path", "tag": tag, "dt": dt, "values": [float(v) for v in acc], "factor": 1.0})
+        patterns.append({"type": "UniformExcitation", "tag": tag, "direction": direction, "accel": tag})
+    case = {
+        "schema_version": "1.0",
+        "metadata": {"name": name or f"rc_frame3d_{stories}x{bays_x}x{bays_y}", "units": "kN-m-t-s"},
+        "model": {"ndm": 3, "ndf": 6},
+        "nodes": nodes,
+        "materials": rc_materials(),
+        "sections": sections,
+        "elements": elements,
+        "masses": masses,
+        "time_series": series,
+        "patterns": patterns,
+        "rayleigh": {"alphaM": 2.0 * zeta * w1 * w3 / (w1 + w3), "betaKInit": 2.0 * zeta / (w1 + w3)},
+        "analysis": {"type": "transient_nonlinear", "steps": steps, "dt": dt, "tol": tol, "max_iters": max_iters,
+                     "system": "BandGeneral", "numberer": "Plain",
+                     "constraints": "Transformation" if diaphragms else "Plain", "algorithm": "Newton",
+                     "test_type": "NormUnbalance", "integrator": {"type": "Newmark", "gamma": 0.5, "beta": 0.25}},
+        "recorders": [{"type": "node_displacement",
+                       "nodes": [masters[stories] if diaphragms else node_at[(0, 0, stories)]],
+                       "dofs": [1, 2, 6], "output": "roof"}],
+    }
+    if mps:
+        case["mp_constraints"] = mps
+    return case
+
+
+def config1_case(steps: int = 4000) -> dict:
+    """20 stories x 5 x 5 bays, forceBeamColumn3d columns and
+    dispBeamColumn3d beams (5 IPs, 268-fiber sections: 12 x 12 core, 12
+    bars), rigid diaphragms, two records at 0.5 g and 0.35 g, dt 0.005 s."""
+    return rc_frame3d_case(20, 5, 5, steps=steps, dt=0.005, seeds=(0, 1), pgas_g=(0.5, 0.35))

@@ -1,0 +1,182 @@
+"""Pins the oracle's 3D transient driver (oracle/transient3d.c:
+transient_nonlinear.mojo:830-3638 with the Transformation handler and
+several UniformExcitation patterns) before it is trusted:
+  * a planar frame built in 3D (out-of-plane DOFs fixed, no constraints)
+    follows the 2D driver (itself pinned against the reference's golden
+    vectors) on the same fibers: the same Newton iteration counts and the
+    same displacement history;
+  * with elastic sections and Linear transformations the system is linear,
+    so the response to both records at once is the sum of the responses to
+    each alone (superposition fixes the pattern loads and their collapse);
+  * every recorded step keeps each floor rigid in plan
+    (u_x = u_xm - dy rz, u_y = u_ym + dx rz, the closed form);
+  * a plan-symmetric frame excited along x alone does not move along y or
+    twist;
+  * the band the reference takes from the element free map
+    (linear_solver_backend.mojo:1481-1506) is the brute-force maximum, and
+    truncating the collapsed tangent to it changes how Newton gets there,
+    not where it converges.
+"""
+
+import copy
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from strut_amd import cases
+from strut_amd.frame2d import load_frame2d_case
+from strut_amd.frame3d import NDF, load_frame3d_case
+
+
+def _planar_3d_from_2d(case2d: dict) -> dict:
+    """The 2D frame in the x-y plane of a 3D model: uz, rx, ry fixed, the
+    same fibers (3D sections of the same patches), vecxz = +z so local z is
+    global z and in-plane bending is about local z."""
+    c = copy.deepcopy(case2d)
+    c["model"] = {"ndm": 3, "ndf": 6}
+    for n in c["nodes"]:
+        n["z"] = 0.0
+        cons = n.get("constraints", [])
+        n["constraints"] = [1, 2, 3, 4, 5, 6] if cons else [3, 4, 5]
+    sections = []
+    for s in c["sections"]:
+        s3 = copy.deepcopy(s)
+        s3["type"] = "FiberSection3d"
+        sections.append(s3)
+    c["sections"] = sections
+    for e in c["elements"]:
+        e["type"] = e["type"].replace("2d", "3d")
+        e["vecxz"] = [0.0, 0.0, 1.0]
+    return c
+
+
+def test_planar_3d_frame_follows_the_2d_driver():
+    case2d = cases.rc_frame2d_case(2, 1, steps=120, dt=0.01, seed=3, pga_g=0.35, strong=1.0)
+    r2 = oracle.run_transient2d(load_frame2d_case(case2d))
+    fc3 = load_frame3d_case(_planar_3d_from_2d(case2d))
+    r3 = oracle.run_transient3d(fc3)
+    assert r2["status"] == 0 and r3["status"] == 0
+    np.testing.assert_array_equal(r3["iters"], r2["iters"])
+    n = fc3.n_nodes
+    u3 = r3["u"].reshape(-1, n, NDF)[:, :, [0, 1, 5]]
+    u2 = r2["u"].reshape(-1, n, 3)
+    scale = np.abs(u2).max()
+    assert scale > 1e-3  # the frame yields at this intensity
+    assert np.abs(u3 - u2).max() <= 1e-6 * scale
+    assert np.abs(r3["u"].reshape(-1, n, NDF)[:, :, [2, 3, 4]]).max() == 0.0
+
+
+def _elastic(case: dict) -> dict:
+    c = copy.deepcopy(case)
+    c["materials"] = [{"id": k, "type": "Elastic", "params": {"E": 30.0e6 if k < 3 else 200.0e6}} for k in (1, 2, 3)]
+    return c
+
+
+def _small(**kw):
+    args = dict(steps=80, dt=0.005, n_core=4, n_bars=4, column_transf="Linear")
+    args.update(kw)
+    return cases.rc_frame3d_case(2, 2, 1, **args)
+
+
+def test_superposition_of_the_two_records_on_an_elastic_frame():
+    both = _elastic(_small())
+    rb = oracle.run_transient3d(load_frame3d_case(both))
+    parts = []
+    for keep in (0, 1):
+        c = copy.deepcopy(both)
+        c["patterns"] = [c["patterns"][keep]]
+        parts.append(oracle.run_transient3d(load_frame3d_case(c)))
+    assert all(r["status"] == 0 for r in [rb] + parts)
+    scale = np.abs(rb["u"]).max()
+    assert np.abs(parts[0]["u"]).max() > 0.1 * scale and np.abs(parts[1]["u"]).max() > 0.1 * scale
+    assert np.abs(rb["u"] - parts[0]["u"] - parts[1]["u"]).max() <= 1e-7 * scale
+
+
+def test_floors_stay_rigid_and_a_symmetric_plan_does_not_twist():
+    c = _small(bidirectional=False, column_transf="PDelta")
+    fc = load_frame3d_case(c)
+    r = oracle.run_transient3d(fc)
+    assert r["status"] == 0
+    u = r["u"].reshape(-1, fc.n_nodes, NDF)
+    masters = [i for i, n in enumerate(c["nodes"]) if n.get("constraints") == [3, 4, 5]]
+    for m in masters:
+        xm, ym, zm = fc.coords[m]
+        floor = [i for i in range(fc.n_nodes) if abs(fc.coords[i, 2] - zm) < 1e-12 and i != m]
+        for i in floor:
+            dx, dy = fc.coords[i, 0] - xm, fc.coords[i, 1] - ym
+            assert np.abs(u[:, i, 0] - (u[:, m, 0] - dy * u[:, m, 5])).max() <= 1e-15
+            assert np.abs(u[:, i, 1] - (u[:, m, 1] + dx * u[:, m, 5])).max() <= 1e-15
+            assert np.array_equal(u[:, i, 5], u[:, m, 5])
+        ux = np.abs(u[:, m, 0]).max()
+        assert ux > 1e-3
+        assert np.abs(u[:, m, 1]).max() <= 1e-9 * ux
+        assert np.abs(u[:, m, 5]).max() <= 1e-9 * ux
+
+
+def test_band_rule_and_truncation_change_the_path_not_the_solution():
+    fc = load_frame3d_case(_small(steps=60))
+    # brute-force half bandwidth over the element free map
+    free_index = -np.ones(fc.total_dofs, dtype=int)
+    free = np.flatnonzero(~fc.constrained)
+    free_index[free] = np.arange(len(free))
+    hbw = 0
+    for i, j in fc.elem_nodes:
+        fd = [free_index[6 * i + k] for k in range(6)] + [free_index[6 * j + k] for k in range(6)]
+        fd = [f for f in fd if f >= 0]
+        if fd:
+            hbw = max(hbw, max(fd) - min(fd))
+    r0 = oracle.run_transient3d(fc, hbw_rule=0)
+    r1 = oracle.run_transient3d(fc, hbw_rule=1)
+    assert r0["hbw"] == hbw and r1["hbw"] > hbw
+    assert r0["status"] == 0 and r1["status"] == 0
+    assert r0["iters"].sum() > r1["iters"].sum()
+    scale = np.abs(r1["u"]).max()
+    assert np.abs(r0["u"] - r1["u"]).max() <= 1e-6 * scale
+
+
+def test_elastic_diaphragm_frame_is_the_collapsed_linear_newmark_solution():
+    """Elastic sections, Linear transformations: the oracle's history equals
+    numpy's linear Newmark on T^T K T, T^T M T, Rayleigh and T^T P, with T
+    the expansion matrix of the diaphragm tables (K from the oracle's element
+    stiffness at rest, which test_oracle_elem3d pins against the textbook
+    frame stiffness)."""
+    from strut_amd.constraints import expansion_matrix
+
+    fc = load_frame3d_case(_elastic(_small(steps=100)))
+    r = oracle.run_transient3d(fc, hbw_rule=1)
+    assert r["status"] == 0
+    _, elems = oracle.frame3d_elements(fc)
+    nt = fc.total_dofs
+    K = np.zeros((nt, nt))
+    for el, (i, j) in zip(elems, fc.elem_nodes):
+        k, _ = el.state(np.zeros(12))
+        dofs = list(range(6 * i, 6 * i + 6)) + list(range(6 * j, 6 * j + 6))
+        K[np.ix_(dofs, dofs)] += k
+    free = np.flatnonzero(~fc.constrained)
+    T = expansion_matrix(fc.row_offsets, fc.dof_pool, fc.coeff_pool)[:, free]
+    Kr, Mr = T.T @ K @ T, T.T @ np.diag(fc.mass) @ T
+    Cr = fc.alpha_m * Mr + fc.beta_k_init * Kr
+    dt, g, b = fc.dt, fc.gamma, fc.beta
+    a0, a1, a2, a3 = 1 / (b * dt * dt), g / (b * dt), 1 / (b * dt), 1 / (2 * b) - 1
+    Keff = Kr + a1 * Cr + a0 * Mr
+    n = len(free)
+    u, v, a = np.zeros(n), np.zeros(n), np.zeros(n)
+    want = np.zeros((fc.steps, nt))
+    for step in range(fc.steps):
+        t = (step + 1) * dt
+        load = np.zeros(nt)
+        for pt in fc.patterns:
+            i1 = int(np.floor(t / pt.dt))  # Path: 0 from the last sample on (time_series.mojo:71-96)
+            ag = 0.0 if i1 + 1 >= len(pt.values) else np.interp(t, np.arange(len(pt.values)) * pt.dt, pt.values)
+            ag *= pt.factor
+            load[pt.direction - 1::6] += -fc.mass[pt.direction - 1::6] * ag
+        rhs = T.T @ load + Mr @ (a0 * u + a2 * v + a3 * a) + Cr @ (a1 * u + (g / b - 1) * v + dt * (g / (2 * b) - 1) * a)
+        un = np.linalg.solve(Keff, rhs)
+        an = a0 * (un - u) - a2 * v - a3 * a
+        v = v + dt * ((1 - g) * a + g * an)
+        u, a = un, an
+        want[step] = T @ u
+    scale = np.abs(want).max()
+    assert scale > 1e-3
+    assert np.abs(r["u"] - want).max() <= 1e-7 * scale
