@@ -244,6 +244,71 @@ int sgpu_mpc_collapse_vector(sgpu_ctx *ctx, const double *F, double *out);
 int sgpu_mpc_enforce(sgpu_ctx *ctx, double *u);
 
 /* ------------------------------------------------------------------
+ * Transient nonlinear analysis of one 3D fiber beam-column frame
+ * (transient_nonlinear.mojo:830-3638 with ndf 6): the sgpu_elem3d elements
+ * of the context, the Transformation handler's expansion tables
+ * (loader.mojo:2230-2372; identity rows when has_mpc is 0), lumped masses,
+ * Rayleigh alphaM / betaK / betaKInit, Newmark, Newton with the
+ * NormUnbalance test and a BandGeneral band; one or more UniformExcitation
+ * patterns whose collapsed inertia loads are summed (one is Strut's schema,
+ * two is the bidirectional extension).
+ * ------------------------------------------------------------------ */
+#define SGPU_F3_MAX_PATTERNS 3
+
+/* run status beyond 0 (all steps done) and k > 0 (step k did not converge
+ * within max_iters, the reference's abort) */
+#define SGPU_F3_ELEMENT_FAILED (-1)   /* forceBeamColumn3d compatibility failed (the reference aborts) */
+#define SGPU_F3_SINGULAR (-11)        /* a band pivot at or below 1e-14 */
+
+typedef struct {
+    int n_nodes;                     /* 6 DOFs per node, plain numbering */
+    const int32_t *sp_constrained;   /* 6 n_nodes single-point flags */
+    const double *mass;              /* 6 n_nodes lumped masses */
+    int has_mpc;                     /* Transformation handler with constraints */
+    const int32_t *row_offsets;      /* 6 n_nodes + 1 */
+    const int32_t *dof_pool;
+    const double *coeff_pool;
+    const int32_t *slave;            /* 6 n_nodes */
+    int n_elem;
+    const int32_t *elem_ids;         /* sgpu_elem3d ids in element order */
+    const int32_t *elem_nodes;       /* 2 node indices per element */
+    double dt, gamma, beta, tol;
+    int max_iters;
+    double alpha_m, beta_k, beta_k_init;
+    int n_patterns;
+    int exc_dir[SGPU_F3_MAX_PATTERNS];             /* 1-based direction */
+    int ts_n[SGPU_F3_MAX_PATTERNS];                /* Path values (start 0) */
+    const double *ts_values[SGPU_F3_MAX_PATTERNS];
+    double ts_dt[SGPU_F3_MAX_PATTERNS], ts_factor[SGPU_F3_MAX_PATTERNS];
+    int hbw_rule;   /* 0: half bandwidth from the element free map, as the
+                       reference (linear_solver_backend.mojo:1481-1506),
+                       which drops the collapse's couplings outside it;
+                       1: every coupling of the collapsed system */
+} sgpu_frame3d_desc;
+
+/* Tables of the frame (after sgpu_sections_finalize; one frame per
+ * context).  Replaces the set-up half of run_transient_nonlinear
+ * (:830-1700) for this path. */
+int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *desc, int32_t *hbw, int32_t *free_count);
+
+/* Run `steps` more steps (the step loop :1698-3110): u_hist[step * 6
+ * n_nodes] the full displacement after each converged step (slaves
+ * enforced) or NULL, iters[step] the Newton iterations, *status as above.
+ * The band LU never pivots; *dense_fallbacks (may be NULL) counts the
+ * factorizations where the reference would have moved to its dense
+ * partial-pivot LU (linear_solver_backend.mojo:608-628) -- the solution is
+ * the same within rounding. */
+int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, int32_t *steps_done, int32_t *status,
+                     int32_t *dense_fallbacks);
+
+/* The effective tangent K + a1 C + a0 M in the band (n_free x (2 hbw + 1),
+ * row major, slot (i, j) at i (2 hbw + 1) + j - i + hbw), as the reference
+ * loads it before each factorization (transient_nonlinear.mojo:2400-2411),
+ * with the elements evaluated at the current displacement (a trial state;
+ * before the first step: at rest). */
+int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band);
+
+/* ------------------------------------------------------------------
  * Transient nonlinear analysis of 2D forceBeamColumn fiber frames.
  * One device workgroup runs one (frame, ground motion) instance's whole
  * Newmark / Newton loop; many instances run in one launch.
@@ -428,7 +493,9 @@ enum {
     SGPU_KERNEL_COMMIT = 2,       /* commit / revert copies                        */
     SGPU_KERNEL_FRAME2D = 3,      /* frame2d_kernel                                */
     SGPU_KERNEL_MPC = 4,          /* mpc_collapse_matrix_kernel                    */
-    SGPU_KERNEL_COUNT = 5
+    SGPU_KERNEL_F3_ELEM = 5,      /* 3D frame driver: element state launches       */
+    SGPU_KERNEL_F3_SOLVE = 6,     /* 3D frame driver: band LU and substitutions    */
+    SGPU_KERNEL_COUNT = 7
 };
 int sgpu_profile_enable(sgpu_ctx *ctx, int on);
 int sgpu_profile_read(sgpu_ctx *ctx, int kernel, double *total_ms, int64_t *launches);

@@ -532,7 +532,7 @@ class _Tr3dCase(ctypes.Structure):
 
 class _Tr3dOut(ctypes.Structure):
     _fields_ = [("u_hist", POINTER(c_double)), ("iters", POINTER(c_int)), ("steps_done", c_int), ("hbw", c_int),
-                ("free_count", c_int), ("dense_fallbacks", ctypes.c_long)]
+                ("free_count", c_int), ("dense_fallbacks", ctypes.c_long), ("band_first", POINTER(c_double))]
 
 
 def frame3d_elements(fc):
@@ -555,7 +555,8 @@ def frame3d_elements(fc):
     return secs, elems
 
 
-def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: int | None = None) -> dict:
+def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: int | None = None,
+                    capture_band: bool = False) -> dict:
     """Run a loaded 3D fiber-frame case (strut_amd.frame3d.Frame3dCase) on the
     oracle.  hbw_rule 0 takes the band from the element free map as the
     reference does (the collapse's couplings outside it are dropped); 1
@@ -604,6 +605,13 @@ def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: i
     iters = np.zeros(steps, dtype=np.int32)
     o.u_hist = _d_out(u_hist) if record_history else None
     o.iters = iters.ctypes.data_as(POINTER(c_int))
+    band = None
+    if capture_band:  # sized generously: the band is n x (2 hbw + 1) <= total^2
+        band = np.zeros(fc.total_dofs * fc.total_dofs)
+        o.band_first = _d_out(band)
     status = lib.or_tr3d_run(ctypes.byref(c), ctypes.byref(o))
+    if band is not None:
+        band = band[:o.free_count * (2 * o.hbw + 1)].reshape(o.free_count, 2 * o.hbw + 1)
     return {"status": status, "u": u_hist, "iters": iters, "steps_done": o.steps_done, "hbw": o.hbw,
-            "free_count": o.free_count, "dense_fallbacks": o.dense_fallbacks, "_elems": elems, "_sections": secs}
+            "free_count": o.free_count, "dense_fallbacks": o.dense_fallbacks, "band_first": band,
+            "_elems": elems, "_sections": secs}

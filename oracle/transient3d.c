@@ -255,6 +255,8 @@ int or_tr3d_run(const or_tr3d_case *c, or_tr3d_out *out) {
                     for (int col = c0; col <= c1; ++col)
                         bd.band[band_slot(hbw, row, col)] += a0 * Mff[(size_t)row * n + col];
                 }
+            if (out->band_first && step == 0 && it == 0)
+                memcpy(out->band_first, bd.band, sizeof(double) * (size_t)n * (2 * hbw + 1));
             if (!band_factor(&bd)) { status = -2; goto done; }
             band_solve(&bd, R, du);
             for (int i = 0; i < n; ++i) uf[i] += du[i];

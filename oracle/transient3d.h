@@ -42,6 +42,7 @@ typedef struct {
     int steps_done;
     int hbw, free_count;
     long dense_fallbacks;
+    double *band_first; /* n x (2 hbw + 1): K_eff of the first Newton solve, or NULL */
 } or_tr3d_out;
 
 int or_tr3d_run(const or_tr3d_case *c, or_tr3d_out *out);

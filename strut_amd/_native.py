@@ -87,6 +87,9 @@ EXPORTED = (
     "sgpu_mpc_collapse_matrix_device",
     "sgpu_mpc_collapse_vector",
     "sgpu_mpc_enforce",
+    "sgpu_frame3d_create",
+    "sgpu_frame3d_run",
+    "sgpu_frame3d_effective_band",
 )
 
 # Kernel ids of sgpu_profile_read.
@@ -95,6 +98,8 @@ KERNEL_SECTION_EVAL = 1
 KERNEL_COMMIT = 2
 KERNEL_FRAME2D = 3
 KERNEL_MPC = 4
+KERNEL_F3_ELEM = 5
+KERNEL_F3_SOLVE = 6
 
 # Schedules of sgpu_frame2d_run.
 SCHEDULE_AUTO = 0

@@ -501,7 +501,7 @@ def rc_frame3d_case(stories: int, bays_x: int, bays_y: int, steps: int, dt: floa
     for tag, (seed, pga, direction) in enumerate(zip(seeds, pgas_g, (1, 2)), start=1):
         if not bidirectional and tag > 1:
             break
-        acc = ground_motion(seed, dt, steps, pga, strong=strong)
+        acc = ground_motion(seed, dt, max(steps, 100), pga, strong=strong)  # the band-pass needs 100 samples
         series.append({"type": "Path", "tag": tag, "dt": dt, "values": [float(v) for v in acc], "factor": 1.0})
         patterns.append({"type": "UniformExcitation", "tag": tag, "direction": direction, "accel": tag})
     case = {

@@ -19,6 +19,8 @@ struct Elem3dHost;
 void elem3d_host_free(Elem3dHost *h);
 struct MpcHost;
 void mpc_host_free(MpcHost *h);
+struct Frame3dHost;
+void frame3d_host_free(Frame3dHost *h);
 }
 
 namespace sgpu {
@@ -89,6 +91,8 @@ struct sgpu_ctx {
     sgpu::Elem3dHost *elems3d = nullptr;
     // multi-point constraint tables (mpc.hip)
     sgpu::MpcHost *mpc = nullptr;
+    // 3D frame transient (frame3d.hip)
+    sgpu::Frame3dHost *frame3d = nullptr;
     SlotTables tables() const {
         SlotTables tb;
         tb.y = d_tab_y;

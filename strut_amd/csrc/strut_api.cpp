@@ -322,6 +322,7 @@ int sgpu_ctx_destroy(sgpu_ctx *ctx) {
     sgpu::frame2d_host_free(ctx->frames);
     sgpu::elem3d_host_free(ctx->elems3d);
     sgpu::mpc_host_free(ctx->mpc);
+    sgpu::frame3d_host_free(ctx->frame3d);
     delete ctx;
     return SGPU_OK;
 }

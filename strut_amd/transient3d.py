@@ -1,0 +1,124 @@
+"""Transient nonlinear analysis of one 3D fiber beam-column frame on the
+GPU (sgpu_frame3d_*): a loaded Frame3dCase (strut_amd.frame3d) becomes the
+context's fiber sections, sgpu_elem3d elements, expansion tables, masses and
+excitation, and runs step by step on the device -- the driver
+transient_nonlinear.mojo:830-3638 with the Transformation handler and the
+bidirectional UniformExcitation extension.
+
+There is no host fallback: without libstrut_gpu.so or a device the
+constructor raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int, c_int32, c_void_p
+
+import numpy as np
+
+from ._native import dptr, iptr
+from .elements3d import Elements3d
+from .frame3d import Frame3dCase
+from .sections import SectionRuntime
+
+MAX_PATTERNS = 3
+STATUS_ELEMENT_FAILED = -1
+STATUS_SINGULAR = -11
+
+
+class Frame3dDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", c_int), ("sp_constrained", POINTER(c_int32)), ("mass", POINTER(c_double)), ("has_mpc", c_int),
+        ("row_offsets", POINTER(c_int32)), ("dof_pool", POINTER(c_int32)), ("coeff_pool", POINTER(c_double)),
+        ("slave", POINTER(c_int32)), ("n_elem", c_int), ("elem_ids", POINTER(c_int32)),
+        ("elem_nodes", POINTER(c_int32)), ("dt", c_double), ("gamma", c_double), ("beta", c_double),
+        ("tol", c_double), ("max_iters", c_int), ("alpha_m", c_double), ("beta_k", c_double),
+        ("beta_k_init", c_double), ("n_patterns", c_int), ("exc_dir", c_int * MAX_PATTERNS),
+        ("ts_n", c_int * MAX_PATTERNS), ("ts_values", POINTER(c_double) * MAX_PATTERNS),
+        ("ts_dt", c_double * MAX_PATTERNS), ("ts_factor", c_double * MAX_PATTERNS), ("hbw_rule", c_int),
+    ]
+
+
+def _declare(lib):
+    if getattr(lib, "_f3_declared", False):
+        return
+    lib.sgpu_frame3d_create.restype = c_int
+    lib.sgpu_frame3d_create.argtypes = [c_void_p, POINTER(Frame3dDesc), POINTER(c_int32), POINTER(c_int32)]
+    lib.sgpu_frame3d_run.restype = c_int
+    lib.sgpu_frame3d_run.argtypes = [c_void_p, c_int, POINTER(c_double), POINTER(c_int32), POINTER(c_int32),
+                                     POINTER(c_int32), POINTER(c_int32)]
+    lib.sgpu_frame3d_effective_band.restype = c_int
+    lib.sgpu_frame3d_effective_band.argtypes = [c_void_p, POINTER(c_double)]
+    lib._f3_declared = True
+
+
+class Frame3dRun:
+    """One 3D frame on the device.  hbw_rule 0 takes the band from the element
+    free map as the reference does; 1 keeps every coupling of the collapsed
+    system."""
+
+    def __init__(self, fc: Frame3dCase, device: int = 0, hbw_rule: int = 0):
+        self.fc = fc
+        self.rt = SectionRuntime(fc.materials, device=device)
+        _declare(self.rt.lib)
+        defs = {sid: self.rt.add_definition(3, fc.section_cells[sid]) for sid in fc.section_ids}
+        self.elems = Elements3d(self.rt)
+        kinds = ("fbc", "dbc")
+        ids = []
+        for e in range(fc.n_elem):
+            sid = int(fc.elem_section[e])
+            nip = int(fc.elem_nip[e])
+            vz = None if np.isnan(fc.elem_vecxz[e]).any() else fc.elem_vecxz[e]
+            G, J = fc.section_gj[sid]
+            i, j = fc.elem_nodes[e]
+            ids.append(self.elems.add(kinds[int(fc.elem_kind[e])], defs[sid], fc.elem_xi[e, :nip], fc.elem_w[e, :nip],
+                                      fc.coords[i], fc.coords[j], vecxz=vz, G=G, J=J,
+                                      transf=("Linear", "PDelta")[int(fc.elem_transf[e])]))
+        self.rt.finalize()
+        k = dict(sp=np.ascontiguousarray(fc.sp, dtype=np.int32), mass=np.ascontiguousarray(fc.mass),
+                 ro=np.ascontiguousarray(fc.row_offsets, dtype=np.int32),
+                 dp=np.ascontiguousarray(fc.dof_pool if len(fc.dof_pool) else np.zeros(1), dtype=np.int32),
+                 cp=np.ascontiguousarray(fc.coeff_pool if len(fc.coeff_pool) else np.zeros(1), dtype=np.float64),
+                 sl=np.ascontiguousarray(fc.slave, dtype=np.int32), ids=np.array(ids, dtype=np.int32),
+                 en=np.ascontiguousarray(fc.elem_nodes.reshape(-1), dtype=np.int32),
+                 ts=[np.ascontiguousarray(p.values, dtype=np.float64) for p in fc.patterns])
+        self._keep = k
+        d = Frame3dDesc()
+        d.n_nodes = fc.n_nodes
+        d.sp_constrained, d.mass, d.has_mpc = iptr(k["sp"]), dptr(k["mass"]), 1 if fc.has_mpc else 0
+        d.row_offsets, d.dof_pool, d.coeff_pool, d.slave = iptr(k["ro"]), iptr(k["dp"]), dptr(k["cp"]), iptr(k["sl"])
+        d.n_elem, d.elem_ids, d.elem_nodes = fc.n_elem, iptr(k["ids"]), iptr(k["en"])
+        d.dt, d.gamma, d.beta, d.tol, d.max_iters = fc.dt, fc.gamma, fc.beta, fc.tol, fc.max_iters
+        d.alpha_m, d.beta_k, d.beta_k_init = fc.alpha_m, fc.beta_k, fc.beta_k_init
+        if len(fc.patterns) > MAX_PATTERNS:
+            raise ValueError(f"at most {MAX_PATTERNS} excitation patterns")
+        d.n_patterns = len(fc.patterns)
+        for q, p in enumerate(fc.patterns):
+            d.exc_dir[q], d.ts_n[q], d.ts_values[q] = p.direction, len(k["ts"][q]), dptr(k["ts"][q])
+            d.ts_dt[q], d.ts_factor[q] = p.dt, p.factor
+        d.hbw_rule = hbw_rule
+        hbw, nf = c_int32(), c_int32()
+        self.rt._check(self.rt.lib.sgpu_frame3d_create(self.rt.ctx, ctypes.byref(d), ctypes.byref(hbw),
+                                                       ctypes.byref(nf)))
+        self.hbw, self.free_count = hbw.value, nf.value
+        self.steps_done = 0
+
+    def run(self, steps: int, record_history: bool = True) -> dict:
+        u = np.zeros((steps, self.fc.total_dofs)) if record_history else None
+        iters = np.zeros(steps, dtype=np.int32)
+        done, status, fallbacks = c_int32(), c_int32(), c_int32()
+        self.rt._check(self.rt.lib.sgpu_frame3d_run(self.rt.ctx, steps, dptr(u) if record_history else None,
+                                                    iptr(iters), ctypes.byref(done), ctypes.byref(status),
+                                                    ctypes.byref(fallbacks)))
+        self.steps_done += done.value
+        return {"status": status.value, "u": u, "iters": iters, "steps_done": done.value, "hbw": self.hbw,
+                "free_count": self.free_count, "dense_fallbacks": fallbacks.value}
+
+    def effective_band(self) -> np.ndarray:
+        """K + a1 C + a0 M in the band at the current displacement."""
+        band = np.zeros((self.free_count, 2 * self.hbw + 1))
+        self.rt._check(self.rt.lib.sgpu_frame3d_effective_band(self.rt.ctx, dptr(band)))
+        return band
+
+    def close(self) -> None:
+        self.rt.close()
