@@ -267,6 +267,51 @@ def gm_cpu_baseline(steps: int) -> dict:
             "sample": f"one record (seed 0), {r['steps_done']} steps, {dt:.2f} s, oracle C restatement, 1 thread"}
 
 
+def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0) -> dict:
+    """Steps/s of the 3D RC frame transient (20 stories x 5 x 5 bays, rigid
+    diaphragms, two simultaneous records) on the device: warmup steps, then
+    `steps` timed steps of the same run."""
+    from strut_amd import _native, cases
+    from strut_amd.frame3d import load_frame3d_case
+    from strut_amd.transient3d import Frame3dRun
+
+    fc = load_frame3d_case(cases.config1_case(steps=warmup + steps))
+    t0 = time.perf_counter()
+    run = Frame3dRun(fc, hbw_rule=hbw_rule)
+    setup_s = time.perf_counter() - t0
+    w = run.run(warmup, record_history=False)
+    run.rt.profile_enable(True)
+    t0 = time.perf_counter()
+    r = run.run(steps, record_history=False)
+    dt = time.perf_counter() - t0
+    elem_ms, elem_n = run.rt.profile_read(_native.KERNEL_F3_ELEM)
+    solve_ms, solve_n = run.rt.profile_read(_native.KERNEL_F3_SOLVE)
+    run.rt.profile_enable(False)
+    run.close()
+    iters = int(r["iters"][:r["steps_done"]].sum())
+    return {"frame": "3D RC frame 20 stories x 5 x 5 bays", "nodes": fc.n_nodes, "elements": fc.n_elem,
+            "free_dofs": run.free_count, "hbw": run.hbw, "band_rule": ("element free map", "collapsed")[hbw_rule],
+            "setup_seconds": setup_s, "warmup_steps": w["steps_done"], "steps": r["steps_done"],
+            "status": r["status"], "seconds": dt, "steps_per_s": r["steps_done"] / dt, "newton_iters": iters,
+            "ms_per_newton_iter": 1e3 * dt / max(iters, 1), "dense_fallbacks_of_the_reference": r["dense_fallbacks"],
+            "element_state_ms_avg": elem_ms / max(elem_n, 1), "band_solve_ms_avg": solve_ms / max(solve_n, 1)}
+
+
+def frame3d_cpu_baseline(steps: int) -> dict:
+    from oracle import oracle
+    from strut_amd import cases
+    from strut_amd.frame3d import load_frame3d_case
+
+    fc = load_frame3d_case(cases.config1_case(steps=steps))
+    t0 = time.perf_counter()
+    r = oracle.run_transient3d(fc, record_history=False)
+    dt = time.perf_counter() - t0
+    return {"value": r["steps_done"] / dt, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"the first {r['steps_done']} steps ({int(r['iters'][:r['steps_done']].sum())} Newton "
+                      f"iterations, {r['dense_fallbacks']} dense-LU fallbacks), {dt:.1f} s, oracle C restatement "
+                      "(dense K, collapse, band with the reference's fallback), 1 thread"}
+
+
 def _motion_case(frame: dict, seed: int, max_steps: int):
     from strut_amd import cases
     from strut_amd.frame2d import load_frame2d_case
@@ -416,6 +461,9 @@ def main() -> int:
     ap.add_argument("--learned-test", type=int, default=64)
     ap.add_argument("--learned-steps", type=int, default=800)
     ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--f3-steps", type=int, default=10, help="timed steps of the 3D frame transient leg (0 = skip)")
+    ap.add_argument("--f3-warmup", type=int, default=2)
+    ap.add_argument("--f3-cpu-steps", type=int, default=2, help="steps of the 3D frame leg's CPU baseline sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -445,6 +493,15 @@ def main() -> int:
         config["gm_steps"] = {"workload": f"2D RC frame 3 stories x 1 bay, forceBeamColumn 5 IPs, {args.gm_steps} "
                                           "steps dt 0.01 s, PGA 0.4 g seeded ground motions (seeds 0..N-1)",
                               **gm}
+    if args.f3_steps > 0:
+        f3 = frame3d_leg(args.f3_steps, args.f3_warmup)
+        _progress(f"3D frame leg: {f3['steps_per_s']:.3g} steps/s, {f3['ms_per_newton_iter']:.3g} ms per Newton iteration")
+        if not args.no_cpu_baseline:
+            f3["cpu_baseline"] = frame3d_cpu_baseline(args.f3_cpu_steps)
+            _progress(f"3D frame leg CPU baseline: {f3['cpu_baseline']['value']:.3g} steps/s")
+        config["frame3d"] = {"workload": "3D RC frame 20 stories x 5 x 5 bays, forceBeamColumn3d columns (PDelta) and "
+                                         "dispBeamColumn3d beams, 5 IPs, 268-fiber sections, rigid diaphragms "
+                                         "(Transformation), two seeded records 0.5 g / 0.35 g, dt 0.005 s", **f3}
     if args.ens_motions > 0:
         config["ensemble"] = ensemble_leg(args.ens_motions, args.ens_steps)
         _progress(f"ensemble leg: batched/one-by-one {config['ensemble']['batched_vs_one_by_one']:.3g}")
