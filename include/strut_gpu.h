@@ -301,6 +301,17 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *desc, int32_t *h
 int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, int32_t *steps_done, int32_t *status,
                      int32_t *dense_fallbacks);
 
+/* BandGeneral factor and solve of one system on the device
+ * (linear_solver_backend.mojo:631-679, :735-769): band n x (2 hbw + 1),
+ * row major as the reference stores it (slot (i, j) at i (2 hbw + 1) + j - i
+ * + hbw), no pivoting.  *info: 0 solved; 1 solved, and the reference would
+ * have moved this factorization to its dense partial-pivot LU (:608-628);
+ * 2 a pivot at or below 1e-14, nothing solved.  variant 0 picks the kernel
+ * (the register-window one up to half bandwidth 127), 1 the global-memory
+ * kernel, 2 the register-window kernel.  Host buffers; allocates. */
+int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
+                    int32_t *info);
+
 /* The effective tangent K + a1 C + a0 M in the band (n_free x (2 hbw + 1),
  * row major, slot (i, j) at i (2 hbw + 1) + j - i + hbw), as the reference
  * loads it before each factorization (transient_nonlinear.mojo:2400-2411),

@@ -90,6 +90,7 @@ EXPORTED = (
     "sgpu_frame3d_create",
     "sgpu_frame3d_run",
     "sgpu_frame3d_effective_band",
+    "sgpu_band_solve",
 )
 
 # Kernel ids of sgpu_profile_read.

@@ -318,6 +318,142 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_kernel(int n, int hb
     if (tid == 0) status[0] = s_flag ? 1.0 : 0.0;
 }
 
+// The same factorization and substitutions with the active window of the
+// band in registers, for half bandwidths up to kF3RegMaxP - 1.  At column k
+// the window holds rows and columns k .. k + P - 1 (P = hbw + 1), every row
+// and column in a circular slot (index mod P); thread t owns the window
+// entries e * 1024 + t (slot pair (row, column) fixed for the whole run).
+// Per column: (A) the pivot row's entries go to LDS (and to the band, final)
+// and the pivot column's entries to LDS; barrier; (B) the pivot column's
+// owners form the multipliers (to LDS and the band) and the forward
+// substitution; barrier; (C) every owner applies the rank-1 update in
+// registers, and the retired row and column slots reload the next row and
+// column of the band (never touched before: earlier columns only reach rows
+// and columns up to k + hbw).  Pivot row and multipliers alternate between
+// two LDS buffers, so (C) of one column and (A) of the next need no barrier
+// between them.  The back substitution runs in wavefront 0 alone: lane l
+// holds U(i, i + 1 + l) and U(i, i + 65 + l), the solved x values ride in a
+// register window shifted one lane per row, the next row's U entries load
+// while the current row reduces.
+constexpr int kF3RegMaxP = 128;
+constexpr int kF3RegEntries = kF3RegMaxP * kF3RegMaxP / kF3SolveThreads;  // 16
+
+__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_reg_kernel(int n, int hbw, double *band,
+                                                                       const double *rhs, double *x, double *status) {
+    extern __shared__ double s_y[];  // n
+    __shared__ double s_col[kF3RegMaxP], s_l[2][kF3RegMaxP], s_u[2][kF3RegMaxP];
+    __shared__ double s_diag;
+    __shared__ int s_flag;
+    const int P = hbw + 1, W = 2 * hbw + 1, tid = threadIdx.x;
+    const int cells = P * P;
+    double a[kF3RegEntries];
+    int slot[kF3RegEntries];  // row slot << 8 | column slot, -1 past the window
+#pragma unroll
+    for (int e = 0; e < kF3RegEntries; ++e) {
+        const int L = e * kF3SolveThreads + tid;
+        slot[e] = L < cells ? ((L / P) << 8) | (L % P) : -1;
+        a[e] = 0.0;
+        if (L < cells) {
+            const int i = L / P, j = L % P;  // at k = 0 the slots are the indices
+            if (i < n && j < n) a[e] = band[(size_t)i * W + (j - i + hbw)];
+        }
+    }
+    for (int i = tid; i < n; i += kF3SolveThreads) s_y[i] = rhs[i];
+    if (tid == 0) s_flag = 0;
+    __syncthreads();
+    int kc = 0;  // k mod P
+    for (int k = 0; k < n; ++k) {
+        const int buf = k & 1;
+        // (A) pivot row -> s_u and the band (final U); pivot column -> s_col
+#pragma unroll
+        for (int e = 0; e < kF3RegEntries; ++e) {
+            const int sl = slot[e];
+            if (sl < 0) continue;
+            const int rs = sl >> 8, cs = sl & 255;
+            if (rs == kc) {
+                const int j = k + (cs - kc + P) % P;
+                s_u[buf][cs] = a[e];
+                if (cs == kc) s_diag = a[e];
+                if (j < n) band[(size_t)k * W + (j - k + hbw)] = a[e];
+            } else if (cs == kc) {
+                s_col[rs] = a[e];
+            }
+        }
+        __syncthreads();
+        const double diag = s_diag;
+        if (fabs(diag) <= 1.0e-14) {
+            if (tid == 0) status[0] = 2.0;
+            return;
+        }
+        // (B) multipliers, the reference's dense-fallback test, forward substitution
+        const double yk = s_y[k];
+        for (int rs = tid; rs < P; rs += kF3SolveThreads) {
+            if (rs == kc) continue;
+            const int i = k + (rs - kc + P) % P;
+            const double v = s_col[rs];
+            if (fabs(v) > fabs(diag)) s_flag = 1;
+            const double l = v / diag;
+            s_l[buf][rs] = l;
+            if (i < n) {
+                band[(size_t)i * W + (k - i + hbw)] = l;
+                s_y[i] -= l * yk;
+            }
+        }
+        __syncthreads();
+        // (C) rank-1 update; the retired slots take row and column k + P
+        const int k1 = k + P, kc1 = kc + 1 == P ? 0 : kc + 1;
+#pragma unroll
+        for (int e = 0; e < kF3RegEntries; ++e) {
+            const int sl = slot[e];
+            if (sl < 0) continue;
+            const int rs = sl >> 8, cs = sl & 255;
+            if (rs != kc && cs != kc) {
+                a[e] -= s_l[buf][rs] * s_u[buf][cs];
+            } else if (rs == kc) {  // row k + P, columns of the next window
+                const int j = k + 1 + (cs - kc1 + P) % P;
+                a[e] = (k1 < n && j < n) ? band[(size_t)k1 * W + (j - k1 + hbw)] : 0.0;
+            } else {  // column k + P, rows of the next window
+                const int i = k + 1 + (rs - kc1 + P) % P;
+                a[e] = (k1 < n && i < n) ? band[(size_t)i * W + (k1 - i + hbw)] : 0.0;
+            }
+        }
+        kc = kc1;
+    }
+    __syncthreads();
+    // back substitution in wavefront 0
+    if (tid < 64) {
+        const int lane = tid;
+        double xa = 0.0, xb = 0.0;  // x[i + 1 + lane], x[i + 65 + lane]
+        auto load_row = [&](int i, double &u0, double &u1) {
+            u0 = (i >= 0 && lane < hbw && i + 1 + lane < n) ? band[(size_t)i * W + hbw + 1 + lane] : 0.0;
+            u1 = (i >= 0 && lane + 64 < hbw && i + 65 + lane < n) ? band[(size_t)i * W + hbw + 65 + lane] : 0.0;
+        };
+        double u0, u1, d0;
+        load_row(n - 1, u0, u1);
+        d0 = band[(size_t)(n - 1) * W + hbw];
+        for (int i = n - 1; i >= 0; --i) {
+            double n0, n1, nd = 1.0;
+            load_row(i - 1, n0, n1);
+            if (i > 0) nd = band[(size_t)(i - 1) * W + hbw];
+            double s = u0 * xa + u1 * xb;
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            const double xi = (s_y[i] - s) / d0;
+            if (lane == 0) x[i] = xi;
+            const double carry = __shfl(xa, 63, 64);
+            xa = __shfl_up(xa, 1, 64);
+            xb = __shfl_up(xb, 1, 64);
+            if (lane == 0) {
+                xa = xi;
+                xb = carry;
+            }
+            u0 = n0;
+            u1 = n1;
+            d0 = nd;
+        }
+        if (lane == 0) status[0] = s_flag ? 1.0 : 0.0;
+    }
+}
+
 __global__ void f3_update_kernel(int n, double *uf, const double *du) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) uf[i] += du[i];
@@ -350,10 +486,55 @@ double path_at(const std::vector<double> &v, double dt, double factor, double t)
     return factor * (v1 + (v2 - v1) * (incr - (double)i1));
 }
 
+// Band factor + solve on the stream: the register-window kernel when the
+// half bandwidth fits it (variant 0 or 2), else the global-memory one.
+int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
+                      double *d_status) {
+    const bool reg = variant == 2 || (variant == 0 && hbw + 1 <= kF3RegMaxP);
+    if (reg && hbw + 1 > kF3RegMaxP)
+        return fail(ctx, SGPU_ERR_ARG, "half bandwidth %d exceeds the register-window solve (%d)", hbw,
+                    kF3RegMaxP - 1);
+    const size_t lds = sizeof(double) * (size_t)(reg ? n : n + 2 * hbw);
+    if (lds > 60 * 1024) return fail(ctx, SGPU_ERR_ARG, "band of %d rows: the solve keeps them in LDS", n);
+    if (reg)
+        hipLaunchKernelGGL(f3_solve_reg_kernel, dim3(1), dim3(kF3SolveThreads), lds, ctx->stream, n, hbw, d_band,
+                           d_rhs, d_x, d_status);
+    else
+        hipLaunchKernelGGL(f3_solve_kernel, dim3(1), dim3(kF3SolveThreads), lds, ctx->stream, n, hbw, d_band, d_rhs,
+                           d_x, d_status);
+    HIP_OK(ctx, hipGetLastError());
+    return SGPU_OK;
+}
+
 }  // namespace
 }  // namespace sgpu
 
 using namespace sgpu;
+
+int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
+                    int32_t *info) {
+    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 2)
+        return SGPU_ERR_ARG;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    const size_t slots = (size_t)n * (2 * hbw + 1);
+    double *d = nullptr;
+    HIP_OK(ctx, hipMalloc((void **)&d, sizeof(double) * (slots + 2 * (size_t)n + 1)));
+    double *d_rhs = d + slots, *d_x = d_rhs + n, *d_st = d_x + n;
+    int rc = SGPU_OK;
+    if (hipMemcpy(d, band, sizeof(double) * slots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_rhs, rhs, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(ctx, SGPU_ERR_HIP, "band upload failed");
+    } else if ((rc = launch_band_solve(ctx, variant, n, hbw, d, d_rhs, d_x, d_st)) == SGPU_OK) {
+        double st = 0.0;
+        if (hipMemcpyAsync(x, d_x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(&st, d_st, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = fail(ctx, SGPU_ERR_HIP, "band solve readback failed");
+        *info = (int32_t)st;
+    }
+    (void)hipFree(d);
+    return rc;
+}
 
 int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_out, int32_t *free_count_out) {
     if (!ctx || !d) return SGPU_ERR_ARG;
@@ -623,7 +804,6 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
     const double a3 = 1.0 / (2.0 * beta) - 1.0;
     const int has_damping = h.alpha_m != 0.0 || h.beta_k != 0.0 || h.beta_k_init != 0.0;
     const int64_t slots = (int64_t)n * h.W;
-    const size_t solve_lds = sizeof(double) * (size_t)(n + 2 * h.hbw);
     int rc;
     *status = 0;
     *steps_done = 0;
@@ -675,10 +855,8 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
                                h.beta_k_init, has_damping);
             {
                 ProfScope ps(ctx, SGPU_KERNEL_F3_SOLVE);
-                hipLaunchKernelGGL(f3_solve_kernel, dim3(1), dim3(kF3SolveThreads), solve_lds, st, n, h.hbw, h.d_band,
-                                   h.d_R, h.d_du, h.d_scalar + 2);
+                if ((rc = launch_band_solve(ctx, 0, n, h.hbw, h.d_band, h.d_R, h.d_du, h.d_scalar + 2))) return rc;
             }
-            HIP_OK(ctx, hipGetLastError());
             HIP_OK(ctx, hipMemcpyAsync(host_scalar + 2, h.d_scalar + 2, sizeof(double), hipMemcpyDeviceToHost, st));
             hipLaunchKernelGGL(f3_update_kernel, grid_of(n), dim3(256), 0, st, n, h.d_uf, h.d_du);
             hipLaunchKernelGGL(f3_expand_kernel, grid_of(h.total), dim3(256), 0, st, h.total, h.d_u, h.d_uf,

@@ -49,6 +49,9 @@ def _declare(lib):
                                      POINTER(c_int32), POINTER(c_int32)]
     lib.sgpu_frame3d_effective_band.restype = c_int
     lib.sgpu_frame3d_effective_band.argtypes = [c_void_p, POINTER(c_double)]
+    lib.sgpu_band_solve.restype = c_int
+    lib.sgpu_band_solve.argtypes = [c_void_p, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double),
+                                    POINTER(c_double), POINTER(c_int32)]
     lib._f3_declared = True
 
 
@@ -122,3 +125,17 @@ class Frame3dRun:
 
     def close(self) -> None:
         self.rt.close()
+
+
+def band_solve(rt: SectionRuntime, band: np.ndarray, rhs: np.ndarray, variant: int = 0):
+    """Factor and solve one BandGeneral system on the device
+    (sgpu_band_solve); returns (x, info)."""
+    _declare(rt.lib)
+    band = np.ascontiguousarray(band, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    n, w = band.shape
+    x = np.zeros(n)
+    info = c_int32()
+    rt._check(rt.lib.sgpu_band_solve(rt.ctx, variant, n, (w - 1) // 2, dptr(band), dptr(rhs), dptr(x),
+                                     ctypes.byref(info)))
+    return x, info.value

@@ -70,3 +70,56 @@ def test_gpu_frame_without_constraints_matches_oracle():
     case = cases.rc_frame3d_case(2, 1, 1, steps=60, dt=0.005, n_core=4, n_bars=4, diaphragms=False,
                                  bidirectional=False, pgas_g=(0.6, 0.0))
     _compare(load_frame3d_case(case), 0, 60)
+
+
+def _spd_band(n, hbw, seed, dominant):
+    """A symmetric positive definite band: diagonally dominant, or with
+    off-diagonal entries above the diagonal (where the reference's factor
+    moves to its dense partial-pivot LU)."""
+    rng = np.random.default_rng(seed)
+    A = np.zeros((n, n))
+    if dominant:
+        for d in range(1, hbw + 1):
+            v = rng.normal(size=n - d)
+            A[np.arange(n - d), np.arange(d, n)] = v
+            A[np.arange(d, n), np.arange(n - d)] = v
+        A[np.arange(n), np.arange(n)] = np.abs(A).sum(axis=1) + 1.0
+    else:  # B B^T with B lower banded (half the band) and a small B[0, 0]
+        B = np.diag(1.0 + np.abs(rng.normal(size=n)))
+        h2 = hbw // 2
+        for d in range(1, h2 + 1):
+            B[np.arange(d, n), np.arange(n - d)] = rng.normal(size=n - d) * (0.5 / h2)
+        B[0, 0] = 0.1
+        if n > 1 and h2 > 0:
+            B[1, 0] = 0.8  # A[1, 0] = 0.08 outweighs A[0, 0] = 0.01
+        A = B @ B.T
+    band = np.zeros((n, 2 * hbw + 1))
+    for i in range(n):
+        for j in range(max(0, i - hbw), min(n - 1, i + hbw) + 1):
+            band[i, j - i + hbw] = A[i, j]
+    return A, band
+
+
+@pytest.mark.parametrize("n,hbw", [(1, 0), (7, 0), (9, 3), (40, 39), (300, 110), (520, 127), (200, 150)])
+@pytest.mark.parametrize("dominant", [True, False])
+def test_gpu_band_solve_kernels_match_the_reference_backend(n, hbw, dominant):
+    from strut_amd.model import MaterialTable
+    from strut_amd.sections import SectionRuntime
+    from strut_amd.transient3d import band_solve
+
+    A, band = _spd_band(n, hbw, seed=n + hbw, dominant=dominant)
+    rhs = np.random.default_rng(n).normal(size=n)
+    ok, want, fallback = oracle.band_solve(band, hbw, rhs)
+    assert ok
+    rt = SectionRuntime(MaterialTable([{"id": 1, "type": "Elastic", "params": {"E": 1.0}}]), device=0)
+    variants = (1, 2) if hbw <= 127 else (1,)
+    for variant in variants:
+        x, info = band_solve(rt, band, rhs, variant=variant)
+        print(f"n {n} hbw {hbw} dominant {dominant} variant {variant}: info {info} fallback {fallback} "
+              f"err {np.abs(x - want).max() / np.abs(want).max():.2e}", flush=True)
+        assert info == (1 if fallback else 0)
+        assert np.abs(x - want).max() <= 1e-12 * np.abs(want).max()
+    if hbw > 127:
+        with pytest.raises(Exception, match="register-window"):
+            band_solve(rt, band, rhs, variant=2)
+    rt.close()
