@@ -307,8 +307,10 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
  * + hbw), no pivoting.  *info: 0 solved; 1 solved, and the reference would
  * have moved this factorization to its dense partial-pivot LU (:608-628);
  * 2 a pivot at or below 1e-14, nothing solved.  variant 0 picks the kernel
- * (the register-window one up to half bandwidth 127), 1 the global-memory
- * kernel, 2 the register-window kernel.  Host buffers; allocates. */
+ * (the blocked one up to half bandwidth 119 and about 3300 rows, then the
+ * register-window one up to 127, then the global-memory one), 1 the
+ * global-memory kernel, 2 the register-window kernel, 3 the blocked kernel.
+ * Host buffers; allocates. */
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info);
 

@@ -454,6 +454,323 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_reg_kernel(int n, in
     }
 }
 
+// The factorization and substitutions with no global memory access on the
+// sequential path, for half bandwidths up to kF3BlkMaxP - 1 and n up to what
+// LDS holds.  Roles: wavefront 15 writes the finished U rows back to the
+// band (so no other wavefront ever waits on a store); threads below B^2
+// (B = ceil(P / 4)) own 4 x 4 blocks of window slots in registers, so one
+// column's update reads 4 multipliers and 4 pivot-row entries from LDS for
+// 16 products; the rows and columns entering the window are fetched a batch
+// (kF3BlkD columns) ahead by every thread below 960 into registers and land
+// in an LDS ring at the next batch start, long after their loads completed.
+// The back substitution stages U rows in LDS, kF3BlkR rows at a time,
+// loaded by wavefronts 1-15 while wavefront 0 solves the previous rows.
+// A workgroup barrier that waits only for this wavefront's LDS traffic
+// (lgkmcnt(0); vmcnt and expcnt left at their maxima in the gfx9 encoding),
+// so global stores and prefetch loads stay in flight across it.  The empty
+// asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+constexpr int kF3BlkMaxP = 120;                 // B <= 30: 900 owner threads
+constexpr int kF3BlkD = 8;                      // columns per prefetch batch
+constexpr int kF3BlkR = 16;                     // U rows per staged batch
+constexpr int kF3BlkWorkers = 960;              // wavefronts 0-14
+constexpr int kF3BlkRing = 2 * kF3BlkD * 2 * kF3BlkMaxP;  // doubles; also holds 2 x kF3BlkR U rows
+
+// Value of lane l + 1 of the wavefront (lane 63: 0).
+__device__ __forceinline__ double wave_next_lane(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, int hbw, double *band,
+                                                                       const double *rhs, double *x, double *status) {
+    extern __shared__ double s_dyn[];
+    double *s_ring = s_dyn;             // [2 kF3BlkD][2][kF3BlkMaxP]: entering row / column values
+    double *s_y = s_dyn + kF3BlkRing;   // n
+    __shared__ double s_col[2][kF3BlkMaxP], s_u[2][kF3BlkMaxP], s_rdiag[2];
+    __shared__ int s_flag;
+    const int P = hbw + 1, W = 2 * hbw + 1, tid = threadIdx.x;
+    const int B = (P + 3) / 4;
+    const bool owner = tid < B * B;
+    // owned window slots: rows 4 br + r, columns bc + c B (r, c < 4): lanes
+    // of a wavefront read consecutive pivot-row entries
+    const int br = owner ? tid / B : 0, bc = owner ? tid % B : 0;
+    const int per_batch = kF3BlkD * 2 * P;  // values entering the window per batch
+    // value q of the batch starting at column s0: column s0 + q / (2P) takes
+    // row T = s + P (part < P: column slot part) or column T (row slot part - P)
+    auto fetch = [&](int s0, int q) -> double {
+        const int s = s0 + q / (2 * P), part = q % (2 * P);
+        const int T = s + P, base = s + 1, bslot = base % P;
+        int idx;
+        bool ok;
+        if (part < P) {
+            const int j = base + (part - bslot + P) % P;
+            ok = T < n && j < n;
+            idx = T * W + (j - T + hbw);
+        } else {
+            const int i = base + (part - P - bslot + P) % P;
+            ok = T < n && i < n && i != T;
+            idx = i * W + (T - i + hbw);
+        }
+        const double v = band[ok ? idx : 0];
+        return ok ? v : 0.0;
+    };
+    auto ring_at = [&](int s, int part, int slot) -> double & {
+        return s_ring[((s % (2 * kF3BlkD)) * 2 + part) * kF3BlkMaxP + slot];
+    };
+    double a[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = 4 * br + r, j = bc + c * B;  // at k = 0 the slots are the indices
+            a[r][c] = (owner && i < P && j < P && i < n && j < n) ? band[(size_t)i * W + (j - i + hbw)] : 0.0;
+        }
+    for (int i = tid; i < n; i += kF3SolveThreads) s_y[i] = rhs[i];
+    // slots >= P keep pivot-column and pivot-row entries 0: the fast path
+    // updates their (zero, unused) window entries without a test
+    for (int i = tid; i < kF3BlkMaxP; i += kF3SolveThreads) s_u[0][i] = s_u[1][i] = s_col[0][i] = s_col[1][i] = 0.0;
+    if (tid == 0) s_flag = 0;
+    // batch 0 straight into the ring; batch 1 into registers
+    if (tid < kF3BlkWorkers)
+        for (int q = tid; q < per_batch; q += kF3BlkWorkers) {
+            const int s = q / (2 * P), part = q % (2 * P);
+            ring_at(s, part < P ? 0 : 1, part < P ? part : part - P) = fetch(0, q);
+        }
+    double pf[2] = {0.0, 0.0};
+    int pf_so[2], pf_part[2];  // the thread's two batch values: column offset and part, fixed
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int q = tid + m * kF3BlkWorkers;
+        const bool v = tid < kF3BlkWorkers && q < per_batch;
+        pf_so[m] = v ? q / (2 * P) : -1;
+        pf_part[m] = v ? q % (2 * P) : 0;
+        if (v) pf[m] = fetch(kF3BlkD, q);
+    }
+    int land_mod = (kF3BlkD + 1) % P;  // (k + 1) % P at the next landing k, kept without division
+    __syncthreads();
+    int kc = 0, kq = 0, kr = 0;  // kc = k mod P = kq B + kr
+    for (int k = 0; k < n; ++k) {
+        const int buf = k & 1;
+        if ((k & (kF3BlkD - 1)) == 0 && k > 0) {  // batch k / D lands; fetch the next
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                if (pf_so[m] < 0) continue;
+                const int part = pf_part[m];
+                ring_at(k + pf_so[m], part < P ? 0 : 1, part < P ? part : part - P) = pf[m];
+            }
+            int nm = land_mod + kF3BlkD;  // (k + D + 1) % P: the next batch's first base slot
+            while (nm >= P) nm -= P;
+            land_mod = nm;
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                if (pf_so[m] < 0) continue;
+                const int so = pf_so[m], part = pf_part[m];
+                const int s = k + kF3BlkD + so, T = s + P, base = s + 1;
+                int bslot = nm + so;
+                while (bslot >= P) bslot -= P;
+                int idx;
+                bool ok;
+                if (part < P) {
+                    int d = part - bslot;
+                    if (d < 0) d += P;
+                    const int j = base + d;
+                    ok = T < n && j < n;
+                    idx = T * W + (j - T + hbw);
+                } else {
+                    int d = part - P - bslot;
+                    if (d < 0) d += P;
+                    const int i = base + d;
+                    ok = T < n && i < n && i != T;
+                    idx = i * W + (T - i + hbw);
+                }
+                const double v = band[ok ? idx : 0];
+                pf[m] = ok ? v : 0.0;
+            }
+        }
+        // the pivot slot kc = 4 rq + rr = kq B + kr: its row lies in block row
+        // rq (offset rr), its column in column block c = kq of the lanes with
+        // bc == kr; rr and kq are the same for every lane
+        const int rq = kc >> 2, rr = kc & 3;
+        const bool in_row = owner && br == rq, in_col = owner && bc == kr;
+        // (A) pivot row, pivot and pivot column to LDS.  Slots >= P carry
+        // values no valid slot ever reads, so nothing here tests for them.
+        switch (rr) {
+#define F3_ROW(R)                                                                  \
+    case R:                                                                        \
+        if (in_row) {                                                              \
+            s_u[buf][bc] = a[R][0];                                                \
+            s_u[buf][bc + B] = a[R][1];                                            \
+            s_u[buf][bc + 2 * B] = a[R][2];                                        \
+            s_u[buf][bc + 3 * B] = a[R][3];                                        \
+        }                                                                          \
+        break;
+            F3_ROW(0) F3_ROW(1) F3_ROW(2) F3_ROW(3)
+#undef F3_ROW
+        }
+        switch (kq) {
+#define F3_COL(C)                                                                  \
+    case C:                                                                        \
+        if (in_col) {                                                              \
+            s_col[buf][4 * br] = a[0][C];                                          \
+            s_col[buf][4 * br + 1] = a[1][C];                                      \
+            s_col[buf][4 * br + 2] = a[2][C];                                      \
+            s_col[buf][4 * br + 3] = a[3][C];                                      \
+            if (in_row) { /* the pivot's owner publishes its reciprocal */         \
+                switch (rr) {                                                      \
+                    case 0: s_rdiag[buf] = 1.0 / a[0][C]; break;                   \
+                    case 1: s_rdiag[buf] = 1.0 / a[1][C]; break;                   \
+                    case 2: s_rdiag[buf] = 1.0 / a[2][C]; break;                   \
+                    default: s_rdiag[buf] = 1.0 / a[3][C]; break;                  \
+                }                                                                  \
+            }                                                                      \
+        }                                                                          \
+        break;
+            F3_COL(0) F3_COL(1) F3_COL(2) F3_COL(3)
+#undef F3_COL
+        }
+        lds_barrier();
+        const double diag = s_u[buf][kc];  // the pivot row's entry at the pivot column
+        if (fabs(diag) <= 1.0e-14) {
+            if (tid == 0) status[0] = 2.0;
+            return;
+        }
+        const double rdiag = s_rdiag[buf];
+        // (C) rank-1 update with the multipliers l = a_ik / a_kk (as a_ik
+        // times the reciprocal: within one ulp of the reference's quotient);
+        // then the retired row and column slots take row and column k + P
+        // (the update wrote junk there)
+        if (owner) {
+            double l[4], u[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) l[r] = s_col[buf][4 * br + r] * rdiag;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) u[c] = s_u[buf][bc + c * B];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) a[r][c] -= l[r] * u[c];
+            switch (rr) {
+#define F3_ROWIN(R)                                                                \
+    case R:                                                                        \
+        if (in_row) {                                                              \
+            a[R][0] = ring_at(k, 0, bc);                                           \
+            a[R][1] = ring_at(k, 0, bc + B);                                       \
+            a[R][2] = ring_at(k, 0, bc + 2 * B);                                   \
+            a[R][3] = ring_at(k, 0, bc + 3 * B);                                   \
+        }                                                                          \
+        break;
+                F3_ROWIN(0) F3_ROWIN(1) F3_ROWIN(2) F3_ROWIN(3)
+#undef F3_ROWIN
+            }
+            switch (kq) {
+#define F3_COLIN(C)                                                                \
+    case C:                                                                        \
+        if (in_col) {                                                              \
+            if (!(in_row && rr == 0)) a[0][C] = ring_at(k, 1, 4 * br);             \
+            if (!(in_row && rr == 1)) a[1][C] = ring_at(k, 1, 4 * br + 1);         \
+            if (!(in_row && rr == 2)) a[2][C] = ring_at(k, 1, 4 * br + 2);         \
+            if (!(in_row && rr == 3)) a[3][C] = ring_at(k, 1, 4 * br + 3);         \
+        }                                                                          \
+        break;
+                F3_COLIN(0) F3_COLIN(1) F3_COLIN(2) F3_COLIN(3)
+#undef F3_COLIN
+            }
+        } else if (tid >= kF3BlkWorkers) {
+            // wavefront 15: forward substitution (the same multipliers), the
+            // reference's dense-fallback test, row k of U back to the band
+            const double yk = s_y[k];
+            for (int rs = tid - kF3BlkWorkers; rs < P; rs += 64) {
+                const int d = rs >= kc ? rs - kc : rs - kc + P;
+                if (d != 0) {
+                    const double v = s_col[buf][rs];
+                    if (fabs(v) > fabs(diag)) s_flag = 1;
+                    if (k + d < n) s_y[k + d] -= (v * rdiag) * yk;
+                }
+                if (k + d < n) band[(size_t)k * W + (d + hbw)] = s_u[buf][rs];
+            }
+        }
+        if (++kr == B) {
+            kr = 0;
+            ++kq;
+        }
+        if (++kc == P) {
+            kc = 0;
+            kq = 0;
+            kr = 0;
+        }
+    }
+    if (tid >= kF3BlkWorkers) __threadfence();  // U rows visible to the staging loads
+    __syncthreads();
+    // Back substitution by columns in wavefront 0: at row i lane l holds
+    // y[i - l] and y[i - 64 - l]; x_i = y_i / U_ii, then every lane takes
+    // U(row, i) x_i off its rows and the window moves down one row (DPP lane
+    // shift; y[i - 128] enters from LDS, untouched so far because hbw < 128).
+    // Wavefronts 1-15 stage U columns (diagonal first) kF3BlkR at a time.
+    const int R = kF3BlkR, collen = kF3BlkMaxP;
+    auto stage = [&](int batch, int half) {
+        const int top = n - 1 - batch * R;  // columns top, top - 1, ..., top - R + 1
+        for (int q = tid - 64; q < R * P; q += kF3SolveThreads - 64) {
+            const int rr = q / P, c = q % P, i = top - rr;
+            double v = 0.0;
+            if (i >= 0 && i - c >= 0) v = band[(size_t)(i - c) * W + (c + hbw)];
+            s_ring[(half * R + rr) * collen + c] = v;
+        }
+    };
+    const int n_batches = (n + R - 1) / R;
+    if (tid >= 64) stage(0, 0);
+    __syncthreads();
+    const int lane = tid & 63;
+    double ya = 0.0, yb = 0.0;
+    if (tid < 64) {
+        ya = n - 1 - lane >= 0 ? s_y[n - 1 - lane] : 0.0;
+        yb = n - 65 - lane >= 0 ? s_y[n - 65 - lane] : 0.0;
+    }
+    for (int b = 0; b < n_batches; ++b) {
+        const int half = b & 1;
+        if (tid >= 64) {
+            if (b + 1 < n_batches) stage(b + 1, half ^ 1);
+        } else {
+            const int top = n - 1 - b * R;
+            for (int rr = 0; rr < R; ++rr) {
+                const int i = top - rr;
+                if (i < 0) break;
+                const double *col = s_ring + (half * R + rr) * collen;
+                const double ua = lane >= 1 && lane <= hbw ? col[lane] : 0.0;
+                const double ub = lane + 64 <= hbw ? col[64 + lane] : 0.0;
+                const double yin = i - 128 >= 0 ? s_y[i - 128] : 0.0;
+                const double xi = lane_value(ya, 0) / col[0];
+                ya -= ua * xi;
+                yb -= ub * xi;
+                if (lane == 0) s_y[i] = xi;  // y_i is spent: x_i takes its place
+                const double yb0 = lane_value(yb, 0);
+                ya = wave_next_lane(ya);
+                yb = wave_next_lane(yb);
+                if (lane == 63) {
+                    ya = yb0;
+                    yb = yin;
+                }
+            }
+        }
+        lds_barrier();
+    }
+    for (int i = tid; i < n; i += kF3SolveThreads) x[i] = s_y[i];
+    if (tid == 0) status[0] = s_flag ? 1.0 : 0.0;
+}
+
 __global__ void f3_update_kernel(int n, double *uf, const double *du) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) uf[i] += du[i];
@@ -490,12 +807,24 @@ double path_at(const std::vector<double> &v, double dt, double factor, double t)
 // half bandwidth fits it (variant 0 or 2), else the global-memory one.
 int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
                       double *d_status) {
-    const bool reg = variant == 2 || (variant == 0 && hbw + 1 <= kF3RegMaxP);
+    const size_t blk_lds = sizeof(double) * ((size_t)kF3BlkRing + n);
+    // dynamic LDS stays under 56 KiB: the kernels' static arrays take the rest of 64 KiB
+    if (variant == 0) variant = (hbw + 1 <= kF3BlkMaxP && blk_lds <= 56 * 1024) ? 3 : hbw + 1 <= kF3RegMaxP ? 2 : 1;
+    if (variant == 3) {
+        if (hbw + 1 > kF3BlkMaxP || blk_lds > 56 * 1024)
+            return fail(ctx, SGPU_ERR_ARG, "band %d x %d exceeds the blocked solve (half bandwidth <= %d, LDS)", n,
+                        2 * hbw + 1, kF3BlkMaxP - 1);
+        hipLaunchKernelGGL(f3_solve_blk_kernel, dim3(1), dim3(kF3SolveThreads), blk_lds, ctx->stream, n, hbw, d_band,
+                           d_rhs, d_x, d_status);
+        HIP_OK(ctx, hipGetLastError());
+        return SGPU_OK;
+    }
+    const bool reg = variant == 2;
     if (reg && hbw + 1 > kF3RegMaxP)
         return fail(ctx, SGPU_ERR_ARG, "half bandwidth %d exceeds the register-window solve (%d)", hbw,
                     kF3RegMaxP - 1);
     const size_t lds = sizeof(double) * (size_t)(reg ? n : n + 2 * hbw);
-    if (lds > 60 * 1024) return fail(ctx, SGPU_ERR_ARG, "band of %d rows: the solve keeps them in LDS", n);
+    if (lds > 56 * 1024) return fail(ctx, SGPU_ERR_ARG, "band of %d rows: the solve keeps them in LDS", n);
     if (reg)
         hipLaunchKernelGGL(f3_solve_reg_kernel, dim3(1), dim3(kF3SolveThreads), lds, ctx->stream, n, hbw, d_band,
                            d_rhs, d_x, d_status);
@@ -513,7 +842,7 @@ using namespace sgpu;
 
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info) {
-    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 2)
+    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 3)
         return SGPU_ERR_ARG;
     HIP_OK(ctx, hipSetDevice(ctx->device));
     const size_t slots = (size_t)n * (2 * hbw + 1);
@@ -646,9 +975,9 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
     const int W = 2 * hbw + 1;
     const int64_t slots = (int64_t)n * W;
     if (slots > (int64_t)1 << 31) return fail(ctx, SGPU_ERR_ARG, "band of %d x %d is too large", n, W);
-    if ((size_t)(n + 2 * hbw) * sizeof(double) > 60 * 1024)
+    if ((size_t)(n + 2 * hbw) * sizeof(double) > 56 * 1024)
         return fail(ctx, SGPU_ERR_ARG, "%d free DOFs: the solve keeps the right-hand side in LDS (at most %zu)", n,
-                    60 * 1024 / sizeof(double) - 2 * hbw);
+                    56 * 1024 / sizeof(double) - 2 * hbw);
     std::vector<int32_t> moff(n + 1, 0), mcol;
     std::vector<double> mval, mass_band(slots, 0.0);
     for (int r = 0; r < n; ++r) {
