@@ -309,8 +309,9 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
  * 2 a pivot at or below 1e-14, nothing solved.  variant 0 picks the kernel
  * (the blocked one up to half bandwidth 119 and about 3300 rows, then the
  * register-window one up to 127, then the global-memory one), 1 the
- * global-memory kernel, 2 the register-window kernel, 3 the blocked kernel.
- * Host buffers; allocates. */
+ * global-memory kernel, 2 the register-window kernel, 3 the blocked kernel,
+ * 4 the blocked kernel with 8-row blocks on 512 threads (half bandwidth up
+ * to 115).  Host buffers; allocates. */
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info);
 

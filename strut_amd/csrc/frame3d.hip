@@ -455,16 +455,23 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_reg_kernel(int n, in
 }
 
 // The factorization and substitutions with no global memory access on the
-// sequential path, for half bandwidths up to kF3BlkMaxP - 1 and n up to what
-// LDS holds.  Roles: wavefront 15 writes the finished U rows back to the
-// band (so no other wavefront ever waits on a store); threads below B^2
-// (B = ceil(P / 4)) own 4 x 4 blocks of window slots in registers, so one
-// column's update reads 4 multipliers and 4 pivot-row entries from LDS for
-// 16 products; the rows and columns entering the window are fetched a batch
-// (kF3BlkD columns) ahead by every thread below 960 into registers and land
-// in an LDS ring at the next batch start, long after their loads completed.
-// The back substitution stages U rows in LDS, kF3BlkR rows at a time,
-// loaded by wavefronts 1-15 while wavefront 0 solves the previous rows.
+// per-column path, for half bandwidths up to kF3BlkMaxP - 1 and n up to what
+// LDS holds (RB-row x 4-column owner blocks on NT threads):
+//   * the active window lives in registers; an owner holds rows RB br + r and
+//     columns bc + c Bc (Bc = ceil(P / 4)), so the lanes of a wavefront read
+//     consecutive pivot-row entries;
+//   * one barrier per column: the owners of the pivot row and column publish
+//     them (double-buffered LDS) and the pivot's owner its reciprocal; then
+//     every owner applies RB x 4 updates with l = a_ik (1 / a_kk), and two
+//     switches on wavefront-uniform values overwrite the retired row and
+//     column from the ring (no per-entry tests anywhere);
+//   * entering rows and columns are fetched kF3BlkD columns ahead into
+//     registers by every wavefront but the last, and land in an LDS ring;
+//   * the last wavefront does the forward substitution, the reference's
+//     dense-fallback test and the write-back of finished U rows, so no
+//     other wavefront issues a store;
+//   * the back substitution runs by columns in wavefront 0 with DPP lane
+//     shifts while the other wavefronts stage U columns in LDS.
 // A workgroup barrier that waits only for this wavefront's LDS traffic
 // (lgkmcnt(0); vmcnt and expcnt left at their maxima in the gfx9 encoding),
 // so global stores and prefetch loads stay in flight across it.  The empty
@@ -476,10 +483,9 @@ __device__ __forceinline__ void lds_barrier() {
     asm volatile("" ::: "memory");
 }
 
-constexpr int kF3BlkMaxP = 120;                 // B <= 30: 900 owner threads
+constexpr int kF3BlkMaxP = 120;                 // 4-row blocks: 30 x 30 owners on 1024 threads
 constexpr int kF3BlkD = 8;                      // columns per prefetch batch
 constexpr int kF3BlkR = 16;                     // U rows per staged batch
-constexpr int kF3BlkWorkers = 960;              // wavefronts 0-14
 constexpr int kF3BlkRing = 2 * kF3BlkD * 2 * kF3BlkMaxP;  // doubles; also holds 2 x kF3BlkR U rows
 
 // Value of lane l + 1 of the wavefront (lane 63: 0).
@@ -495,19 +501,22 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, int hbw, double *band,
-                                                                       const double *rhs, double *x, double *status) {
+template <int RB, int NT>
+__global__ __launch_bounds__(NT) void f3_solve_blk_kernel(int n, int hbw, double *band, const double *rhs, double *x,
+                                                       double *status) {
+    constexpr int kWorkers = NT - 64;  // all but the last wavefront
+    constexpr int kPf = (kF3BlkD * 2 * kF3BlkMaxP + kWorkers - 1) / kWorkers;
     extern __shared__ double s_dyn[];
     double *s_ring = s_dyn;             // [2 kF3BlkD][2][kF3BlkMaxP]: entering row / column values
     double *s_y = s_dyn + kF3BlkRing;   // n
     __shared__ double s_col[2][kF3BlkMaxP], s_u[2][kF3BlkMaxP], s_rdiag[2];
     __shared__ int s_flag;
     const int P = hbw + 1, W = 2 * hbw + 1, tid = threadIdx.x;
-    const int B = (P + 3) / 4;
-    const bool owner = tid < B * B;
-    // owned window slots: rows 4 br + r, columns bc + c B (r, c < 4): lanes
-    // of a wavefront read consecutive pivot-row entries
-    const int br = owner ? tid / B : 0, bc = owner ? tid % B : 0;
+    const int Bc = (P + 3) / 4, Br = (P + RB - 1) / RB;
+    const bool owner = tid < Br * Bc;
+    // owned window slots: rows RB br + r, columns bc + c Bc (r < RB, c < 4):
+    // lanes of a wavefront read consecutive pivot-row entries
+    const int br = owner ? tid / Bc : 0, bc = owner ? tid % Bc : 0;
     const int per_batch = kF3BlkD * 2 * P;  // values entering the window per batch
     // value q of the batch starting at column s0: column s0 + q / (2P) takes
     // row T = s + P (part < P: column slot part) or column T (row slot part - P)
@@ -531,31 +540,32 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
     auto ring_at = [&](int s, int part, int slot) -> double & {
         return s_ring[((s % (2 * kF3BlkD)) * 2 + part) * kF3BlkMaxP + slot];
     };
-    double a[4][4];
+    double a[RB][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < RB; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int i = 4 * br + r, j = bc + c * B;  // at k = 0 the slots are the indices
+            const int i = RB * br + r, j = bc + c * Bc;  // at k = 0 the slots are the indices
             a[r][c] = (owner && i < P && j < P && i < n && j < n) ? band[(size_t)i * W + (j - i + hbw)] : 0.0;
         }
-    for (int i = tid; i < n; i += kF3SolveThreads) s_y[i] = rhs[i];
+    for (int i = tid; i < n; i += NT) s_y[i] = rhs[i];
     // slots >= P keep pivot-column and pivot-row entries 0: the fast path
     // updates their (zero, unused) window entries without a test
-    for (int i = tid; i < kF3BlkMaxP; i += kF3SolveThreads) s_u[0][i] = s_u[1][i] = s_col[0][i] = s_col[1][i] = 0.0;
+    for (int i = tid; i < kF3BlkMaxP; i += NT) s_u[0][i] = s_u[1][i] = s_col[0][i] = s_col[1][i] = 0.0;
     if (tid == 0) s_flag = 0;
     // batch 0 straight into the ring; batch 1 into registers
-    if (tid < kF3BlkWorkers)
-        for (int q = tid; q < per_batch; q += kF3BlkWorkers) {
+    if (tid < kWorkers)
+        for (int q = tid; q < per_batch; q += kWorkers) {
             const int s = q / (2 * P), part = q % (2 * P);
             ring_at(s, part < P ? 0 : 1, part < P ? part : part - P) = fetch(0, q);
         }
-    double pf[2] = {0.0, 0.0};
-    int pf_so[2], pf_part[2];  // the thread's two batch values: column offset and part, fixed
+    double pf[kPf];
+    int pf_so[kPf], pf_part[kPf];  // the thread's batch values: column offset and part, fixed
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int q = tid + m * kF3BlkWorkers;
-        const bool v = tid < kF3BlkWorkers && q < per_batch;
+    for (int m = 0; m < kPf; ++m) {
+        pf[m] = 0.0;
+        const int q = tid + m * kWorkers;
+        const bool v = tid < kWorkers && q < per_batch;
         pf_so[m] = v ? q / (2 * P) : -1;
         pf_part[m] = v ? q % (2 * P) : 0;
         if (v) pf[m] = fetch(kF3BlkD, q);
@@ -567,7 +577,7 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
         const int buf = k & 1;
         if ((k & (kF3BlkD - 1)) == 0 && k > 0) {  // batch k / D lands; fetch the next
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
+            for (int m = 0; m < kPf; ++m) {
                 if (pf_so[m] < 0) continue;
                 const int part = pf_part[m];
                 ring_at(k + pf_so[m], part < P ? 0 : 1, part < P ? part : part - P) = pf[m];
@@ -576,7 +586,7 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
             while (nm >= P) nm -= P;
             land_mod = nm;
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
+            for (int m = 0; m < kPf; ++m) {
                 if (pf_so[m] < 0) continue;
                 const int so = pf_so[m], part = pf_part[m];
                 const int s = k + kF3BlkD + so, T = s + P, base = s + 1;
@@ -601,41 +611,36 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
                 pf[m] = ok ? v : 0.0;
             }
         }
-        // the pivot slot kc = 4 rq + rr = kq B + kr: its row lies in block row
-        // rq (offset rr), its column in column block c = kq of the lanes with
-        // bc == kr; rr and kq are the same for every lane
-        const int rq = kc >> 2, rr = kc & 3;
+        // the pivot slot kc = RB rq + rr = kq Bc + kr: its row lies in block
+        // row rq (offset rr), its column in column block c = kq of the lanes
+        // with bc == kr; rr and kq are the same for every lane
+        const int rq = kc / RB, rr = kc % RB;
         const bool in_row = owner && br == rq, in_col = owner && bc == kr;
-        // (A) pivot row, pivot and pivot column to LDS.  Slots >= P carry
-        // values no valid slot ever reads, so nothing here tests for them.
+        // (A) pivot row, pivot column and the pivot's reciprocal to LDS.
+        // Slots >= P carry values no valid slot ever reads, so nothing here
+        // tests for them.
         switch (rr) {
 #define F3_ROW(R)                                                                  \
     case R:                                                                        \
-        if (in_row) {                                                              \
-            s_u[buf][bc] = a[R][0];                                                \
-            s_u[buf][bc + B] = a[R][1];                                            \
-            s_u[buf][bc + 2 * B] = a[R][2];                                        \
-            s_u[buf][bc + 3 * B] = a[R][3];                                        \
+        if constexpr (R < RB) {                                                    \
+            if (in_row) {                                                          \
+                s_u[buf][bc] = a[R][0];                                            \
+                s_u[buf][bc + Bc] = a[R][1];                                       \
+                s_u[buf][bc + 2 * Bc] = a[R][2];                                   \
+                s_u[buf][bc + 3 * Bc] = a[R][3];                                   \
+            }                                                                      \
         }                                                                          \
         break;
-            F3_ROW(0) F3_ROW(1) F3_ROW(2) F3_ROW(3)
+            F3_ROW(0) F3_ROW(1) F3_ROW(2) F3_ROW(3) F3_ROW(4) F3_ROW(5) F3_ROW(6) F3_ROW(7)
 #undef F3_ROW
         }
         switch (kq) {
 #define F3_COL(C)                                                                  \
     case C:                                                                        \
         if (in_col) {                                                              \
-            s_col[buf][4 * br] = a[0][C];                                          \
-            s_col[buf][4 * br + 1] = a[1][C];                                      \
-            s_col[buf][4 * br + 2] = a[2][C];                                      \
-            s_col[buf][4 * br + 3] = a[3][C];                                      \
-            if (in_row) { /* the pivot's owner publishes its reciprocal */         \
-                switch (rr) {                                                      \
-                    case 0: s_rdiag[buf] = 1.0 / a[0][C]; break;                   \
-                    case 1: s_rdiag[buf] = 1.0 / a[1][C]; break;                   \
-                    case 2: s_rdiag[buf] = 1.0 / a[2][C]; break;                   \
-                    default: s_rdiag[buf] = 1.0 / a[3][C]; break;                  \
-                }                                                                  \
+            _Pragma("unroll") for (int r = 0; r < RB; ++r) {                       \
+                s_col[buf][RB * br + r] = a[r][C];                                 \
+                if (in_row && r == rr) s_rdiag[buf] = 1.0 / a[r][C];               \
             }                                                                      \
         }                                                                          \
         break;
@@ -654,46 +659,46 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
         // then the retired row and column slots take row and column k + P
         // (the update wrote junk there)
         if (owner) {
-            double l[4], u[4];
+            double l[RB], u[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) l[r] = s_col[buf][4 * br + r] * rdiag;
+            for (int r = 0; r < RB; ++r) l[r] = s_col[buf][RB * br + r] * rdiag;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) u[c] = s_u[buf][bc + c * B];
+            for (int c = 0; c < 4; ++c) u[c] = s_u[buf][bc + c * Bc];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < RB; ++r)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) a[r][c] -= l[r] * u[c];
             switch (rr) {
 #define F3_ROWIN(R)                                                                \
     case R:                                                                        \
-        if (in_row) {                                                              \
-            a[R][0] = ring_at(k, 0, bc);                                           \
-            a[R][1] = ring_at(k, 0, bc + B);                                       \
-            a[R][2] = ring_at(k, 0, bc + 2 * B);                                   \
-            a[R][3] = ring_at(k, 0, bc + 3 * B);                                   \
+        if constexpr (R < RB) {                                                    \
+            if (in_row) {                                                          \
+                a[R][0] = ring_at(k, 0, bc);                                       \
+                a[R][1] = ring_at(k, 0, bc + Bc);                                  \
+                a[R][2] = ring_at(k, 0, bc + 2 * Bc);                              \
+                a[R][3] = ring_at(k, 0, bc + 3 * Bc);                              \
+            }                                                                      \
         }                                                                          \
         break;
-                F3_ROWIN(0) F3_ROWIN(1) F3_ROWIN(2) F3_ROWIN(3)
+                F3_ROWIN(0) F3_ROWIN(1) F3_ROWIN(2) F3_ROWIN(3) F3_ROWIN(4) F3_ROWIN(5) F3_ROWIN(6) F3_ROWIN(7)
 #undef F3_ROWIN
             }
             switch (kq) {
 #define F3_COLIN(C)                                                                \
     case C:                                                                        \
         if (in_col) {                                                              \
-            if (!(in_row && rr == 0)) a[0][C] = ring_at(k, 1, 4 * br);             \
-            if (!(in_row && rr == 1)) a[1][C] = ring_at(k, 1, 4 * br + 1);         \
-            if (!(in_row && rr == 2)) a[2][C] = ring_at(k, 1, 4 * br + 2);         \
-            if (!(in_row && rr == 3)) a[3][C] = ring_at(k, 1, 4 * br + 3);         \
+            _Pragma("unroll") for (int r = 0; r < RB; ++r)                         \
+                if (!(in_row && r == rr)) a[r][C] = ring_at(k, 1, RB * br + r);    \
         }                                                                          \
         break;
                 F3_COLIN(0) F3_COLIN(1) F3_COLIN(2) F3_COLIN(3)
 #undef F3_COLIN
             }
-        } else if (tid >= kF3BlkWorkers) {
+        } else if (tid >= kWorkers) {
             // wavefront 15: forward substitution (the same multipliers), the
             // reference's dense-fallback test, row k of U back to the band
             const double yk = s_y[k];
-            for (int rs = tid - kF3BlkWorkers; rs < P; rs += 64) {
+            for (int rs = tid - kWorkers; rs < P; rs += 64) {
                 const int d = rs >= kc ? rs - kc : rs - kc + P;
                 if (d != 0) {
                     const double v = s_col[buf][rs];
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
                 if (k + d < n) band[(size_t)k * W + (d + hbw)] = s_u[buf][rs];
             }
         }
-        if (++kr == B) {
+        if (++kr == Bc) {
             kr = 0;
             ++kq;
         }
@@ -713,17 +718,17 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
             kr = 0;
         }
     }
-    if (tid >= kF3BlkWorkers) __threadfence();  // U rows visible to the staging loads
+    if (tid >= kWorkers) __threadfence();  // U rows visible to the staging loads
     __syncthreads();
     // Back substitution by columns in wavefront 0: at row i lane l holds
     // y[i - l] and y[i - 64 - l]; x_i = y_i / U_ii, then every lane takes
     // U(row, i) x_i off its rows and the window moves down one row (DPP lane
     // shift; y[i - 128] enters from LDS, untouched so far because hbw < 128).
-    // Wavefronts 1-15 stage U columns (diagonal first) kF3BlkR at a time.
+    // The other wavefronts stage U columns (diagonal first) kF3BlkR at a time.
     const int R = kF3BlkR, collen = kF3BlkMaxP;
     auto stage = [&](int batch, int half) {
         const int top = n - 1 - batch * R;  // columns top, top - 1, ..., top - R + 1
-        for (int q = tid - 64; q < R * P; q += kF3SolveThreads - 64) {
+        for (int q = tid - 64; q < R * P; q += NT - 64) {
             const int rr = q / P, c = q % P, i = top - rr;
             double v = 0.0;
             if (i >= 0 && i - c >= 0) v = band[(size_t)(i - c) * W + (c + hbw)];
@@ -767,7 +772,7 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_blk_kernel(int n, in
         }
         lds_barrier();
     }
-    for (int i = tid; i < n; i += kF3SolveThreads) x[i] = s_y[i];
+    for (int i = tid; i < n; i += NT) x[i] = s_y[i];
     if (tid == 0) status[0] = s_flag ? 1.0 : 0.0;
 }
 
@@ -808,14 +813,21 @@ double path_at(const std::vector<double> &v, double dt, double factor, double t)
 int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
                       double *d_status) {
     const size_t blk_lds = sizeof(double) * ((size_t)kF3BlkRing + n);
+    const int P = hbw + 1, Bc = (P + 3) / 4;
+    const bool fits8 = ((P + 7) / 8) * Bc <= 512 - 64, fits4 = ((P + 3) / 4) * Bc <= 1024 - 64;
     // dynamic LDS stays under 56 KiB: the kernels' static arrays take the rest of 64 KiB
-    if (variant == 0) variant = (hbw + 1 <= kF3BlkMaxP && blk_lds <= 56 * 1024) ? 3 : hbw + 1 <= kF3RegMaxP ? 2 : 1;
-    if (variant == 3) {
-        if (hbw + 1 > kF3BlkMaxP || blk_lds > 56 * 1024)
+    const bool lds_ok = P <= kF3BlkMaxP && blk_lds <= 56 * 1024;
+    if (variant == 0) variant = lds_ok && fits4 ? 3 : hbw + 1 <= kF3RegMaxP ? 2 : 1;
+    if (variant == 3 || variant == 4) {
+        if (!lds_ok || (variant == 4 ? !fits8 : !fits4))
             return fail(ctx, SGPU_ERR_ARG, "band %d x %d exceeds the blocked solve (half bandwidth <= %d, LDS)", n,
-                        2 * hbw + 1, kF3BlkMaxP - 1);
-        hipLaunchKernelGGL(f3_solve_blk_kernel, dim3(1), dim3(kF3SolveThreads), blk_lds, ctx->stream, n, hbw, d_band,
-                           d_rhs, d_x, d_status);
+                        2 * hbw + 1, variant == 4 ? 115 : kF3BlkMaxP - 1);
+        if (variant == 4)
+            hipLaunchKernelGGL((f3_solve_blk_kernel<8, 512>), dim3(1), dim3(512), blk_lds, ctx->stream, n, hbw, d_band,
+                               d_rhs, d_x, d_status);
+        else
+            hipLaunchKernelGGL((f3_solve_blk_kernel<4, 1024>), dim3(1), dim3(1024), blk_lds, ctx->stream, n, hbw,
+                               d_band, d_rhs, d_x, d_status);
         HIP_OK(ctx, hipGetLastError());
         return SGPU_OK;
     }
@@ -842,7 +854,7 @@ using namespace sgpu;
 
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info) {
-    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 3)
+    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 4)
         return SGPU_ERR_ARG;
     HIP_OK(ctx, hipSetDevice(ctx->device));
     const size_t slots = (size_t)n * (2 * hbw + 1);

@@ -113,7 +113,7 @@ def test_gpu_band_solve_kernels_match_the_reference_backend(n, hbw, dominant):
     ok, want, fallback = oracle.band_solve(band, hbw, rhs)
     assert ok
     rt = SectionRuntime(MaterialTable([{"id": 1, "type": "Elastic", "params": {"E": 1.0}}]), device=0)
-    variants = (1, 2, 3) if hbw <= 119 else (1, 2) if hbw <= 127 else (1,)
+    variants = (1, 2, 3, 4) if hbw <= 115 else (1, 2, 3) if hbw <= 119 else (1, 2) if hbw <= 127 else (1,)
     for variant in variants:
         x, info = band_solve(rt, band, rhs, variant=variant)
         print(f"n {n} hbw {hbw} dominant {dominant} variant {variant}: info {info} fallback {fallback} "
