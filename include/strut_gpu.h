@@ -315,6 +315,23 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info);
 
+/* Recorder of the 3D frame: the full displacement of each listed DOF
+ * (6 node + dof - 1) after every converged step, up to max_steps, kept on
+ * the device (node_displacement recorders, transient_nonlinear.mojo:3228-
+ * 3272).  Set before sgpu_frame3d_run. */
+int sgpu_frame3d_set_recorder(sgpu_ctx *ctx, int n_record, const int32_t *dofs, int max_steps);
+/* The recorded rows (steps x n_record) and their number. */
+int sgpu_frame3d_recorded(sgpu_ctx *ctx, double *hist, int32_t *steps);
+/* Envelopes over the recorded steps, computed on the device:
+ * disp_env[3 c + {0,1,2}] = min, max and largest magnitude of recorded
+ * column c (envelope_node_displacement, _update_envelope,
+ * helpers.mojo:3153-3186, the first step seeding all three);
+ * drift_env[3 d + ...] the same for drift channel d,
+ * (u[col_j[d]] - u[col_i[d]]) / dx[d] (_drift_value, helpers.mojo:3113-3142;
+ * dx = coordinate_j - coordinate_i along perp_dirn). */
+int sgpu_frame3d_envelopes(sgpu_ctx *ctx, int n_drift, const int32_t *col_i, const int32_t *col_j, const double *dx,
+                           double *disp_env, double *drift_env);
+
 /* The effective tangent K + a1 C + a0 M in the band (n_free x (2 hbw + 1),
  * row major, slot (i, j) at i (2 hbw + 1) + j - i + hbw), as the reference
  * loads it before each factorization (transient_nonlinear.mojo:2400-2411),
@@ -426,6 +443,16 @@ int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out5);
  * recovery failed) and steps completed.  Any pointer may be NULL. */
 int sgpu_frame2d_result(sgpu_ctx *ctx, int32_t instance, double *u_hist, int32_t *iters, int32_t *status,
                         int32_t *steps_done);
+
+/* Envelopes of every instance of frame_id over its completed steps,
+ * computed on the device from the recorded history (no history download):
+ * per instance, in instance order, disp_env[(i n_record + c) 3 + {0,1,2}]
+ * the min, max and largest magnitude of recorded column c, and
+ * drift_env[(i n_drift + d) 3 + ...] those of drift channel d,
+ * (u[col_j[d]] - u[col_i[d]]) / dx[d] (_update_envelope and _drift_value,
+ * helpers.mojo:3113-3186). */
+int sgpu_frame2d_envelopes(sgpu_ctx *ctx, int32_t frame_id, int n_drift, const int32_t *col_i, const int32_t *col_j,
+                           const double *dx, double *disp_env, double *drift_env);
 
 /* out4 = {free DOFs, band half-width, recorded DOFs, total DOFs}. */
 int sgpu_frame2d_info(sgpu_ctx *ctx, int32_t frame_id, int32_t *out4);

@@ -615,3 +615,33 @@ def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: i
     return {"status": status, "u": u_hist, "iters": iters, "steps_done": o.steps_done, "hbw": o.hbw,
             "free_count": o.free_count, "dense_fallbacks": o.dense_fallbacks, "band_first": band,
             "_elems": elems, "_sections": secs}
+
+
+# ---------------------------------------------------------------- recorders
+def envelopes(hist, steps_done: int, col_i=(), col_j=(), dx=()):
+    """Envelopes of a recorded history (steps x n_record) over its first
+    steps_done rows: per recorded column and per drift channel
+    (u[col_j] - u[col_i]) / dx (_drift_value, helpers.mojo:3113-3142), the
+    min, max and largest magnitude, the first step seeding all three and each
+    later step updating them in order (_update_envelope,
+    helpers.mojo:3153-3186).  Returns (disp_env (n_record, 3), drift_env
+    (n_drift, 3)); zeros when no step was recorded."""
+    hist = np.asarray(hist, dtype=np.float64)
+    n_record = hist.shape[1]
+    disp = np.zeros((n_record, 3))
+    drift = np.zeros((len(col_i), 3))
+    for s in range(steps_done):
+        row = hist[s]
+        values = list(row) + [(row[j] - row[i]) / d for i, j, d in zip(col_i, col_j, dx)]
+        for c, v in enumerate(values):
+            out = disp[c] if c < n_record else drift[c - n_record]
+            if s == 0:
+                out[0], out[1], out[2] = v, v, abs(v)
+                continue
+            if v < out[0]:
+                out[0] = v
+            if v > out[1]:
+                out[1] = v
+            if abs(v) > out[2]:
+                out[2] = abs(v)
+    return disp, drift

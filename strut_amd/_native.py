@@ -91,6 +91,10 @@ EXPORTED = (
     "sgpu_frame3d_run",
     "sgpu_frame3d_effective_band",
     "sgpu_band_solve",
+    "sgpu_frame3d_set_recorder",
+    "sgpu_frame3d_recorded",
+    "sgpu_frame3d_envelopes",
+    "sgpu_frame2d_envelopes",
 )
 
 # Kernel ids of sgpu_profile_read.

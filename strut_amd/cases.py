@@ -524,6 +524,16 @@ def rc_frame3d_case(stories: int, bays_x: int, bays_y: int, steps: int, dt: floa
                        "nodes": [masters[stories] if diaphragms else node_at[(0, 0, stories)]],
                        "dofs": [1, 2, 6], "output": "roof"}],
     }
+    # inter-story drift of the corner column line in x and y, and the floor
+    # envelopes (the retained nodes with diaphragms)
+    line = [node_at[(0, 0, k)] for k in range(stories + 1)]
+    for k in range(1, stories + 1):
+        for d in (1, 2):
+            case["recorders"].append({"type": "drift", "i_node": line[k - 1], "j_node": line[k], "dof": d,
+                                      "perp_dirn": 3, "output": f"drift_{k}_{d}"})
+    case["recorders"].append({"type": "envelope_node_displacement",
+                              "nodes": [masters[k] for k in range(1, stories + 1)] if diaphragms else line[1:],
+                              "dofs": [1, 2, 6], "output": "floor_envelope"})
     if mps:
         case["mp_constraints"] = mps
     return case

@@ -828,3 +828,29 @@ int sgpu_frame2d_inst_stats(sgpu_ctx *ctx, int32_t instance, int32_t *out4) {
 }
 
 }  // extern "C"
+
+namespace sgpu {
+int envelopes_device(sgpu_ctx *ctx, const double *d_hist, const std::vector<int64_t> &bases,
+                     const std::vector<int32_t> &steps, int n_record, int n_drift, const int32_t *col_i,
+                     const int32_t *col_j, const double *dx, double *disp_env, double *drift_env);
+}
+
+int sgpu_frame2d_envelopes(sgpu_ctx *ctx, int32_t frame_id, int n_drift, const int32_t *col_i, const int32_t *col_j,
+                           const double *dx, double *disp_env, double *drift_env) {
+    if (!ctx || !disp_env || (n_drift > 0 && !drift_env)) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (frame_id < 0 || frame_id >= (int)fh.frames.size()) return fail(ctx, SGPU_ERR_ARG, "frame out of range");
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<InstState> st(fh.insts.size());
+    HIP_OK(ctx, hipMemcpy(st.data(), fh.d_state, sizeof(InstState) * st.size(), hipMemcpyDeviceToHost));
+    std::vector<int64_t> bases;
+    std::vector<int32_t> steps;
+    for (size_t i = 0; i < fh.insts.size(); ++i) {
+        if (fh.insts[i].frame != frame_id) continue;
+        bases.push_back(fh.hist_base[i]);
+        steps.push_back(std::min(st[i].step, (int32_t)fh.insts[i].n_steps));
+    }
+    return envelopes_device(ctx, fh.d_hist, bases, steps, fh.frames[frame_id].dev.n_record, n_drift, col_i, col_j, dx,
+                            disp_env, drift_env);
+}

@@ -70,6 +70,10 @@ struct Frame3dHost {
     int32_t *d_status = nullptr;
     double *d_band = nullptr;
     double *d_scalar = nullptr;  // [0] residual norm^2, [1] max element status, [2] solve status
+    // recorder: chosen DOFs after every converged step, kept on the device
+    int n_rec = 0, rec_cap = 0;
+    int32_t *d_rec_dofs = nullptr;
+    double *d_rec = nullptr;
 };
 
 // ------------------------------------------------------------------ kernels
@@ -776,6 +780,11 @@ __global__ __launch_bounds__(NT) void f3_solve_blk_kernel(int n, int hbw, double
     if (tid == 0) status[0] = s_flag ? 1.0 : 0.0;
 }
 
+__global__ void f3_record_kernel(int n_rec, const int32_t *dofs, const double *u, double *row) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_rec) row[c] = u[dofs[c]];
+}
+
 __global__ void f3_update_kernel(int n, double *uf, const double *du) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) uf[i] += du[i];
@@ -1222,6 +1231,9 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
         if (u_hist)
             HIP_OK(ctx, hipMemcpyAsync(u_hist + (size_t)step * h.total, h.d_u, sizeof(double) * h.total,
                                        hipMemcpyDeviceToHost, st));
+        if (h.n_rec > 0 && h.steps_done < h.rec_cap)
+            hipLaunchKernelGGL(f3_record_kernel, grid_of(h.n_rec), dim3(256), 0, st, h.n_rec, h.d_rec_dofs, h.d_u,
+                               h.d_rec + (size_t)h.steps_done * h.n_rec);
         h.steps_done++;
         (*steps_done)++;
     }
@@ -1262,4 +1274,46 @@ int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band_out) {
     HIP_OK(ctx, hipMemcpyAsync(band_out, h.d_band, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
     HIP_OK(ctx, hipStreamSynchronize(st));
     return SGPU_OK;
+}
+
+namespace sgpu {
+int envelopes_device(sgpu_ctx *ctx, const double *d_hist, const std::vector<int64_t> &bases,
+                     const std::vector<int32_t> &steps, int n_record, int n_drift, const int32_t *col_i,
+                     const int32_t *col_j, const double *dx, double *disp_env, double *drift_env);
+}
+
+int sgpu_frame3d_set_recorder(sgpu_ctx *ctx, int n_record, const int32_t *dofs, int max_steps) {
+    if (!ctx || n_record <= 0 || !dofs || max_steps <= 0) return SGPU_ERR_ARG;
+    Frame3dHost *h = ctx->frame3d;
+    if (!h) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
+    if (h->n_rec > 0) return fail(ctx, SGPU_ERR_STATE, "the recorder is already set");
+    for (int c = 0; c < n_record; ++c)
+        if (dofs[c] < 0 || dofs[c] >= h->total) return fail(ctx, SGPU_ERR_ARG, "recorded DOF %d out of range", c);
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = upload(ctx, &h->d_rec_dofs, std::vector<int32_t>(dofs, dofs + n_record)))) return rc;
+    if ((rc = dalloc(ctx, &h->d_rec, (size_t)n_record * max_steps))) return rc;
+    h->n_rec = n_record;
+    h->rec_cap = max_steps;
+    return SGPU_OK;
+}
+
+int sgpu_frame3d_recorded(sgpu_ctx *ctx, double *hist, int32_t *steps) {
+    if (!ctx || !hist || !steps) return SGPU_ERR_ARG;
+    Frame3dHost *h = ctx->frame3d;
+    if (!h || h->n_rec == 0) return fail(ctx, SGPU_ERR_STATE, "no recorder is set");
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    *steps = std::min(h->steps_done, h->rec_cap);
+    HIP_OK(ctx, hipMemcpy(hist, h->d_rec, sizeof(double) * (size_t)*steps * h->n_rec, hipMemcpyDeviceToHost));
+    return SGPU_OK;
+}
+
+int sgpu_frame3d_envelopes(sgpu_ctx *ctx, int n_drift, const int32_t *col_i, const int32_t *col_j, const double *dx,
+                           double *disp_env, double *drift_env) {
+    if (!ctx || !disp_env || (n_drift > 0 && !drift_env)) return SGPU_ERR_ARG;
+    Frame3dHost *h = ctx->frame3d;
+    if (!h || h->n_rec == 0) return fail(ctx, SGPU_ERR_STATE, "no recorder is set");
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    return envelopes_device(ctx, h->d_rec, {0}, {std::min(h->steps_done, h->rec_cap)}, h->n_rec, n_drift, col_i,
+                            col_j, dx, disp_env, drift_env);
 }

@@ -74,6 +74,8 @@ class Frame3dCase:
     beta_k_init: float
     patterns: List[Pattern] = field(default_factory=list)
     record_dofs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    envelope_dofs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
+    drift: List[tuple] = field(default_factory=list)  # (i_dof, j_dof, dx) per drift recorder
 
     @property
     def total_dofs(self) -> int:
@@ -163,12 +165,21 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
                                 float(series["dt"]), float(series.get("factor", 1.0))))
     if len(patterns) > 3:
         raise ValueError("at most three excitation patterns")
-    rec = []
+    rec, env, drift = [], [], []
     for r in case.get("recorders", []):
-        if r["type"] == "node_displacement":
+        if r["type"] in ("node_displacement", "envelope_node_displacement"):
             for nid in r["nodes"]:
                 for d in r["dofs"]:
-                    rec.append(NDF * idx[int(nid)] + int(d) - 1)
+                    (rec if r["type"] == "node_displacement" else env).append(NDF * idx[int(nid)] + int(d) - 1)
+        elif r["type"] == "drift":  # helpers.mojo:3113-3142
+            i, j = idx[int(r["i_node"])], idx[int(r["j_node"])]
+            d, perp = int(r["dof"]), int(r["perp_dirn"])
+            if not 1 <= d <= NDF or not 1 <= perp <= 3:
+                raise ValueError("drift recorder dof / perp_dirn out of range")
+            dx = float(coords[j, perp - 1] - coords[i, perp - 1])
+            if dx == 0.0:
+                raise ValueError("drift denominator is zero")
+            drift.append((NDF * i + d - 1, NDF * j + d - 1, dx))
     return Frame3dCase(
         n_nodes=n_nodes, coords=coords, sp=sp, mass=mass, materials=mats, section_ids=used, section_cells=cells,
         section_gj=gj, elem_nodes=elem_nodes, elem_kind=elem_kind, elem_section=elem_section,
@@ -178,4 +189,22 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
         gamma=float(integ.get("gamma", 0.5)), beta=float(integ.get("beta", 0.25)), tol=float(an.get("tol", 1.0e-10)),
         max_iters=int(an.get("max_iters", 20)), alpha_m=float(ray.get("alphaM", 0.0)),
         beta_k=float(ray.get("betaK", 0.0)), beta_k_init=float(ray.get("betaKInit", 0.0)), patterns=patterns,
-        record_dofs=np.array(rec, dtype=np.int32))
+        record_dofs=np.array(rec, dtype=np.int32), envelope_dofs=np.array(env, dtype=np.int32), drift=drift)
+
+
+def recorder_plan(fc: Frame3dCase):
+    """The device recorder of a case's envelope and drift recorders: the DOFs
+    to record (envelope DOFs first, then the drift recorders' DOFs not
+    already there) and each drift channel as (column i, column j, dx) over
+    them."""
+    cols = [int(d) for d in fc.envelope_dofs]
+    col_i, col_j, dx = [], [], []
+    for i_dof, j_dof, d in fc.drift:
+        for dof in (i_dof, j_dof):
+            if dof not in cols:
+                cols.append(dof)
+        col_i.append(cols.index(i_dof))
+        col_j.append(cols.index(j_dof))
+        dx.append(d)
+    return np.array(cols, dtype=np.int32), np.array(col_i, dtype=np.int32), np.array(col_j, dtype=np.int32), \
+        np.array(dx, dtype=np.float64)

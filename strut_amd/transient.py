@@ -64,6 +64,8 @@ def _declare(lib):
         "sgpu_frame2d_set_predictor": (ctypes.c_int, [c_void_p, c_int32, c_void_p]),
         "sgpu_frame2d_inst_stats": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_int32)]),
         "sgpu_frame2d_launch_queue": (ctypes.c_int, [c_void_p, c_int32]),
+        "sgpu_frame2d_envelopes": (ctypes.c_int, [c_void_p, c_int32, ctypes.c_int, POINTER(c_int32), POINTER(c_int32),
+                                                  POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -184,6 +186,22 @@ class FrameBatch:
         self._check(self.lib.sgpu_frame2d_result(self.rt.ctx, inst, dptr(u), iptr(it), ctypes.byref(st),
                                                  ctypes.byref(done)))
         return {"u": u, "iters": it, "status": st.value, "steps_done": done.value}
+
+    def envelopes(self, frame_id: int, col_i=(), col_j=(), dx=()):
+        """Envelopes of every instance of a frame over its completed steps,
+        computed on the device from the recorded history: (disp_env
+        [instances, n_record, 3], drift_env [instances, n_drift, 3]) with
+        min, max and largest magnitude; drift channel d is
+        (u[col_j[d]] - u[col_i[d]]) / dx[d] over recorded columns."""
+        n_inst = sum(1 for f, _ in self.insts if f == frame_id)
+        ci = np.ascontiguousarray(col_i, dtype=np.int32)
+        cj = np.ascontiguousarray(col_j, dtype=np.int32)
+        d = np.ascontiguousarray(dx, dtype=np.float64)
+        disp = np.zeros((n_inst, self.n_record[frame_id], 3))
+        drift = np.zeros((n_inst, max(len(ci), 1), 3))
+        self._check(self.lib.sgpu_frame2d_envelopes(self.rt.ctx, frame_id, len(ci), iptr(ci), iptr(cj), dptr(d),
+                                                    dptr(disp), dptr(drift)))
+        return disp, drift[:, :len(ci)]
 
     def info(self, frame_id: int) -> dict:
         out = np.zeros(4, dtype=np.int32)

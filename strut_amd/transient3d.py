@@ -52,6 +52,13 @@ def _declare(lib):
     lib.sgpu_band_solve.restype = c_int
     lib.sgpu_band_solve.argtypes = [c_void_p, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double),
                                     POINTER(c_double), POINTER(c_int32)]
+    lib.sgpu_frame3d_set_recorder.restype = c_int
+    lib.sgpu_frame3d_set_recorder.argtypes = [c_void_p, c_int, POINTER(c_int32), c_int]
+    lib.sgpu_frame3d_recorded.restype = c_int
+    lib.sgpu_frame3d_recorded.argtypes = [c_void_p, POINTER(c_double), POINTER(c_int32)]
+    lib.sgpu_frame3d_envelopes.restype = c_int
+    lib.sgpu_frame3d_envelopes.argtypes = [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_double),
+                                           POINTER(c_double), POINTER(c_double)]
     lib._f3_declared = True
 
 
@@ -122,6 +129,31 @@ class Frame3dRun:
         band = np.zeros((self.free_count, 2 * self.hbw + 1))
         self.rt._check(self.rt.lib.sgpu_frame3d_effective_band(self.rt.ctx, dptr(band)))
         return band
+
+    def set_recorder(self, dofs, max_steps: int) -> None:
+        """Record these full-DOF displacements after every converged step,
+        on the device."""
+        self._rec = np.ascontiguousarray(dofs, dtype=np.int32)
+        self._rec_cap = max_steps
+        self.rt._check(self.rt.lib.sgpu_frame3d_set_recorder(self.rt.ctx, len(self._rec), iptr(self._rec), max_steps))
+
+    def recorded(self) -> np.ndarray:
+        hist = np.zeros((self._rec_cap, len(self._rec)))
+        steps = c_int32()
+        self.rt._check(self.rt.lib.sgpu_frame3d_recorded(self.rt.ctx, dptr(hist), ctypes.byref(steps)))
+        return hist[:steps.value]
+
+    def envelopes(self, col_i=(), col_j=(), dx=()):
+        """(disp_env (n_record, 3), drift_env (n_drift, 3)): min, max and
+        largest magnitude over the recorded steps, computed on the device."""
+        ci = np.ascontiguousarray(col_i, dtype=np.int32)
+        cj = np.ascontiguousarray(col_j, dtype=np.int32)
+        d = np.ascontiguousarray(dx, dtype=np.float64)
+        disp = np.zeros((len(self._rec), 3))
+        drift = np.zeros((max(len(ci), 1), 3))
+        self.rt._check(self.rt.lib.sgpu_frame3d_envelopes(self.rt.ctx, len(ci), iptr(ci), iptr(cj), dptr(d),
+                                                          dptr(disp), dptr(drift)))
+        return disp, drift[:len(ci)]
 
     def close(self) -> None:
         self.rt.close()
