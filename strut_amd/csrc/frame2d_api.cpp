@@ -661,6 +661,7 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
     bool tune = fh.schedule == SGPU_SCHEDULE_AUTO;
     int active_at_tune = active;
     int since_tune = 0;
+    double tuned_rate = 0.0;  // the chosen schedule's rate when it was chosen
     int64_t chunks = 0;
     const int64_t max_chunks = 4 + 2 * fh.max_rounds / rounds;
     while (active > 0) {
@@ -685,16 +686,22 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
             fh.rate_rounds = rate[0];
             fh.rate_queue = rate[1];
             fh.chosen = rate[1] > rate[0] ? SGPU_SCHEDULE_QUEUE : SGPU_SCHEDULE_ROUNDS;
+            tuned_rate = rate[1] > rate[0] ? rate[1] : rate[0];
             fh.tunings += 1;
             tune = false;
             active_at_tune = active;
             continue;
         }
         if ((rc = run_chunk(ctx, fh, fh.chosen, rounds, st, adv, active, sec, 0.0))) return rc;
-        // measure again when the active set has halved, and every 64 chunks
-        // (the cost of an instance's step changes as its records go nonlinear)
+        // measure again when the active set has halved, every 64 chunks, and
+        // when the chosen schedule's rate has fallen well below the rate it
+        // was chosen at: the cost of an instance's step changes as its record
+        // goes nonlinear, and lockstep rounds then wait for every round's
+        // slowest element task while the queue keeps its workers busy
+        const double chunk_rate = sec > 0.0 ? (double)adv / sec : 0.0;
+        ++since_tune;
         if (fh.schedule == SGPU_SCHEDULE_AUTO && active > 0 &&
-            (2 * active <= active_at_tune || ++since_tune >= 64)) {
+            (2 * active <= active_at_tune || since_tune >= 64 || (since_tune >= 2 && chunk_rate < 0.6 * tuned_rate))) {
             tune = true;
             since_tune = 0;
         }
