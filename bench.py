@@ -455,7 +455,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gm-batch", type=int, default=256, help="records in the batched GM-steps leg (0 = skip)")
     ap.add_argument("--gm-steps", type=int, default=1000)
-    ap.add_argument("--ens-motions", type=int, default=32, help="motions per frame in the ensemble leg (0 = skip)")
+    ap.add_argument("--ens-motions", type=int, default=256,
+                    help="motions per frame in the ensemble leg (0 = skip; 256 is the ensemble configuration's count)")
     ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--learned-train", type=int, default=256, help="training records of the learned leg (0 = skip)")
     ap.add_argument("--learned-test", type=int, default=64)
