@@ -124,3 +124,32 @@ def test_gpu_band_solve_kernels_match_the_reference_backend(n, hbw, dominant):
         with pytest.raises(Exception, match="register-window"):
             band_solve(rt, band, rhs, variant=2)
     rt.close()
+
+
+def test_gpu_frame3d_entry_points_refuse_bad_calls():
+    """Host-side refusals of the C ABI (nothing reaches a kernel)."""
+    from strut_amd._native import StrutGpuError
+    from strut_amd.transient3d import Frame3dDesc
+
+    case = cases.rc_frame3d_case(1, 1, 1, steps=4, dt=0.005, n_core=4, n_bars=4)
+    fc = load_frame3d_case(case)
+    run = Frame3dRun(fc)
+    with pytest.raises(StrutGpuError, match="no recorder"):
+        run.envelopes()
+    with pytest.raises(StrutGpuError, match="out of range"):
+        run.set_recorder([fc.total_dofs], 4)
+    # a second frame in the same context
+    import ctypes
+    d = Frame3dDesc()
+    d.n_nodes = 1
+    with pytest.raises(StrutGpuError, match="already holds"):
+        run.rt._check(run.rt.lib.sgpu_frame3d_create(run.rt.ctx, ctypes.byref(d), None, None))
+    run.set_recorder([0, 6], 4)
+    with pytest.raises(StrutGpuError, match="already set"):
+        run.set_recorder([0], 4)
+    with pytest.raises(StrutGpuError, match="out of range"):
+        run.envelopes([0], [5], [3.0])
+    with pytest.raises(StrutGpuError, match="zero denominator"):
+        run.envelopes([0], [1], [0.0])
+    assert run.run(4)["status"] == 0
+    run.close()
