@@ -112,6 +112,8 @@ class Frame3dRun:
                                                        ctypes.byref(nf)))
         self.hbw, self.free_count = hbw.value, nf.value
         self.steps_done = 0
+        self._rec = np.zeros(0, dtype=np.int32)
+        self._rec_cap = 0
 
     def run(self, steps: int, record_history: bool = True) -> dict:
         u = np.zeros((steps, self.fc.total_dofs)) if record_history else None
@@ -149,11 +151,11 @@ class Frame3dRun:
         ci = np.ascontiguousarray(col_i, dtype=np.int32)
         cj = np.ascontiguousarray(col_j, dtype=np.int32)
         d = np.ascontiguousarray(dx, dtype=np.float64)
-        disp = np.zeros((len(self._rec), 3))
+        disp = np.zeros((max(len(self._rec), 1), 3))
         drift = np.zeros((max(len(ci), 1), 3))
         self.rt._check(self.rt.lib.sgpu_frame3d_envelopes(self.rt.ctx, len(ci), iptr(ci), iptr(cj), dptr(d),
                                                           dptr(disp), dptr(drift)))
-        return disp, drift[:len(ci)]
+        return disp[:len(self._rec)], drift[:len(ci)]
 
     def close(self) -> None:
         self.rt.close()
