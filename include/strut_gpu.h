@@ -332,11 +332,32 @@ int sgpu_frame3d_recorded(sgpu_ctx *ctx, double *hist, int32_t *steps);
 int sgpu_frame3d_envelopes(sgpu_ctx *ctx, int n_drift, const int32_t *col_i, const int32_t *col_j, const double *dx,
                            double *disp_env, double *drift_env);
 
+/* Another record (ground-motion instance) of the same frame, before the
+ * first run and before the recorder: its own elements (sgpu_elem3d_add, so
+ * their own section states; element i of this list plays element i of the
+ * frame) and its own series for each of the frame's patterns.  The frame's
+ * description is record 0.  Records run together: every kernel of a Newton
+ * iteration covers all records still iterating, one band solve workgroup
+ * per record. */
+int sgpu_frame3d_add_record(sgpu_ctx *ctx, const int32_t *elem_ids, const double *const *ts_values,
+                            const int32_t *ts_n, const double *ts_dt, const double *ts_factor, int32_t *record);
+/* `steps` more steps of every running record, in lockstep by step, each
+ * record converging, stopping or failing on its own: iters[r * steps +
+ * step], steps_done[r], status[r] (as sgpu_frame3d_run), dense_fallbacks[r]
+ * (may be NULL).  sgpu_frame3d_run is the one-record form. */
+int sgpu_frame3d_run_records(sgpu_ctx *ctx, int steps, int32_t *iters, int32_t *steps_done, int32_t *status,
+                             int32_t *dense_fallbacks);
+/* The recorder's rows and envelopes of one record (the forms without a
+ * record read record 0). */
+int sgpu_frame3d_recorded_record(sgpu_ctx *ctx, int32_t record, double *hist, int32_t *steps);
+int sgpu_frame3d_envelopes_record(sgpu_ctx *ctx, int32_t record, int n_drift, const int32_t *col_i,
+                                  const int32_t *col_j, const double *dx, double *disp_env, double *drift_env);
+
 /* The effective tangent K + a1 C + a0 M in the band (n_free x (2 hbw + 1),
  * row major, slot (i, j) at i (2 hbw + 1) + j - i + hbw), as the reference
  * loads it before each factorization (transient_nonlinear.mojo:2400-2411),
- * with the elements evaluated at the current displacement (a trial state;
- * before the first step: at rest). */
+ * with record 0's elements evaluated at its current displacement (a trial
+ * state; before the first step: at rest). */
 int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band);
 
 /* ------------------------------------------------------------------

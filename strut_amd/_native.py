@@ -95,6 +95,10 @@ EXPORTED = (
     "sgpu_frame3d_recorded",
     "sgpu_frame3d_envelopes",
     "sgpu_frame2d_envelopes",
+    "sgpu_frame3d_add_record",
+    "sgpu_frame3d_run_records",
+    "sgpu_frame3d_recorded_record",
+    "sgpu_frame3d_envelopes_record",
 )
 
 # Kernel ids of sgpu_profile_read.

@@ -47,14 +47,21 @@ struct Frame3dHost {
     int n_nodes = 0, total = 0, n = 0, hbw = 0, W = 0, n_el = 0;
     double dt = 0, gamma = 0, beta = 0, tol = 0, alpha_m = 0, beta_k = 0, beta_k_init = 0;
     int max_iters = 0, n_patterns = 0;
-    std::vector<double> ts[SGPU_F3_MAX_PATTERNS];
-    double ts_dt[SGPU_F3_MAX_PATTERNS] = {}, ts_factor[SGPU_F3_MAX_PATTERNS] = {};
-    int steps_done = 0;
-    bool initialized = false;
+    // records: ground-motion instances of the frame, each with its own
+    // elements (section states) and series; per-record device arrays hold
+    // R slices
+    int R = 0;
+    std::vector<int32_t> rec_elem_ids;                 // R x n_el
+    std::vector<std::vector<double>> rec_ts;           // R x n_patterns series
+    std::vector<double> rec_ts_dt, rec_ts_factor;      // R x n_patterns
+    std::vector<int32_t> rec_steps, rec_status;        // completed steps; 0 running, > 0 stopped, < 0 failed
+    bool allocated = false, initialized = false;
+    int32_t *d_list = nullptr, *d_list_ids = nullptr, *d_row_of = nullptr;
+    double *d_ag = nullptr;  // R x 3 ground accelerations of the step
     // device tables
     int32_t *d_free_idx = nullptr, *d_row_offsets = nullptr, *d_dof_pool = nullptr, *d_slave = nullptr;
     double *d_coeff_pool = nullptr;
-    int32_t *d_elem_nodes = nullptr, *d_elem_ids = nullptr, *d_ret = nullptr, *d_ret_n = nullptr;
+    int32_t *d_elem_nodes = nullptr, *d_ret = nullptr, *d_ret_n = nullptr;
     double *d_T = nullptr;                                 // n_el x 12 x 12: T_e[a][a']
     int32_t *d_band_off = nullptr, *d_band_src = nullptr;  // per band slot: k_hat entries
     double *d_mass_band = nullptr, *d_kinit_band = nullptr;
@@ -78,72 +85,87 @@ struct Frame3dHost {
 
 // ------------------------------------------------------------------ kernels
 
+// Every kernel below works on a list of records (ground-motion instances
+// of the same frame): record r = recs[blockIdx.y] (or recs[blockIdx.x] for
+// the one-workgroup-per-record kernels) owns the r-th slice of each
+// per-record array.
+
 // :553-590 predictor; the step's base state kept in un/vn/an
-__global__ void f3_predict_kernel(int n, double *uf, const double *vf, const double *af, double *un, double *vn,
-                                  double *an, double dt, double beta) {
+__global__ void f3_predict_kernel(const int32_t *recs, int n, double *uf, const double *vf, const double *af,
+                                  double *un, double *vn, double *an, double dt, double beta) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const size_t o = (size_t)recs[blockIdx.y] * n + i;
     const double coeff = dt * dt * (0.5 - beta);
-    const double u0 = uf[i], v0 = vf[i], a0 = af[i];
-    un[i] = u0;
-    vn[i] = v0;
-    an[i] = a0;
-    uf[i] = u0 + dt * v0 + coeff * a0;
+    const double u0 = uf[o], v0 = vf[o], a0 = af[o];
+    un[o] = u0;
+    vn[o] = v0;
+    an[o] = a0;
+    uf[o] = u0 + dt * v0 + coeff * a0;
 }
 
 // scatter of the free vector and the slave update (_enforce_mpc_values,
 // helpers.mojo:3380-3397; retained single-point-fixed DOFs hold 0)
-__global__ void f3_expand_kernel(int total, double *u, const double *uf, const int32_t *free_idx,
-                                 const int32_t *row_offsets, const int32_t *dof_pool, const double *coeff_pool,
-                                 const int32_t *slave) {
+__global__ void f3_expand_kernel(const int32_t *recs, int total, int n, double *u, const double *uf,
+                                 const int32_t *free_idx, const int32_t *row_offsets, const int32_t *dof_pool,
+                                 const double *coeff_pool, const int32_t *slave) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= total) return;
+    const int r = recs[blockIdx.y];
+    uf += (size_t)r * n;
     const int fi = free_idx[d];
     double v = 0.0;
     if (fi >= 0) {
         v = uf[fi];
     } else if (slave[d]) {
         for (int t = row_offsets[d]; t < row_offsets[d + 1]; ++t) {
-            const int r = free_idx[dof_pool[t]];
-            v += coeff_pool[t] * (r >= 0 ? uf[r] : 0.0);
+            const int q = free_idx[dof_pool[t]];
+            v += coeff_pool[t] * (q >= 0 ? uf[q] : 0.0);
         }
     }
-    u[d] = v;
+    u[(size_t)r * total + d] = v;
 }
 
-__global__ void f3_gather_u12_kernel(int n_el, const int32_t *elem_nodes, const double *u, double *u12) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_el * 12) return;
-    const int e = t / 12, a = t % 12;
+// element end displacements in list order: position b = list index * n_el + e
+__global__ void f3_gather_u12_kernel(const int32_t *recs, int n_list, int n_el, int total, const int32_t *elem_nodes,
+                                     const double *u, double *u12) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_list * n_el * 12) return;
+    const int64_t b = t / 12;
+    const int a = (int)(t % 12), e = (int)(b % n_el), r = recs[b / n_el];
     const int node = elem_nodes[2 * e + (a >= 6)];
-    u12[t] = u[6 * node + (a % 6)];
+    u12[t] = u[(size_t)r * total + 6 * node + (a % 6)];
 }
 
 // :496-531 trial acceleration and velocity
-__global__ void f3_trial_kernel(int n, const double *uf, const double *un, const double *vn, const double *an,
-                                double *at, double *vt, double a0, double a2, double a3, double gamma, double dt) {
+__global__ void f3_trial_kernel(const int32_t *recs, int n, const double *uf, const double *un, const double *vn,
+                                const double *an, double *at, double *vt, double a0, double a2, double a3,
+                                double gamma, double dt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double anext = a0 * (uf[i] - un[i]) - a2 * vn[i] - a3 * an[i];
-    at[i] = anext;
-    vt[i] = vn[i] + dt * ((1.0 - gamma) * an[i] + gamma * anext);
+    const size_t o = (size_t)recs[blockIdx.y] * n + i;
+    const double anext = a0 * (uf[o] - un[o]) - a2 * vn[o] - a3 * an[o];
+    at[o] = anext;
+    vt[o] = vn[o] + dt * ((1.0 - gamma) * an[o] + gamma * anext);
 }
 
 // one 256-thread workgroup per element: k_hat = T^T k T, f_hat = T^T f, and
 // the element-level products k_hat v, k_hat_init v at the trial velocity
-__global__ __launch_bounds__(256) void f3_elem_collapse_kernel(int n_el, const int32_t *ret, const int32_t *ret_n,
-                                                               const double *T, const double *k, const double *f,
-                                                               const double *kinit_hat, const double *vt,
-                                                               double *khat, double *fhat, double *yk, double *yi) {
+__global__ __launch_bounds__(256) void f3_elem_collapse_kernel(const int32_t *recs, int n, int n_el, const int32_t *ret,
+                                                               const int32_t *ret_n, const double *T, const double *k,
+                                                               const double *f, const double *kinit_hat,
+                                                               const double *vt, double *khat, double *fhat, double *yk,
+                                                               double *yi) {
     __shared__ double sT[144], sK[144], sKT[144], sV[12], sKh[144];
-    const int e = blockIdx.x, tid = threadIdx.x;
-    if (e >= n_el) return;
+    const int b = blockIdx.x, tid = threadIdx.x;  // list position: k and f are in list order
+    const int e = b % n_el, r = recs[b / n_el];
+    const size_t re = (size_t)r * n_el + e;  // record layout of the collapsed results
     const int m = ret_n[e];
     if (tid < 144) {
         sT[tid] = T[(size_t)e * 144 + tid];
-        sK[tid] = k[(size_t)e * 144 + tid];
+        sK[tid] = k[(size_t)b * 144 + tid];
     }
-    if (tid < 12) sV[tid] = tid < m ? vt[ret[e * 12 + tid]] : 0.0;
+    if (tid < 12) sV[tid] = tid < m ? vt[(size_t)r * n + ret[e * 12 + tid]] : 0.0;
     __syncthreads();
     if (tid < 144) {  // (k T)[a][b'] = sum_b k[a][b] T[b][b']
         const int a = tid / 12, bp = tid % 12;
@@ -157,31 +179,36 @@ __global__ __launch_bounds__(256) void f3_elem_collapse_kernel(int n_el, const i
         double s = 0.0;
         for (int a = 0; a < 12; ++a) s += sT[a * 12 + ap] * sKT[a * 12 + bp];
         sKh[tid] = s;
-        khat[(size_t)e * 144 + tid] = s;
+        khat[re * 144 + tid] = s;
     }
     __syncthreads();
     if (tid < 12) {
         const int ap = tid;
         double sf = 0.0, sy = 0.0, syi = 0.0;
-        for (int a = 0; a < 12; ++a) sf += sT[a * 12 + ap] * f[(size_t)e * 12 + a];
+        for (int a = 0; a < 12; ++a) sf += sT[a * 12 + ap] * f[(size_t)b * 12 + a];
         for (int bp = 0; bp < 12; ++bp) {
             sy += sKh[ap * 12 + bp] * sV[bp];
-            syi += kinit_hat[(size_t)e * 144 + ap * 12 + bp] * sV[bp];
+            syi += kinit_hat[re * 144 + ap * 12 + bp] * sV[bp];
         }
-        fhat[(size_t)e * 12 + ap] = sf;
-        yk[(size_t)e * 12 + ap] = sy;
-        yi[(size_t)e * 12 + ap] = syi;
+        fhat[re * 12 + ap] = sf;
+        yk[re * 12 + ap] = sy;
+        yi[re * 12 + ap] = syi;
     }
 }
 
 // every band slot: K from its k_hat entries, then (:2302-2335, :2400-2411)
 // band = K; band += a1 C with C = alphaM M + betaK K + betaKInit K_init;
 // band += a0 M.  raw != 0 writes K alone (the initial tangent).
-__global__ void f3_band_kernel(int64_t slots, const int32_t *off, const int32_t *src, const double *khat,
-                               const double *mass_band, const double *kinit_band, double *band, int raw, double a0,
-                               double a1, double alpha_m, double beta_k, double beta_k_init, int has_damping) {
+__global__ void f3_band_kernel(const int32_t *recs, int64_t slots, int64_t khat_stride, const int32_t *off,
+                               const int32_t *src, const double *khat, const double *mass_band,
+                               const double *kinit_band, double *band, int raw, double a0, double a1, double alpha_m,
+                               double beta_k, double beta_k_init, int has_damping) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= slots) return;
+    const int r = recs[blockIdx.y];
+    khat += (size_t)r * khat_stride;
+    kinit_band += (size_t)r * slots;
+    band += (size_t)r * slots;
     double kv = 0.0;
     for (int t = off[s]; t < off[s + 1]; ++t) kv += khat[src[t]];
     if (raw) {
@@ -202,14 +229,21 @@ __global__ void f3_band_kernel(int64_t slots, const int32_t *off, const int32_t 
 }
 
 // :2337-2356 R = P - F_int - C v - M a on the free DOFs
-__global__ void f3_residual_kernel(int n, int n_patterns, const double *load, double ag0, double ag1, double ag2,
-                                   const int32_t *dof_off, const int32_t *dof_src, const double *fhat,
-                                   const double *yk, const double *yi, const int32_t *m_off, const int32_t *m_col,
-                                   const double *m_val, const double *vt, const double *at, double alpha_m,
-                                   double beta_k, double beta_k_init, int has_damping, double *R) {
+__global__ void f3_residual_kernel(const int32_t *recs, int n, int n_el, int n_patterns, const double *load,
+                                   const double *ag_all, const int32_t *dof_off, const int32_t *dof_src,
+                                   const double *fhat, const double *yk, const double *yi, const int32_t *m_off,
+                                   const int32_t *m_col, const double *m_val, const double *vt, const double *at,
+                                   double alpha_m, double beta_k, double beta_k_init, int has_damping, double *R) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double ag[3] = {ag0, ag1, ag2};
+    const int r = recs[blockIdx.y];
+    const double *ag = ag_all + 3 * r;
+    fhat += (size_t)r * n_el * 12;
+    yk += (size_t)r * n_el * 12;
+    yi += (size_t)r * n_el * 12;
+    vt += (size_t)r * n;
+    at += (size_t)r * n;
+    R += (size_t)r * n;
     double p = 0.0;
     for (int q = 0; q < n_patterns; ++q) p += ag[q] * load[(size_t)q * n + i];
     double fi = 0.0, ky = 0.0, kiy = 0.0;
@@ -230,10 +264,14 @@ __global__ void f3_residual_kernel(int n, int n_patterns, const double *load, do
 }
 
 // one workgroup: sum of R^2 and the largest element status
-__global__ __launch_bounds__(1024) void f3_norm_kernel(int n, const double *R, int n_el, const int32_t *status,
-                                                       double *out) {
+__global__ __launch_bounds__(1024) void f3_norm_kernel(const int32_t *recs, int n, const double *R, int n_el,
+                                                       const int32_t *status, double *out) {
     __shared__ double s_sum[1024];
     __shared__ int s_st[1024];
+    const int r = recs[blockIdx.x];
+    R += (size_t)r * n;
+    status += (size_t)blockIdx.x * n_el;  // element status in list order
+    out += 4 * r;
     double s = 0.0;
     int st = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s += R[i] * R[i];
@@ -262,8 +300,15 @@ __global__ __launch_bounds__(1024) void f3_norm_kernel(int n, const double *R, i
 // the band elimination continues and gives the same solution within
 // rounding, and the solve reports status 1 (the reference's fallback
 // counted).  A pivot at or below 1e-14: status 2, nothing solved.
-__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_kernel(int n, int hbw, double *band, const double *rhs,
-                                                                   double *x, double *status) {
+__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_kernel(const int32_t *recs, int n, int hbw, double *band,
+                                                                   const double *rhs, double *x, double *status) {
+    if (recs) {  // one workgroup per listed record
+        const int r_ = recs[blockIdx.x];
+        band += (size_t)r_ * n * (2 * hbw + 1);
+        rhs += (size_t)r_ * n;
+        x += (size_t)r_ * n;
+        status += 4 * r_;
+    }
     extern __shared__ double s_mem[];
     double *s_y = s_mem;             // n
     double *s_l = s_mem + n;         // hbw multipliers of the column
@@ -342,8 +387,16 @@ __global__ __launch_bounds__(kF3SolveThreads) void f3_solve_kernel(int n, int hb
 constexpr int kF3RegMaxP = 128;
 constexpr int kF3RegEntries = kF3RegMaxP * kF3RegMaxP / kF3SolveThreads;  // 16
 
-__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_reg_kernel(int n, int hbw, double *band,
-                                                                       const double *rhs, double *x, double *status) {
+__global__ __launch_bounds__(kF3SolveThreads) void f3_solve_reg_kernel(const int32_t *recs, int n, int hbw,
+                                                                       double *band, const double *rhs, double *x,
+                                                                       double *status) {
+    if (recs) {  // one workgroup per listed record
+        const int r_ = recs[blockIdx.x];
+        band += (size_t)r_ * n * (2 * hbw + 1);
+        rhs += (size_t)r_ * n;
+        x += (size_t)r_ * n;
+        status += 4 * r_;
+    }
     extern __shared__ double s_y[];  // n
     __shared__ double s_col[kF3RegMaxP], s_l[2][kF3RegMaxP], s_u[2][kF3RegMaxP];
     __shared__ double s_diag;
@@ -506,8 +559,15 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 }
 
 template <int RB, int NT>
-__global__ __launch_bounds__(NT) void f3_solve_blk_kernel(int n, int hbw, double *band, const double *rhs, double *x,
-                                                       double *status) {
+__global__ __launch_bounds__(NT) void f3_solve_blk_kernel(const int32_t *recs, int n, int hbw, double *band,
+                                                       const double *rhs, double *x, double *status) {
+    if (recs) {  // one workgroup per listed record
+        const int r_ = recs[blockIdx.x];
+        band += (size_t)r_ * n * (2 * hbw + 1);
+        rhs += (size_t)r_ * n;
+        x += (size_t)r_ * n;
+        status += 4 * r_;
+    }
     constexpr int kWorkers = NT - 64;  // all but the last wavefront
     constexpr int kPf = (kF3BlkD * 2 * kF3BlkMaxP + kWorkers - 1) / kWorkers;
     extern __shared__ double s_dyn[];
@@ -780,24 +840,32 @@ __global__ __launch_bounds__(NT) void f3_solve_blk_kernel(int n, int hbw, double
     if (tid == 0) status[0] = s_flag ? 1.0 : 0.0;
 }
 
-__global__ void f3_record_kernel(int n_rec, const int32_t *dofs, const double *u, double *row) {
+// recorded DOFs after a converged step: record r's row `row` of its history
+__global__ void f3_record_kernel(const int32_t *recs, int n_rec, int total, const int32_t *dofs, const double *u,
+                                 double *hist, int64_t rec_stride, const int32_t *row_of) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n_rec) row[c] = u[dofs[c]];
+    if (c >= n_rec) return;
+    const int r = recs[blockIdx.y];
+    hist[(size_t)r * rec_stride + (size_t)row_of[r] * n_rec + c] = u[(size_t)r * total + dofs[c]];
 }
 
-__global__ void f3_update_kernel(int n, double *uf, const double *du) {
+__global__ void f3_update_kernel(const int32_t *recs, int n, double *uf, const double *du) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) uf[i] += du[i];
+    if (i >= n) return;
+    const size_t o = (size_t)recs[blockIdx.y] * n + i;
+    uf[o] += du[o];
 }
 
 // :594-647 converged step: a and v from the final displacement
-__global__ void f3_newmark_kernel(int n, const double *uf, const double *un, const double *vn, const double *an,
-                                  double *af, double *vf, double a0, double a2, double a3, double gamma, double dt) {
+__global__ void f3_newmark_kernel(const int32_t *recs, int n, const double *uf, const double *un, const double *vn,
+                                  const double *an, double *af, double *vf, double a0, double a2, double a3,
+                                  double gamma, double dt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double anext = a0 * (uf[i] - un[i]) - a2 * vn[i] - a3 * an[i];
-    af[i] = anext;
-    vf[i] = vn[i] + dt * ((1.0 - gamma) * an[i] + gamma * anext);
+    const size_t o = (size_t)recs[blockIdx.y] * n + i;
+    const double anext = a0 * (uf[o] - un[o]) - a2 * vn[o] - a3 * an[o];
+    af[o] = anext;
+    vf[o] = vn[o] + dt * ((1.0 - gamma) * an[o] + gamma * anext);
 }
 
 void frame3d_host_free(Frame3dHost *h) { delete h; }  // device buffers live in ctx->allocations
@@ -819,8 +887,10 @@ double path_at(const std::vector<double> &v, double dt, double factor, double t)
 
 // Band factor + solve on the stream: the register-window kernel when the
 // half bandwidth fits it (variant 0 or 2), else the global-memory one.
+// recs == nullptr: one system at the given pointers; else `count` systems,
+// record r's band, right-hand side, solution and status at its slices.
 int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
-                      double *d_status) {
+                      double *d_status, const int32_t *recs = nullptr, int count = 1) {
     const size_t blk_lds = sizeof(double) * ((size_t)kF3BlkRing + n);
     const int P = hbw + 1, Bc = (P + 3) / 4;
     const bool fits8 = ((P + 7) / 8) * Bc <= 512 - 64, fits4 = ((P + 3) / 4) * Bc <= 1024 - 64;
@@ -832,11 +902,11 @@ int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band
             return fail(ctx, SGPU_ERR_ARG, "band %d x %d exceeds the blocked solve (half bandwidth <= %d, LDS)", n,
                         2 * hbw + 1, variant == 4 ? 115 : kF3BlkMaxP - 1);
         if (variant == 4)
-            hipLaunchKernelGGL((f3_solve_blk_kernel<8, 512>), dim3(1), dim3(512), blk_lds, ctx->stream, n, hbw, d_band,
-                               d_rhs, d_x, d_status);
+            hipLaunchKernelGGL((f3_solve_blk_kernel<8, 512>), dim3(count), dim3(512), blk_lds, ctx->stream, recs, n,
+                               hbw, d_band, d_rhs, d_x, d_status);
         else
-            hipLaunchKernelGGL((f3_solve_blk_kernel<4, 1024>), dim3(1), dim3(1024), blk_lds, ctx->stream, n, hbw,
-                               d_band, d_rhs, d_x, d_status);
+            hipLaunchKernelGGL((f3_solve_blk_kernel<4, 1024>), dim3(count), dim3(1024), blk_lds, ctx->stream, recs, n,
+                               hbw, d_band, d_rhs, d_x, d_status);
         HIP_OK(ctx, hipGetLastError());
         return SGPU_OK;
     }
@@ -847,11 +917,11 @@ int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band
     const size_t lds = sizeof(double) * (size_t)(reg ? n : n + 2 * hbw);
     if (lds > 56 * 1024) return fail(ctx, SGPU_ERR_ARG, "band of %d rows: the solve keeps them in LDS", n);
     if (reg)
-        hipLaunchKernelGGL(f3_solve_reg_kernel, dim3(1), dim3(kF3SolveThreads), lds, ctx->stream, n, hbw, d_band,
-                           d_rhs, d_x, d_status);
+        hipLaunchKernelGGL(f3_solve_reg_kernel, dim3(count), dim3(kF3SolveThreads), lds, ctx->stream, recs, n, hbw,
+                           d_band, d_rhs, d_x, d_status);
     else
-        hipLaunchKernelGGL(f3_solve_kernel, dim3(1), dim3(kF3SolveThreads), lds, ctx->stream, n, hbw, d_band, d_rhs,
-                           d_x, d_status);
+        hipLaunchKernelGGL(f3_solve_kernel, dim3(count), dim3(kF3SolveThreads), lds, ctx->stream, recs, n, hbw, d_band,
+                           d_rhs, d_x, d_status);
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;
 }
@@ -1077,76 +1147,144 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
     h->beta_k_init = d->beta_k_init;
     h->max_iters = d->max_iters;
     h->n_patterns = d->n_patterns;
-    for (int p = 0; p < d->n_patterns; ++p) {
-        h->ts[p].assign(d->ts_values[p], d->ts_values[p] + d->ts_n[p]);
-        h->ts_dt[p] = d->ts_dt[p];
-        h->ts_factor[p] = d->ts_factor[p];
-    }
     std::vector<int32_t> slave(total), rov(ro, ro + total + 1), dpv(dp, dp + ro[total]), en(d->elem_nodes,
-                                                                                           d->elem_nodes + 2 * n_el),
-        ids(d->elem_ids, d->elem_ids + n_el);
+                                                                                           d->elem_nodes + 2 * n_el);
     std::vector<double> cpv(cp, cp + ro[total]);
     for (int i = 0; i < total; ++i) slave[i] = d->has_mpc && d->slave[i] ? 1 : 0;
     int rc;
     if ((rc = upload(ctx, &h->d_free_idx, free_idx)) || (rc = upload(ctx, &h->d_row_offsets, rov)) ||
         (rc = upload(ctx, &h->d_dof_pool, dpv)) || (rc = upload(ctx, &h->d_coeff_pool, cpv)) ||
         (rc = upload(ctx, &h->d_slave, slave)) || (rc = upload(ctx, &h->d_elem_nodes, en)) ||
-        (rc = upload(ctx, &h->d_elem_ids, ids)) || (rc = upload(ctx, &h->d_ret, ret)) ||
-        (rc = upload(ctx, &h->d_ret_n, ret_n)) || (rc = upload(ctx, &h->d_T, T)) ||
-        (rc = upload(ctx, &h->d_band_off, cnt)) || (rc = upload(ctx, &h->d_band_src, band_src)) ||
-        (rc = upload(ctx, &h->d_mass_band, mass_band)) || (rc = upload(ctx, &h->d_dof_off, doff)) ||
-        (rc = upload(ctx, &h->d_dof_src, dsrc)) || (rc = upload(ctx, &h->d_m_off, moff)) ||
-        (rc = upload(ctx, &h->d_m_col, mcol)) || (rc = upload(ctx, &h->d_m_val, mval)) ||
-        (rc = upload(ctx, &h->d_load, load)))
+        (rc = upload(ctx, &h->d_ret, ret)) || (rc = upload(ctx, &h->d_ret_n, ret_n)) ||
+        (rc = upload(ctx, &h->d_T, T)) || (rc = upload(ctx, &h->d_band_off, cnt)) ||
+        (rc = upload(ctx, &h->d_band_src, band_src)) || (rc = upload(ctx, &h->d_mass_band, mass_band)) ||
+        (rc = upload(ctx, &h->d_dof_off, doff)) || (rc = upload(ctx, &h->d_dof_src, dsrc)) ||
+        (rc = upload(ctx, &h->d_m_off, moff)) || (rc = upload(ctx, &h->d_m_col, mcol)) ||
+        (rc = upload(ctx, &h->d_m_val, mval)) || (rc = upload(ctx, &h->d_load, load)))
         return rc;
-    double **vecs[] = {&h->d_uf, &h->d_vf, &h->d_af, &h->d_un, &h->d_vn, &h->d_an, &h->d_vt, &h->d_at, &h->d_R, &h->d_du};
-    for (double **p : vecs)
-        if ((rc = dalloc(ctx, p, n))) return rc;
-    if ((rc = dalloc(ctx, &h->d_u, total)) || (rc = dalloc(ctx, &h->d_u12, (size_t)n_el * 12)) ||
-        (rc = dalloc(ctx, &h->d_k, (size_t)n_el * 144)) || (rc = dalloc(ctx, &h->d_f, (size_t)n_el * 12)) ||
-        (rc = dalloc(ctx, &h->d_khat, (size_t)n_el * 144)) || (rc = dalloc(ctx, &h->d_kinit_hat, (size_t)n_el * 144)) ||
-        (rc = dalloc(ctx, &h->d_fhat, (size_t)n_el * 12)) || (rc = dalloc(ctx, &h->d_yk, (size_t)n_el * 12)) ||
-        (rc = dalloc(ctx, &h->d_yi, (size_t)n_el * 12)) || (rc = dalloc(ctx, &h->d_status, n_el)) ||
-        (rc = dalloc(ctx, &h->d_band, slots)) || (rc = dalloc(ctx, &h->d_kinit_band, slots)) ||
-        (rc = dalloc(ctx, &h->d_scalar, 4)))
+    int32_t rec0;
+    if ((rc = sgpu_frame3d_add_record(ctx, d->elem_ids, d->ts_values, d->ts_n, d->ts_dt, d->ts_factor, &rec0)))
         return rc;
-    for (double **p : vecs) HIP_OK(ctx, hipMemset(*p, 0, sizeof(double) * n));
-    HIP_OK(ctx, hipMemset(h->d_u, 0, sizeof(double) * total));
-    HIP_OK(ctx, hipMemset(h->d_kinit_hat, 0, sizeof(double) * (size_t)n_el * 144));
     if (hbw_out) *hbw_out = hbw;
     if (free_count_out) *free_count_out = n;
     return SGPU_OK;
 }
 
+int sgpu_frame3d_add_record(sgpu_ctx *ctx, const int32_t *elem_ids, const double *const *ts_values,
+                            const int32_t *ts_n, const double *ts_dt, const double *ts_factor, int32_t *record) {
+    if (!ctx || !elem_ids) return SGPU_ERR_ARG;
+    Frame3dHost *h = ctx->frame3d;
+    if (!h) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
+    if (h->allocated) return fail(ctx, SGPU_ERR_STATE, "records are added before the first run");
+    if (h->n_rec > 0) return fail(ctx, SGPU_ERR_STATE, "records are added before the recorder is set");
+    for (int p = 0; p < h->n_patterns; ++p)
+        if (!ts_values || !ts_values[p] || !ts_n || ts_n[p] <= 0 || !ts_dt || ts_dt[p] <= 0.0 || !ts_factor)
+            return fail(ctx, SGPU_ERR_ARG, "pattern %d: bad Path series", p);
+    h->rec_elem_ids.insert(h->rec_elem_ids.end(), elem_ids, elem_ids + h->n_el);
+    for (int p = 0; p < h->n_patterns; ++p) {
+        h->rec_ts.emplace_back(ts_values[p], ts_values[p] + ts_n[p]);
+        h->rec_ts_dt.push_back(ts_dt[p]);
+        h->rec_ts_factor.push_back(ts_factor[p]);
+    }
+    h->rec_steps.push_back(0);
+    h->rec_status.push_back(0);
+    if (record) *record = h->R;
+    h->R += 1;
+    return SGPU_OK;
+}
+
 namespace {
 
-// one element state determination at the current u and its collapse
-int f3_elements(sgpu_ctx *ctx, Frame3dHost &h) {
+// device state of every record, allocated at the first run
+int f3_allocate(sgpu_ctx *ctx, Frame3dHost &h) {
+    if (h.allocated) return SGPU_OK;
+    const size_t R = h.R, n = h.n, n_el = h.n_el, slots = (size_t)h.n * h.W;
+    int rc;
+    double **vecs[] = {&h.d_uf, &h.d_vf, &h.d_af, &h.d_un, &h.d_vn, &h.d_an, &h.d_vt, &h.d_at, &h.d_R, &h.d_du};
+    for (double **p : vecs)
+        if ((rc = dalloc(ctx, p, R * n))) return rc;
+    if ((rc = dalloc(ctx, &h.d_u, R * h.total)) || (rc = dalloc(ctx, &h.d_u12, R * n_el * 12)) ||
+        (rc = dalloc(ctx, &h.d_k, R * n_el * 144)) || (rc = dalloc(ctx, &h.d_f, R * n_el * 12)) ||
+        (rc = dalloc(ctx, &h.d_khat, R * n_el * 144)) || (rc = dalloc(ctx, &h.d_kinit_hat, R * n_el * 144)) ||
+        (rc = dalloc(ctx, &h.d_fhat, R * n_el * 12)) || (rc = dalloc(ctx, &h.d_yk, R * n_el * 12)) ||
+        (rc = dalloc(ctx, &h.d_yi, R * n_el * 12)) || (rc = dalloc(ctx, &h.d_status, R * n_el)) ||
+        (rc = dalloc(ctx, &h.d_band, R * slots)) || (rc = dalloc(ctx, &h.d_kinit_band, R * slots)) ||
+        (rc = dalloc(ctx, &h.d_scalar, 4 * R)) || (rc = dalloc(ctx, &h.d_list, R)) ||
+        (rc = dalloc(ctx, &h.d_list_ids, R * n_el)) || (rc = dalloc(ctx, &h.d_row_of, R)) ||
+        (rc = dalloc(ctx, &h.d_ag, 3 * R)))
+        return rc;
+    for (double **p : vecs) HIP_OK(ctx, hipMemset(*p, 0, sizeof(double) * R * n));
+    HIP_OK(ctx, hipMemset(h.d_u, 0, sizeof(double) * R * h.total));
+    HIP_OK(ctx, hipMemset(h.d_kinit_hat, 0, sizeof(double) * R * n_el * 144));
+    HIP_OK(ctx, hipMemset(h.d_ag, 0, sizeof(double) * 3 * R));
+    h.allocated = true;
+    return SGPU_OK;
+}
+
+// the listed records become the kernels' record list (and their element
+// ids, in list order, the element pass's)
+int f3_set_list(sgpu_ctx *ctx, Frame3dHost &h, const std::vector<int32_t> &list) {
+    std::vector<int32_t> ids;
+    ids.reserve(list.size() * h.n_el);
+    for (int r : list) ids.insert(ids.end(), h.rec_elem_ids.begin() + (size_t)r * h.n_el,
+                                  h.rec_elem_ids.begin() + (size_t)(r + 1) * h.n_el);
+    HIP_OK(ctx, hipMemcpyAsync(h.d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIP_OK(ctx, hipMemcpyAsync(h.d_list_ids, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
+    return SGPU_OK;
+}
+
+inline dim3 grid2(int64_t n, int count) { return dim3((unsigned)((n + 255) / 256), (unsigned)count); }
+
+// one element state determination of the listed records at their current u,
+// and its collapse
+int f3_elements(sgpu_ctx *ctx, Frame3dHost &h, int count) {
     hipStream_t st = ctx->stream;
-    hipLaunchKernelGGL(f3_gather_u12_kernel, grid_of((int64_t)h.n_el * 12), dim3(256), 0, st, h.n_el, h.d_elem_nodes,
-                       h.d_u, h.d_u12);
+    const int64_t m = (int64_t)count * h.n_el;
+    hipLaunchKernelGGL(f3_gather_u12_kernel, grid_of(m * 12), dim3(256), 0, st, h.d_list, count, h.n_el, h.total,
+                       h.d_elem_nodes, h.d_u, h.d_u12);
     int rc;
     {
         ProfScope ps(ctx, SGPU_KERNEL_F3_ELEM);
-        rc = sgpu_elem3d_state_device(ctx, h.n_el, h.d_elem_ids, h.d_u12, h.d_k, h.d_f, h.d_status);
+        rc = sgpu_elem3d_state_device(ctx, (int)m, h.d_list_ids, h.d_u12, h.d_k, h.d_f, h.d_status);
     }
     if (rc) return rc;
-    hipLaunchKernelGGL(f3_elem_collapse_kernel, dim3(h.n_el), dim3(256), 0, st, h.n_el, h.d_ret, h.d_ret_n, h.d_T,
-                       h.d_k, h.d_f, h.d_kinit_hat, h.d_vt, h.d_khat, h.d_fhat, h.d_yk, h.d_yi);
+    hipLaunchKernelGGL(f3_elem_collapse_kernel, dim3((unsigned)m), dim3(256), 0, st, h.d_list, h.n, h.n_el, h.d_ret,
+                       h.d_ret_n, h.d_T, h.d_k, h.d_f, h.d_kinit_hat, h.d_vt, h.d_khat, h.d_fhat, h.d_yk, h.d_yi);
     HIP_OK(ctx, hipGetLastError());
     return SGPU_OK;
 }
 
-}  // namespace
+// the initial assembly of every record at u = 0 (:1207), its collapse
+// (:1307-1318), the initial tangent (:1329-1337) and the revert (:1452-1453)
+int f3_initialize(sgpu_ctx *ctx, Frame3dHost &h) {
+    if (h.initialized) return SGPU_OK;
+    int rc;
+    if ((rc = f3_allocate(ctx, h))) return rc;
+    std::vector<int32_t> all(h.R);
+    for (int r = 0; r < h.R; ++r) all[r] = r;
+    if ((rc = f3_set_list(ctx, h, all))) return rc;
+    if ((rc = f3_elements(ctx, h, h.R))) return rc;
+    hipStream_t st = ctx->stream;
+    HIP_OK(ctx, hipMemcpyAsync(h.d_kinit_hat, h.d_khat, sizeof(double) * (size_t)h.R * h.n_el * 144,
+                               hipMemcpyDeviceToDevice, st));
+    const int64_t slots = (int64_t)h.n * h.W;
+    hipLaunchKernelGGL(f3_band_kernel, grid2(slots, h.R), dim3(256), 0, st, h.d_list, slots, (int64_t)h.n_el * 144,
+                       h.d_band_off, h.d_band_src, h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_kinit_band, 1, 0.0,
+                       0.0, 0.0, 0.0, 0.0, 0);
+    HIP_OK(ctx, hipGetLastError());
+    if ((rc = sgpu_sections_revert_all(ctx))) return rc;
+    h.initialized = true;
+    return SGPU_OK;
+}
 
-int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, int32_t *steps_done,
-                     int32_t *status, int32_t *dense_fallbacks) {
-    if (!ctx || steps < 0 || !iters || !steps_done || !status) return SGPU_ERR_ARG;
-    if (dense_fallbacks) *dense_fallbacks = 0;
-    Frame3dHost *hp = ctx->frame3d;
-    if (!hp) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
-    Frame3dHost &h = *hp;
-    HIP_OK(ctx, hipSetDevice(ctx->device));
+// `steps` more steps of every running record, in lockstep by step; each
+// record iterates until it converges, stops or fails.  iters[r * steps +
+// step], dense_fallbacks[r] (may be NULL); u_hist (R == 1 only) as
+// sgpu_frame3d_run documents.
+int f3_run(sgpu_ctx *ctx, Frame3dHost &h, int steps, double *u_hist, int32_t *iters, int32_t *dense_fallbacks) {
     hipStream_t st = ctx->stream;
     const int n = h.n;
     const double dt = h.dt, gamma = h.gamma, beta = h.beta;
@@ -1155,90 +1293,153 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
     const int has_damping = h.alpha_m != 0.0 || h.beta_k != 0.0 || h.beta_k_init != 0.0;
     const int64_t slots = (int64_t)n * h.W;
     int rc;
-    *status = 0;
-    *steps_done = 0;
-    if (!h.initialized) {
-        // initial assembly at u = 0 (:1207), collapse (:1307-1318), initial
-        // tangent (:1329-1337), revert (:1452-1453)
-        if ((rc = f3_elements(ctx, h))) return rc;
-        HIP_OK(ctx, hipMemcpyAsync(h.d_kinit_hat, h.d_khat, sizeof(double) * (size_t)h.n_el * 144,
-                                   hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(f3_band_kernel, grid_of(slots), dim3(256), 0, st, slots, h.d_band_off, h.d_band_src,
-                           h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_kinit_band, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0);
-        HIP_OK(ctx, hipGetLastError());
-        if ((rc = sgpu_sections_revert_all(ctx))) return rc;
-        h.initialized = true;
-    }
-    double host_scalar[4];
+    if ((rc = f3_initialize(ctx, h))) return rc;
+    std::vector<double> scal(4 * (size_t)h.R), ag(3 * (size_t)h.R, 0.0);
     for (int step = 0; step < steps; ++step) {
-        hipLaunchKernelGGL(f3_predict_kernel, grid_of(n), dim3(256), 0, st, n, h.d_uf, h.d_vf, h.d_af, h.d_un,
-                           h.d_vn, h.d_an, dt, beta);
-        hipLaunchKernelGGL(f3_expand_kernel, grid_of(h.total), dim3(256), 0, st, h.total, h.d_u, h.d_uf,
-                           h.d_free_idx, h.d_row_offsets, h.d_dof_pool, h.d_coeff_pool, h.d_slave);
-        const double t = (double)(h.steps_done + 1) * dt;
-        double ag[3] = {0.0, 0.0, 0.0};
-        for (int p = 0; p < h.n_patterns; ++p) ag[p] = path_at(h.ts[p], h.ts_dt[p], h.ts_factor[p], t);
-        int converged = 0, it_count = 0;
-        for (int it = 0; it < h.max_iters; ++it) {
-            it_count++;
-            hipLaunchKernelGGL(f3_trial_kernel, grid_of(n), dim3(256), 0, st, n, h.d_uf, h.d_un, h.d_vn, h.d_an,
-                               h.d_at, h.d_vt, a0, a2, a3, gamma, dt);
-            if ((rc = f3_elements(ctx, h))) return rc;
-            hipLaunchKernelGGL(f3_residual_kernel, grid_of(n), dim3(256), 0, st, n, h.n_patterns, h.d_load, ag[0],
-                               ag[1], ag[2], h.d_dof_off, h.d_dof_src, h.d_fhat, h.d_yk, h.d_yi, h.d_m_off, h.d_m_col,
-                               h.d_m_val, h.d_vt, h.d_at, h.alpha_m, h.beta_k, h.beta_k_init, has_damping, h.d_R);
-            hipLaunchKernelGGL(f3_norm_kernel, dim3(1), dim3(1024), 0, st, n, h.d_R, h.n_el, h.d_status, h.d_scalar);
+        std::vector<int32_t> live;
+        for (int r = 0; r < h.R; ++r)
+            if (h.rec_status[r] == 0) live.push_back(r);
+        if (live.empty()) break;
+        for (int r : live) {
+            const double t = (double)(h.rec_steps[r] + 1) * dt;
+            for (int p = 0; p < h.n_patterns; ++p) {
+                const int q = r * h.n_patterns + p;
+                ag[3 * r + p] = path_at(h.rec_ts[q], h.rec_ts_dt[q], h.rec_ts_factor[q], t);
+            }
+        }
+        HIP_OK(ctx, hipMemcpyAsync(h.d_ag, ag.data(), sizeof(double) * ag.size(), hipMemcpyHostToDevice, st));
+        if ((rc = f3_set_list(ctx, h, live))) return rc;
+        const int nl = (int)live.size();
+        hipLaunchKernelGGL(f3_predict_kernel, grid2(n, nl), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_vf, h.d_af,
+                           h.d_un, h.d_vn, h.d_an, dt, beta);
+        hipLaunchKernelGGL(f3_expand_kernel, grid2(h.total, nl), dim3(256), 0, st, h.d_list, h.total, n, h.d_u,
+                           h.d_uf, h.d_free_idx, h.d_row_offsets, h.d_dof_pool, h.d_coeff_pool, h.d_slave);
+        std::vector<int32_t> iter = live, converged;
+        for (int it = 0; it < h.max_iters && !iter.empty(); ++it) {
+            if (it > 0 && (rc = f3_set_list(ctx, h, iter))) return rc;
+            const int ni = (int)iter.size();
+            hipLaunchKernelGGL(f3_trial_kernel, grid2(n, ni), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_un, h.d_vn,
+                               h.d_an, h.d_at, h.d_vt, a0, a2, a3, gamma, dt);
+            if ((rc = f3_elements(ctx, h, ni))) return rc;
+            hipLaunchKernelGGL(f3_residual_kernel, grid2(n, ni), dim3(256), 0, st, h.d_list, n, h.n_el, h.n_patterns,
+                               h.d_load, h.d_ag, h.d_dof_off, h.d_dof_src, h.d_fhat, h.d_yk, h.d_yi, h.d_m_off,
+                               h.d_m_col, h.d_m_val, h.d_vt, h.d_at, h.alpha_m, h.beta_k, h.beta_k_init, has_damping,
+                               h.d_R);
+            hipLaunchKernelGGL(f3_norm_kernel, dim3(ni), dim3(1024), 0, st, h.d_list, n, h.d_R, h.n_el, h.d_status,
+                               h.d_scalar);
             HIP_OK(ctx, hipGetLastError());
-            HIP_OK(ctx, hipMemcpyAsync(host_scalar, h.d_scalar, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+            HIP_OK(ctx, hipMemcpyAsync(scal.data(), h.d_scalar, sizeof(double) * scal.size(), hipMemcpyDeviceToHost,
+                                       st));
             HIP_OK(ctx, hipStreamSynchronize(st));
-            if (host_scalar[1] != 0.0) {  // the reference aborts on an element's local non-convergence
-                iters[step] = it_count;
-                *status = SGPU_F3_ELEMENT_FAILED;
-                return SGPU_OK;
+            std::vector<int32_t> next;
+            for (int r : iter) {
+                iters[(size_t)r * steps + step] = it + 1;
+                if (scal[4 * r + 1] != 0.0)  // the reference aborts on an element's local non-convergence
+                    h.rec_status[r] = SGPU_F3_ELEMENT_FAILED;
+                else if (std::sqrt(scal[4 * r]) <= h.tol)
+                    converged.push_back(r);
+                else
+                    next.push_back(r);
             }
-            if (std::sqrt(host_scalar[0]) <= h.tol) {
-                converged = 1;
-                break;
-            }
-            hipLaunchKernelGGL(f3_band_kernel, grid_of(slots), dim3(256), 0, st, slots, h.d_band_off, h.d_band_src,
-                               h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_band, 0, a0, a1, h.alpha_m, h.beta_k,
-                               h.beta_k_init, has_damping);
+            iter.swap(next);
+            if (iter.empty()) break;
+            if ((rc = f3_set_list(ctx, h, iter))) return rc;
+            const int nu = (int)iter.size();
+            hipLaunchKernelGGL(f3_band_kernel, grid2(slots, nu), dim3(256), 0, st, h.d_list, slots,
+                               (int64_t)h.n_el * 144, h.d_band_off, h.d_band_src, h.d_khat, h.d_mass_band,
+                               h.d_kinit_band, h.d_band, 0, a0, a1, h.alpha_m, h.beta_k, h.beta_k_init, has_damping);
             {
                 ProfScope ps(ctx, SGPU_KERNEL_F3_SOLVE);
-                if ((rc = launch_band_solve(ctx, 0, n, h.hbw, h.d_band, h.d_R, h.d_du, h.d_scalar + 2))) return rc;
+                if ((rc = launch_band_solve(ctx, 0, n, h.hbw, h.d_band, h.d_R, h.d_du, h.d_scalar + 2, h.d_list, nu)))
+                    return rc;
             }
-            HIP_OK(ctx, hipMemcpyAsync(host_scalar + 2, h.d_scalar + 2, sizeof(double), hipMemcpyDeviceToHost, st));
-            hipLaunchKernelGGL(f3_update_kernel, grid_of(n), dim3(256), 0, st, n, h.d_uf, h.d_du);
-            hipLaunchKernelGGL(f3_expand_kernel, grid_of(h.total), dim3(256), 0, st, h.total, h.d_u, h.d_uf,
-                               h.d_free_idx, h.d_row_offsets, h.d_dof_pool, h.d_coeff_pool, h.d_slave);
+            hipLaunchKernelGGL(f3_update_kernel, grid2(n, nu), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_du);
+            hipLaunchKernelGGL(f3_expand_kernel, grid2(h.total, nu), dim3(256), 0, st, h.d_list, h.total, n, h.d_u,
+                               h.d_uf, h.d_free_idx, h.d_row_offsets, h.d_dof_pool, h.d_coeff_pool, h.d_slave);
+            HIP_OK(ctx, hipGetLastError());
+            HIP_OK(ctx, hipMemcpyAsync(scal.data(), h.d_scalar, sizeof(double) * scal.size(), hipMemcpyDeviceToHost,
+                                       st));
             HIP_OK(ctx, hipStreamSynchronize(st));
-            if (host_scalar[2] == 1.0 && dense_fallbacks) (*dense_fallbacks)++;
-            if (host_scalar[2] == 2.0) {
-                iters[step] = it_count;
-                *status = SGPU_F3_SINGULAR;
-                return SGPU_OK;
+            next.clear();
+            for (int r : iter) {
+                if (scal[4 * r + 2] == 1.0 && dense_fallbacks) dense_fallbacks[r]++;
+                if (scal[4 * r + 2] == 2.0)
+                    h.rec_status[r] = SGPU_F3_SINGULAR;
+                else
+                    next.push_back(r);
             }
+            iter.swap(next);
         }
-        iters[step] = it_count;
-        if (!converged) {
-            *status = h.steps_done + 1;
-            return SGPU_OK;
-        }
-        if ((rc = sgpu_sections_commit_all(ctx))) return rc;
-        hipLaunchKernelGGL(f3_newmark_kernel, grid_of(n), dim3(256), 0, st, n, h.d_uf, h.d_un, h.d_vn, h.d_an, h.d_af,
-                           h.d_vf, a0, a2, a3, gamma, dt);
+        for (int r : iter) h.rec_status[r] = h.rec_steps[r] + 1;  // not converged within max_iters
+        if (converged.empty()) continue;
+        if ((rc = sgpu_sections_commit_all(ctx))) return rc;  // records that stopped are never read again
+        if ((rc = f3_set_list(ctx, h, converged))) return rc;
+        const int nc = (int)converged.size();
+        hipLaunchKernelGGL(f3_newmark_kernel, grid2(n, nc), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_un, h.d_vn,
+                           h.d_an, h.d_af, h.d_vf, a0, a2, a3, gamma, dt);
         HIP_OK(ctx, hipGetLastError());
-        if (u_hist)
+        if (u_hist && h.R == 1)
             HIP_OK(ctx, hipMemcpyAsync(u_hist + (size_t)step * h.total, h.d_u, sizeof(double) * h.total,
                                        hipMemcpyDeviceToHost, st));
-        if (h.n_rec > 0 && h.steps_done < h.rec_cap)
-            hipLaunchKernelGGL(f3_record_kernel, grid_of(h.n_rec), dim3(256), 0, st, h.n_rec, h.d_rec_dofs, h.d_u,
-                               h.d_rec + (size_t)h.steps_done * h.n_rec);
-        h.steps_done++;
-        (*steps_done)++;
+        if (h.n_rec > 0) {
+            std::vector<int32_t> row_of(h.R, 0), rec_list;
+            for (int r : converged)
+                if (h.rec_steps[r] < h.rec_cap) {
+                    row_of[r] = h.rec_steps[r];
+                    rec_list.push_back(r);
+                }
+            if (!rec_list.empty()) {
+                HIP_OK(ctx, hipMemcpyAsync(h.d_row_of, row_of.data(), sizeof(int32_t) * h.R, hipMemcpyHostToDevice,
+                                           st));
+                if ((rc = f3_set_list(ctx, h, rec_list))) return rc;
+                hipLaunchKernelGGL(f3_record_kernel, grid2(h.n_rec, (int)rec_list.size()), dim3(256), 0, st,
+                                   h.d_list, h.n_rec, h.total, h.d_rec_dofs, h.d_u, h.d_rec,
+                                   (int64_t)h.rec_cap * h.n_rec, h.d_row_of);
+                HIP_OK(ctx, hipStreamSynchronize(st));  // row_of goes out of scope
+            }
+        }
+        for (int r : converged) h.rec_steps[r] += 1;
     }
     HIP_OK(ctx, hipStreamSynchronize(st));
     return SGPU_OK;
+}
+
+}  // namespace
+
+int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, int32_t *steps_done,
+                     int32_t *status, int32_t *dense_fallbacks) {
+    if (!ctx || steps < 0 || !iters || !steps_done || !status) return SGPU_ERR_ARG;
+    Frame3dHost *hp = ctx->frame3d;
+    if (!hp) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
+    Frame3dHost &h = *hp;
+    if (h.R != 1) return fail(ctx, SGPU_ERR_STATE, "the frame holds %d records: use sgpu_frame3d_run_records", h.R);
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    if (dense_fallbacks) *dense_fallbacks = 0;
+    const int before = h.rec_steps[0];
+    for (int s = 0; s < steps; ++s) iters[s] = 0;
+    const int rc = h.rec_status[0] == 0 ? f3_run(ctx, h, steps, u_hist, iters, dense_fallbacks) : SGPU_OK;
+    *steps_done = h.rec_steps[0] - before;
+    *status = h.rec_status[0];
+    return rc;
+}
+
+int sgpu_frame3d_run_records(sgpu_ctx *ctx, int steps, int32_t *iters, int32_t *steps_done, int32_t *status,
+                             int32_t *dense_fallbacks) {
+    if (!ctx || steps < 0 || !iters || !steps_done || !status) return SGPU_ERR_ARG;
+    Frame3dHost *hp = ctx->frame3d;
+    if (!hp) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
+    Frame3dHost &h = *hp;
+    HIP_OK(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> before(h.rec_steps);
+    for (size_t q = 0; q < (size_t)h.R * steps; ++q) iters[q] = 0;
+    if (dense_fallbacks)
+        for (int r = 0; r < h.R; ++r) dense_fallbacks[r] = 0;
+    const int rc = f3_run(ctx, h, steps, nullptr, iters, dense_fallbacks);
+    for (int r = 0; r < h.R; ++r) {
+        steps_done[r] = h.rec_steps[r] - before[r];
+        status[r] = h.rec_status[r];
+    }
+    return rc;
 }
 
 int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band_out) {
@@ -1255,21 +1456,14 @@ int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band_out) {
     const int has_damping = h.alpha_m != 0.0 || h.beta_k != 0.0 || h.beta_k_init != 0.0;
     const int64_t slots = (int64_t)n * h.W;
     int rc;
-    if (!h.initialized) {
-        if ((rc = f3_elements(ctx, h))) return rc;
-        HIP_OK(ctx, hipMemcpyAsync(h.d_kinit_hat, h.d_khat, sizeof(double) * (size_t)h.n_el * 144,
-                                   hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(f3_band_kernel, grid_of(slots), dim3(256), 0, st, slots, h.d_band_off, h.d_band_src,
-                           h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_kinit_band, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0);
-        if ((rc = sgpu_sections_revert_all(ctx))) return rc;
-        h.initialized = true;
-    }
-    hipLaunchKernelGGL(f3_trial_kernel, grid_of(n), dim3(256), 0, st, n, h.d_uf, h.d_un, h.d_vn, h.d_an, h.d_at,
-                       h.d_vt, a0, a2, a3, gamma, dt);
-    if ((rc = f3_elements(ctx, h))) return rc;
-    hipLaunchKernelGGL(f3_band_kernel, grid_of(slots), dim3(256), 0, st, slots, h.d_band_off, h.d_band_src, h.d_khat,
-                       h.d_mass_band, h.d_kinit_band, h.d_band, 0, a0, a1, h.alpha_m, h.beta_k, h.beta_k_init,
-                       has_damping);
+    if ((rc = f3_initialize(ctx, h))) return rc;
+    if ((rc = f3_set_list(ctx, h, {0}))) return rc;  // record 0
+    hipLaunchKernelGGL(f3_trial_kernel, grid2(n, 1), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_un, h.d_vn, h.d_an,
+                       h.d_at, h.d_vt, a0, a2, a3, gamma, dt);
+    if ((rc = f3_elements(ctx, h, 1))) return rc;
+    hipLaunchKernelGGL(f3_band_kernel, grid2(slots, 1), dim3(256), 0, st, h.d_list, slots, (int64_t)h.n_el * 144,
+                       h.d_band_off, h.d_band_src, h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_band, 0, a0, a1,
+                       h.alpha_m, h.beta_k, h.beta_k_init, has_damping);
     HIP_OK(ctx, hipGetLastError());
     HIP_OK(ctx, hipMemcpyAsync(band_out, h.d_band, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
     HIP_OK(ctx, hipStreamSynchronize(st));
@@ -1292,28 +1486,41 @@ int sgpu_frame3d_set_recorder(sgpu_ctx *ctx, int n_record, const int32_t *dofs, 
     HIP_OK(ctx, hipSetDevice(ctx->device));
     int rc;
     if ((rc = upload(ctx, &h->d_rec_dofs, std::vector<int32_t>(dofs, dofs + n_record)))) return rc;
-    if ((rc = dalloc(ctx, &h->d_rec, (size_t)n_record * max_steps))) return rc;
+    if ((rc = dalloc(ctx, &h->d_rec, (size_t)h->R * n_record * max_steps))) return rc;
     h->n_rec = n_record;
     h->rec_cap = max_steps;
     return SGPU_OK;
 }
 
-int sgpu_frame3d_recorded(sgpu_ctx *ctx, double *hist, int32_t *steps) {
+int sgpu_frame3d_recorded_record(sgpu_ctx *ctx, int32_t record, double *hist, int32_t *steps) {
     if (!ctx || !hist || !steps) return SGPU_ERR_ARG;
     Frame3dHost *h = ctx->frame3d;
     if (!h || h->n_rec == 0) return fail(ctx, SGPU_ERR_STATE, "no recorder is set");
+    if (record < 0 || record >= h->R) return fail(ctx, SGPU_ERR_ARG, "record out of range");
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
-    *steps = std::min(h->steps_done, h->rec_cap);
-    HIP_OK(ctx, hipMemcpy(hist, h->d_rec, sizeof(double) * (size_t)*steps * h->n_rec, hipMemcpyDeviceToHost));
+    *steps = std::min(h->rec_steps[record], h->rec_cap);
+    HIP_OK(ctx, hipMemcpy(hist, h->d_rec + (size_t)record * h->rec_cap * h->n_rec,
+                          sizeof(double) * (size_t)*steps * h->n_rec, hipMemcpyDeviceToHost));
     return SGPU_OK;
+}
+
+int sgpu_frame3d_recorded(sgpu_ctx *ctx, double *hist, int32_t *steps) {
+    return sgpu_frame3d_recorded_record(ctx, 0, hist, steps);
+}
+
+int sgpu_frame3d_envelopes_record(sgpu_ctx *ctx, int32_t record, int n_drift, const int32_t *col_i,
+                                  const int32_t *col_j, const double *dx, double *disp_env, double *drift_env) {
+    if (!ctx || !disp_env || (n_drift > 0 && !drift_env)) return SGPU_ERR_ARG;
+    Frame3dHost *h = ctx->frame3d;
+    if (!h || h->n_rec == 0) return fail(ctx, SGPU_ERR_STATE, "no recorder is set");
+    if (record < 0 || record >= h->R) return fail(ctx, SGPU_ERR_ARG, "record out of range");
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    return envelopes_device(ctx, h->d_rec, {(int64_t)record * h->rec_cap * h->n_rec},
+                            {std::min(h->rec_steps[record], h->rec_cap)}, h->n_rec, n_drift, col_i, col_j, dx,
+                            disp_env, drift_env);
 }
 
 int sgpu_frame3d_envelopes(sgpu_ctx *ctx, int n_drift, const int32_t *col_i, const int32_t *col_j, const double *dx,
                            double *disp_env, double *drift_env) {
-    if (!ctx || !disp_env || (n_drift > 0 && !drift_env)) return SGPU_ERR_ARG;
-    Frame3dHost *h = ctx->frame3d;
-    if (!h || h->n_rec == 0) return fail(ctx, SGPU_ERR_STATE, "no recorder is set");
-    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
-    return envelopes_device(ctx, h->d_rec, {0}, {std::min(h->steps_done, h->rec_cap)}, h->n_rec, n_drift, col_i,
-                            col_j, dx, disp_env, drift_env);
+    return sgpu_frame3d_envelopes_record(ctx, 0, n_drift, col_i, col_j, dx, disp_env, drift_env);
 }

@@ -59,6 +59,17 @@ def _declare(lib):
     lib.sgpu_frame3d_envelopes.restype = c_int
     lib.sgpu_frame3d_envelopes.argtypes = [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_double),
                                            POINTER(c_double), POINTER(c_double)]
+    lib.sgpu_frame3d_add_record.restype = c_int
+    lib.sgpu_frame3d_add_record.argtypes = [c_void_p, POINTER(c_int32), POINTER(POINTER(c_double)), POINTER(c_int32),
+                                            POINTER(c_double), POINTER(c_double), POINTER(c_int32)]
+    lib.sgpu_frame3d_run_records.restype = c_int
+    lib.sgpu_frame3d_run_records.argtypes = [c_void_p, c_int, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                             POINTER(c_int32)]
+    lib.sgpu_frame3d_recorded_record.restype = c_int
+    lib.sgpu_frame3d_recorded_record.argtypes = [c_void_p, c_int32, POINTER(c_double), POINTER(c_int32)]
+    lib.sgpu_frame3d_envelopes_record.restype = c_int
+    lib.sgpu_frame3d_envelopes_record.argtypes = [c_void_p, c_int32, c_int, POINTER(c_int32), POINTER(c_int32),
+                                                  POINTER(c_double), POINTER(c_double), POINTER(c_double)]
     lib._f3_declared = True
 
 
@@ -67,23 +78,34 @@ class Frame3dRun:
     free map as the reference does; 1 keeps every coupling of the collapsed
     system."""
 
-    def __init__(self, fc: Frame3dCase, device: int = 0, hbw_rule: int = 0):
+    def __init__(self, fc: Frame3dCase, device: int = 0, hbw_rule: int = 0, records=None):
+        """records: further ground-motion records of the same frame, each a
+        list of Pattern (one per pattern of the case, same directions); the
+        case's own patterns are record 0."""
         self.fc = fc
         self.rt = SectionRuntime(fc.materials, device=device)
         _declare(self.rt.lib)
         defs = {sid: self.rt.add_definition(3, fc.section_cells[sid]) for sid in fc.section_ids}
         self.elems = Elements3d(self.rt)
         kinds = ("fbc", "dbc")
-        ids = []
-        for e in range(fc.n_elem):
-            sid = int(fc.elem_section[e])
-            nip = int(fc.elem_nip[e])
-            vz = None if np.isnan(fc.elem_vecxz[e]).any() else fc.elem_vecxz[e]
-            G, J = fc.section_gj[sid]
-            i, j = fc.elem_nodes[e]
-            ids.append(self.elems.add(kinds[int(fc.elem_kind[e])], defs[sid], fc.elem_xi[e, :nip], fc.elem_w[e, :nip],
-                                      fc.coords[i], fc.coords[j], vecxz=vz, G=G, J=J,
-                                      transf=("Linear", "PDelta")[int(fc.elem_transf[e])]))
+        extra = list(records or [])
+        for rec in extra:
+            if len(rec) != len(fc.patterns) or any(a.direction != b.direction for a, b in zip(rec, fc.patterns)):
+                raise ValueError("every record gives one series per pattern of the case, same directions")
+        all_ids = []
+        for _ in range(1 + len(extra)):  # each record its own elements and section states
+            ids = []
+            for e in range(fc.n_elem):
+                sid = int(fc.elem_section[e])
+                nip = int(fc.elem_nip[e])
+                vz = None if np.isnan(fc.elem_vecxz[e]).any() else fc.elem_vecxz[e]
+                G, J = fc.section_gj[sid]
+                i, j = fc.elem_nodes[e]
+                ids.append(self.elems.add(kinds[int(fc.elem_kind[e])], defs[sid], fc.elem_xi[e, :nip],
+                                          fc.elem_w[e, :nip], fc.coords[i], fc.coords[j], vecxz=vz, G=G, J=J,
+                                          transf=("Linear", "PDelta")[int(fc.elem_transf[e])]))
+            all_ids.append(np.array(ids, dtype=np.int32))
+        ids = all_ids[0]
         self.rt.finalize()
         k = dict(sp=np.ascontiguousarray(fc.sp, dtype=np.int32), mass=np.ascontiguousarray(fc.mass),
                  ro=np.ascontiguousarray(fc.row_offsets, dtype=np.int32),
@@ -111,6 +133,17 @@ class Frame3dRun:
         self.rt._check(self.rt.lib.sgpu_frame3d_create(self.rt.ctx, ctypes.byref(d), ctypes.byref(hbw),
                                                        ctypes.byref(nf)))
         self.hbw, self.free_count = hbw.value, nf.value
+        for rec_ids, rec in zip(all_ids[1:], extra):
+            vals = [np.ascontiguousarray(p.values, dtype=np.float64) for p in rec]
+            self._keep.setdefault("rec", []).append((rec_ids, vals))
+            tv = (POINTER(c_double) * MAX_PATTERNS)(*[dptr(v) for v in vals])
+            tn = (c_int32 * MAX_PATTERNS)(*[len(v) for v in vals])
+            tdt = (c_double * MAX_PATTERNS)(*[p.dt for p in rec])
+            tf = (c_double * MAX_PATTERNS)(*[p.factor for p in rec])
+            out = c_int32()
+            self.rt._check(self.rt.lib.sgpu_frame3d_add_record(self.rt.ctx, iptr(rec_ids), tv, tn, tdt, tf,
+                                                               ctypes.byref(out)))
+        self.n_records = 1 + len(extra)
         self.steps_done = 0
         self._rec = np.zeros(0, dtype=np.int32)
         self._rec_cap = 0
@@ -132,6 +165,16 @@ class Frame3dRun:
         self.rt._check(self.rt.lib.sgpu_frame3d_effective_band(self.rt.ctx, dptr(band)))
         return band
 
+    def run_records(self, steps: int) -> dict:
+        """`steps` more steps of every record together: iters [records,
+        steps], steps_done, status, dense_fallbacks per record."""
+        R = self.n_records
+        iters = np.zeros((R, steps), dtype=np.int32)
+        done, status, fb = (np.zeros(R, dtype=np.int32) for _ in range(3))
+        self.rt._check(self.rt.lib.sgpu_frame3d_run_records(self.rt.ctx, steps, iptr(iters), iptr(done), iptr(status),
+                                                            iptr(fb)))
+        return {"iters": iters, "steps_done": done, "status": status, "dense_fallbacks": fb}
+
     def set_recorder(self, dofs, max_steps: int) -> None:
         """Record these full-DOF displacements after every converged step,
         on the device."""
@@ -139,13 +182,13 @@ class Frame3dRun:
         self._rec_cap = max_steps
         self.rt._check(self.rt.lib.sgpu_frame3d_set_recorder(self.rt.ctx, len(self._rec), iptr(self._rec), max_steps))
 
-    def recorded(self) -> np.ndarray:
-        hist = np.zeros((self._rec_cap, len(self._rec)))
+    def recorded(self, record: int = 0) -> np.ndarray:
+        hist = np.zeros((max(self._rec_cap, 1), max(len(self._rec), 1)))
         steps = c_int32()
-        self.rt._check(self.rt.lib.sgpu_frame3d_recorded(self.rt.ctx, dptr(hist), ctypes.byref(steps)))
-        return hist[:steps.value]
+        self.rt._check(self.rt.lib.sgpu_frame3d_recorded_record(self.rt.ctx, record, dptr(hist), ctypes.byref(steps)))
+        return hist[:steps.value, :len(self._rec)]
 
-    def envelopes(self, col_i=(), col_j=(), dx=()):
+    def envelopes(self, col_i=(), col_j=(), dx=(), record: int = 0):
         """(disp_env (n_record, 3), drift_env (n_drift, 3)): min, max and
         largest magnitude over the recorded steps, computed on the device."""
         ci = np.ascontiguousarray(col_i, dtype=np.int32)
@@ -153,8 +196,8 @@ class Frame3dRun:
         d = np.ascontiguousarray(dx, dtype=np.float64)
         disp = np.zeros((max(len(self._rec), 1), 3))
         drift = np.zeros((max(len(ci), 1), 3))
-        self.rt._check(self.rt.lib.sgpu_frame3d_envelopes(self.rt.ctx, len(ci), iptr(ci), iptr(cj), dptr(d),
-                                                          dptr(disp), dptr(drift)))
+        self.rt._check(self.rt.lib.sgpu_frame3d_envelopes_record(self.rt.ctx, record, len(ci), iptr(ci), iptr(cj),
+                                                                 dptr(d), dptr(disp), dptr(drift)))
         return disp[:len(self._rec)], drift[:len(ci)]
 
     def close(self) -> None:

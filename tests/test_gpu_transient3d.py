@@ -153,3 +153,41 @@ def test_gpu_frame3d_entry_points_refuse_bad_calls():
         run.envelopes([0], [1], [0.0])
     assert run.run(4)["status"] == 0
     run.close()
+
+
+def test_gpu_records_run_together_match_each_oracle_run():
+    """Three records of one rigid-diaphragm frame in one context (their
+    element and band-solve work in the same launches) against the oracle
+    run of each record alone: identical iteration counts and statuses,
+    displacements within 1e-8 of the peak."""
+    from strut_amd.frame3d import Pattern, recorder_plan
+
+    steps = 60
+    base = cases.rc_frame3d_case(2, 1, 1, steps=steps, dt=0.005, n_core=4, n_bars=4, pgas_g=(0.5, 0.35))
+    fc = load_frame3d_case(base)
+    recs, fcs = [], []
+    for seeds, pgas in (((5, 6), (0.6, 0.3)), ((7, 8), (0.2, 0.45))):
+        c = cases.rc_frame3d_case(2, 1, 1, steps=steps, dt=0.005, n_core=4, n_bars=4, seeds=seeds, pgas_g=pgas)
+        f = load_frame3d_case(c)
+        fcs.append(f)
+        recs.append([Pattern(p.direction, p.values, p.dt, p.factor) for p in f.patterns])
+    cols, ci, cj, dx = recorder_plan(fc)
+    full = np.arange(fc.total_dofs, dtype=np.int32)
+    run = Frame3dRun(fc, records=recs)
+    run.set_recorder(full, steps)
+    got = run.run_records(steps)
+    for r, f in enumerate([fc] + fcs):
+        ref = oracle.run_transient3d(f)
+        k = ref["steps_done"]
+        print(f"record {r}: status {got['status'][r]}/{ref['status']} iters {got['iters'][r].sum()}/"
+              f"{ref['iters'].sum()}", flush=True)
+        assert got["status"][r] == ref["status"] and got["steps_done"][r] == k
+        np.testing.assert_array_equal(got["iters"][r][:k], ref["iters"][:k])
+        hist = run.recorded(r)
+        scale = np.abs(ref["u"][:k]).max()
+        assert np.abs(hist - ref["u"][:k]).max() <= 1e-8 * scale
+        fi, fj = cols[ci], cols[cj]  # the drift channels over the full-DOF recording
+        _, drift = run.envelopes(fi, fj, dx, record=r)
+        _, want = oracle.envelopes(hist, k, fi, fj, dx)
+        np.testing.assert_array_equal(drift, want)
+    run.close()
