@@ -267,10 +267,11 @@ def gm_cpu_baseline(steps: int) -> dict:
             "sample": f"one record (seed 0), {r['steps_done']} steps, {dt:.2f} s, oracle C restatement, 1 thread"}
 
 
-def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0) -> dict:
+def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0, records: int = 8) -> dict:
     """Steps/s of the 3D RC frame transient (20 stories x 5 x 5 bays, rigid
     diaphragms, two simultaneous records) on the device: warmup steps, then
-    `steps` timed steps of the same run."""
+    `steps` timed steps of the same run; then `records` record pairs of the
+    same frame run together (record-steps/s against one record alone)."""
     from strut_amd import _native, cases
     from strut_amd.frame3d import load_frame3d_case
     from strut_amd.transient3d import Frame3dRun
@@ -289,7 +290,27 @@ def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0) -> dict:
     run.rt.profile_enable(False)
     run.close()
     iters = int(r["iters"][:r["steps_done"]].sum())
+    multi = None
+    if records > 1:  # the same frame under `records` record pairs, run together
+        from strut_amd.frame3d import Pattern
+
+        recs = []
+        for k in range(1, records):
+            c = cases.rc_frame3d_case(20, 5, 5, steps=warmup + steps, dt=0.005, seeds=(2 * k, 2 * k + 1),
+                                      pgas_g=(0.5, 0.35))
+            recs.append([Pattern(p.direction, p.values, p.dt, p.factor) for p in load_frame3d_case(c).patterns])
+        mrun = Frame3dRun(fc, hbw_rule=hbw_rule, records=recs)
+        mrun.run_records(warmup)
+        t0 = time.perf_counter()
+        mr = mrun.run_records(steps)
+        mdt = time.perf_counter() - t0
+        mrun.close()
+        done = int(mr["steps_done"].sum())
+        multi = {"records": records, "record_steps": done, "seconds": mdt, "record_steps_per_s": done / mdt,
+                 "newton_iters": int(mr["iters"].sum()), "stopped": int((mr["status"] != 0).sum()),
+                 "vs_one_record": (done / mdt) / (r["steps_done"] / dt)}
     return {"frame": "3D RC frame 20 stories x 5 x 5 bays", "nodes": fc.n_nodes, "elements": fc.n_elem,
+            "records_together": multi,
             "free_dofs": run.free_count, "hbw": run.hbw, "band_rule": ("element free map", "collapsed")[hbw_rule],
             "setup_seconds": setup_s, "warmup_steps": w["steps_done"], "steps": r["steps_done"],
             "status": r["status"], "seconds": dt, "steps_per_s": r["steps_done"] / dt, "newton_iters": iters,
@@ -465,6 +486,7 @@ def main() -> int:
     ap.add_argument("--f3-steps", type=int, default=10, help="timed steps of the 3D frame transient leg (0 = skip)")
     ap.add_argument("--f3-warmup", type=int, default=2)
     ap.add_argument("--f3-cpu-steps", type=int, default=2, help="steps of the 3D frame leg's CPU baseline sample")
+    ap.add_argument("--f3-records", type=int, default=8, help="record pairs run together in the 3D frame leg (1 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -495,8 +517,12 @@ def main() -> int:
                                           "steps dt 0.01 s, PGA 0.4 g seeded ground motions (seeds 0..N-1)",
                               **gm}
     if args.f3_steps > 0:
-        f3 = frame3d_leg(args.f3_steps, args.f3_warmup)
+        f3 = frame3d_leg(args.f3_steps, args.f3_warmup, records=args.f3_records)
         _progress(f"3D frame leg: {f3['steps_per_s']:.3g} steps/s, {f3['ms_per_newton_iter']:.3g} ms per Newton iteration")
+        if f3["records_together"]:
+            _progress(f"3D frame leg, {args.f3_records} records together: "
+                      f"{f3['records_together']['record_steps_per_s']:.3g} record-steps/s "
+                      f"({f3['records_together']['vs_one_record']:.3g}x one record)")
         if not args.no_cpu_baseline:
             f3["cpu_baseline"] = frame3d_cpu_baseline(args.f3_cpu_steps)
             _progress(f"3D frame leg CPU baseline: {f3['cpu_baseline']['value']:.3g} steps/s")
