@@ -38,7 +38,7 @@ Two legs, one JSON line.
    step, GM-steps/s and the peak inter-story drift error against the plain
    FP64 solution.
 
-`--gpus N` with N > 1 exits: this build measures one GPU.
+`--gpus N` with N > 1 exits: the benchmark measures one GPU.
 """
 
 from __future__ import annotations
@@ -489,8 +489,8 @@ def main() -> int:
     ap.add_argument("--f3-records", type=int, default=8, help="record pairs run together in the 3D frame leg (1 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(ensembles and models shard across GPUs without collectives; see the design notes)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(ensembles shard by instance and one large model by floors across GPUs; that path is not built)",
               file=sys.stderr)
         return 2
 
