@@ -612,6 +612,10 @@ int sgpu_frame2d_schedule_info(sgpu_ctx *ctx, double *out5) {
 
 namespace {
 
+// How far ahead the rounds schedule's sampled rate must be of the queue's
+// for the autotuner to choose rounds.
+constexpr double kRoundsMargin = 1.25;
+
 // Launch one chunk of the given schedule, wait, and report the GM-steps the
 // instances advanced and whether any instance is still active.
 int run_chunk(sgpu_ctx *ctx, FrameHost &fh, int schedule, int rounds, std::vector<InstState> &st, int64_t &advanced,
@@ -685,8 +689,16 @@ int sgpu_frame2d_run(sgpu_ctx *ctx, int32_t rounds) {
             }
             fh.rate_rounds = rate[0];
             fh.rate_queue = rate[1];
-            fh.chosen = rate[1] > rate[0] ? SGPU_SCHEDULE_QUEUE : SGPU_SCHEDULE_ROUNDS;
-            tuned_rate = rate[1] > rate[0] ? rate[1] : rate[0];
+            // Rounds must be clearly ahead to be chosen: lockstep rounds wait
+            // for each round's slowest element task, so their rate falls as
+            // records go nonlinear and their steps' costs diverge, while the
+            // queue's holds.  A sample taken in the nearly linear start of a
+            // batch puts rounds a little ahead (1.16x on 256 records of the
+            // 3-story frame) and then loses about 5 % of the run to rounds
+            // chunks before the rate-drop check measures again; one record
+            // alone still picks rounds (3x ahead).
+            fh.chosen = rate[0] > kRoundsMargin * rate[1] ? SGPU_SCHEDULE_ROUNDS : SGPU_SCHEDULE_QUEUE;
+            tuned_rate = fh.chosen == SGPU_SCHEDULE_ROUNDS ? rate[0] : rate[1];
             fh.tunings += 1;
             tune = false;
             active_at_tune = active;
