@@ -267,7 +267,7 @@ def gm_cpu_baseline(steps: int) -> dict:
             "sample": f"one record (seed 0), {r['steps_done']} steps, {dt:.2f} s, oracle C restatement, 1 thread"}
 
 
-def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0, records: int = 8) -> dict:
+def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0, records: int = 32) -> dict:
     """Steps/s of the 3D RC frame transient (20 stories x 5 x 5 bays, rigid
     diaphragms, two simultaneous records) on the device: warmup steps, then
     `steps` timed steps of the same run; then `records` record pairs of the
@@ -486,7 +486,7 @@ def main() -> int:
     ap.add_argument("--f3-steps", type=int, default=10, help="timed steps of the 3D frame transient leg (0 = skip)")
     ap.add_argument("--f3-warmup", type=int, default=2)
     ap.add_argument("--f3-cpu-steps", type=int, default=2, help="steps of the 3D frame leg's CPU baseline sample")
-    ap.add_argument("--f3-records", type=int, default=8, help="record pairs run together in the 3D frame leg (1 = skip)")
+    ap.add_argument("--f3-records", type=int, default=32, help="record pairs run together in the 3D frame leg (1 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
