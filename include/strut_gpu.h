@@ -130,6 +130,20 @@ int sgpu_section_eval(sgpu_ctx *ctx, int n, const sgpu_section_point *points,
 int sgpu_section_eval_device(sgpu_ctx *ctx, int n, const sgpu_section_point *d_points,
                              sgpu_section_result *d_results);
 
+/* sgpu_section_eval_device for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The same kernels and results; the aux
+ * pre-pass results go to a fixed device array of 2^20 doubles (8 MiB) that
+ * each GPU holds for the life of the process, so n * (most aux slots of any
+ * definition) * 2 must not exceed 2^20 (else SGPU_ERR_ARG, nothing
+ * launched).  Calls no hipMalloc/hipFree and records no profiling events.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (two streams or two contexts at once) race; calls in order on one stream
+ * do not (every entry the main pass reads is written by the same call's aux
+ * pass first). */
+int sgpu_section_eval_device_noalloc(sgpu_ctx *ctx, int n, const sgpu_section_point *d_points,
+                                     sgpu_section_result *d_results);
+
 /* Commit (trial -> committed) or revert (committed -> trial) every fiber of
  * every instance.  Replaces fiber_section2d_commit_runtime_all /
  * revert_trial_runtime_all (fiber2d.mojo:4191-4252) and

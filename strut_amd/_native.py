@@ -42,6 +42,7 @@ EXPORTED = (
     "sgpu_section_def_info",
     "sgpu_section_eval",
     "sgpu_section_eval_device",
+    "sgpu_section_eval_device_noalloc",
     "sgpu_sections_commit_all",
     "sgpu_sections_revert_all",
     "sgpu_sections_commit",
@@ -181,6 +182,7 @@ def load() -> ctypes.CDLL:
         "sgpu_section_def_info": (c_int, [c_void_p, c_int32, dp]),
         "sgpu_section_eval": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
         "sgpu_section_eval_device": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+        "sgpu_section_eval_device_noalloc": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
         "sgpu_sections_commit_all": (c_int, [c_void_p]),
         "sgpu_sections_revert_all": (c_int, [c_void_p]),
         "sgpu_sections_commit": (c_int, [c_void_p, c_int, ip, ip]),

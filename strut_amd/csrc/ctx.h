@@ -157,7 +157,8 @@ struct ProfScope {
     int kernel;
     hipEvent_t a = nullptr;
     ProfScope(sgpu_ctx *c, int k) : ctx(c), kernel(k) {
-        if (ctx->prof_on && (a = prof_event(ctx)) != nullptr) (void)hipEventRecord(a, ctx->stream);
+        // kernel < 0: never timed (the no-allocation entry points create no events)
+        if (kernel >= 0 && ctx->prof_on && (a = prof_event(ctx)) != nullptr) (void)hipEventRecord(a, ctx->stream);
     }
     ~ProfScope() {
         if (!a) return;

@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -280,6 +281,55 @@ int check_points(sgpu_ctx *ctx, int n, const sgpu_section_point *pts) {
     return SGPU_OK;
 }
 
+// Scratch of sgpu_section_eval_device_noalloc: the aux pre-pass results
+// ((sigma A, E_t A) per aux slot per point) in a file-scope device array, so
+// that entry point allocates nothing.  Each GPU the library runs on holds
+// its own copy for the life of the process; host code reaches a device's
+// copy only through the address hipGetSymbolAddress returns on that device
+// (the array's name is a host-side stand-in).
+constexpr int64_t kNoAllocScratchDoubles = int64_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Once per device, with that device current (sgpu_ctx_create).
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return SGPU_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return SGPU_OK;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return SGPU_ERR_HIP;
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+    return SGPU_OK;
+}
+
+double *noalloc_scratch(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+// Both passes of a section evaluation over device points, with the aux
+// results in `fsks` (n * max_aux * 2 doubles).  `timed`: per-kernel events
+// when profiling is on.
+int launch_section_passes(sgpu_ctx *ctx, int n, const sgpu_section_point *d_points, sgpu_section_result *d_results,
+                          double *fsks, bool timed) {
+    {
+        ProfScope ps(ctx, timed ? SGPU_KERNEL_AUX_EVAL : -1);
+        launch_aux_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c, ctx->t,
+                        d_points, n, fsks, ctx->max_aux);
+    }
+    {
+        ProfScope ps(ctx, timed ? SGPU_KERNEL_SECTION_EVAL : -1);
+        launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats,
+                            (int)ctx->mats.size(), ctx->c, ctx->t, d_points, d_results, n, fsks, ctx->max_aux);
+    }
+    HIP_OK(ctx, hipGetLastError());
+    return SGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -298,6 +348,11 @@ int sgpu_ctx_create(int device, sgpu_ctx **out) {
     sgpu_ctx *ctx = new sgpu_ctx();
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return SGPU_ERR_HIP;
+    }
+    if (noalloc_scratch_init(device) != SGPU_OK) {
+        (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return SGPU_ERR_HIP;
     }
@@ -546,19 +601,23 @@ int sgpu_section_eval_device(sgpu_ctx *ctx, int n, const sgpu_section_point *d_p
         HIP_OK(ctx, hipMalloc((void **)&ctx->d_fsks, sizeof(double) * need));
         ctx->fsks_cap = need;
     }
-    {
-        ProfScope ps(ctx, SGPU_KERNEL_AUX_EVAL);
-        launch_aux_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats, ctx->c, ctx->t,
-                        d_points, n, ctx->d_fsks, ctx->max_aux);
-    }
-    {
-        ProfScope ps(ctx, SGPU_KERNEL_SECTION_EVAL);
-        launch_section_eval(ctx->stream, ctx->d_defs, (int)ctx->defs.size(), ctx->tables(), ctx->d_mats,
-                            (int)ctx->mats.size(), ctx->c, ctx->t, d_points, d_results, n, ctx->d_fsks,
-                            ctx->max_aux);
-    }
-    HIP_OK(ctx, hipGetLastError());
-    return SGPU_OK;
+    return launch_section_passes(ctx, n, d_points, d_results, ctx->d_fsks, true);
+}
+
+int sgpu_section_eval_device_noalloc(sgpu_ctx *ctx, int n, const sgpu_section_point *d_points,
+                                     sgpu_section_result *d_results) {
+    if (!ctx) return SGPU_ERR_ARG;
+    if (!ctx->finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_sections_finalize has not run");
+    if (n < 0 || (n > 0 && (!d_points || !d_results))) return fail(ctx, SGPU_ERR_ARG, "bad point buffers");
+    if (n == 0) return SGPU_OK;
+    const int64_t need = (int64_t)n * ctx->max_aux * 2;
+    if (need > kNoAllocScratchDoubles)
+        return fail(ctx, SGPU_ERR_ARG,
+                    "%d points x %d aux slots need %lld scratch doubles; the fixed scratch holds %lld", n,
+                    ctx->max_aux, (long long)need, (long long)kNoAllocScratchDoubles);
+    double *fsks = noalloc_scratch(ctx->device);
+    if (!fsks) return fail(ctx, SGPU_ERR_STATE, "no scratch address for device %d", ctx->device);
+    return launch_section_passes(ctx, n, d_points, d_results, fsks, false);
 }
 
 int sgpu_section_eval(sgpu_ctx *ctx, int n, const sgpu_section_point *points, sgpu_section_result *results) {
