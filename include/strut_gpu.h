@@ -308,24 +308,29 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *desc, int32_t *h
 /* Run `steps` more steps (the step loop :1698-3110): u_hist[step * 6
  * n_nodes] the full displacement after each converged step (slaves
  * enforced) or NULL, iters[step] the Newton iterations, *status as above.
- * The band LU never pivots; *dense_fallbacks (may be NULL) counts the
- * factorizations where the reference would have moved to its dense
- * partial-pivot LU (linear_solver_backend.mojo:608-628) -- the solution is
- * the same within rounding. */
+ * The band LU makes the reference's row swaps (its dense partial-pivot
+ * fallback, linear_solver_backend.mojo:652-657, :175-214, gives the same
+ * factors as a band LU with partial pivoting); *dense_fallbacks (may be
+ * NULL) counts the factorizations that swapped rows, where the reference
+ * moved to its dense LU. */
 int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, int32_t *steps_done, int32_t *status,
                      int32_t *dense_fallbacks);
 
 /* BandGeneral factor and solve of one system on the device
  * (linear_solver_backend.mojo:631-679, :735-769): band n x (2 hbw + 1),
  * row major as the reference stores it (slot (i, j) at i (2 hbw + 1) + j - i
- * + hbw), no pivoting.  *info: 0 solved; 1 solved, and the reference would
- * have moved this factorization to its dense partial-pivot LU (:608-628);
- * 2 a pivot at or below 1e-14, nothing solved.  variant 0 picks the kernel
- * (the blocked one up to half bandwidth 119 and about 3300 rows, then the
- * register-window one up to 127, then the global-memory one), 1 the
- * global-memory kernel, 2 the register-window kernel, 3 the blocked kernel,
- * 4 the blocked kernel with 8-row blocks on 512 threads (half bandwidth up
- * to 115).  Host buffers; allocates. */
+ * + hbw).  Variants 0, 5 and 6 make the reference's row swaps (its band LU,
+ * and at the first needed swap its dense partial-pivot LU, :652-657,
+ * :175-214: the same factors as a band LU with partial pivoting whose U is
+ * 2 hbw wide); *info: 0 solved without a swap; 1 solved with swaps (where
+ * the reference took its dense fallback); 2 a column's largest magnitude
+ * at or below 1e-14 (the reference aborts), nothing solved.  0 picks the
+ * register kernel (half bandwidth <= 127, n <= 3072) or else the
+ * global-memory one, 5 the register kernel, 6 the global-memory one.
+ * Variants 1-4 and 7 eliminate without row swaps (*info 1: the reference
+ * would have swapped; the solution is the unpivoted one): 1 global memory,
+ * 2 register window, 3 blocked, 4 blocked with 8-row blocks on 512
+ * threads, 7 picked by size.  Host buffers; allocates. */
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info);
 
@@ -373,6 +378,43 @@ int sgpu_frame3d_envelopes_record(sgpu_ctx *ctx, int32_t record, int n_drift, co
  * with record 0's elements evaluated at its current displacement (a trial
  * state; before the first step: at rest). */
 int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band);
+
+/* The band solver of the 3D driver: 0 (default) the reference's row swaps;
+ * 1 the band elimination without swaps (an opt-in for comparison; it counts
+ * where the reference would have swapped, and its solution differs from the
+ * reference's there by rounding, or more on an indefinite tangent). */
+int sgpu_frame3d_set_band_solver(sgpu_ctx *ctx, int solver);
+
+/* ------------------------------------------------------------------
+ * A BandGeneral system with the reference's backend calls split
+ * (LinearSolverBackend, linear_solver_backend.mojo:11-92): create
+ * (initialize_structure / initialize_symbolic, :1664-1685, with the half
+ * bandwidth given), load (load_reduced_matrix :1877-1913 and
+ * add_reduced_matrix :1916-1981), factor (factorize_loaded :2174 ->
+ * :631-679 with the dense fallback :652-657) and solve (:2277-2306 ->
+ * :735-769, :216-232), so one factorization serves several right-hand
+ * sides.  Device-resident; calls on one band are not thread-safe.
+ * ------------------------------------------------------------------ */
+typedef struct sgpu_band sgpu_band;
+int sgpu_band_create(sgpu_ctx *ctx, int n, int hbw, sgpu_band **out);
+int sgpu_band_destroy(sgpu_band *b);
+/* layout 0: n x (2 hbw + 1) band values; 1: a dense n x n row-major matrix
+ * whose entries outside the band are dropped (:413-426).  add 0: the band
+ * becomes scale * values; add 1: scale * values is added to the loaded
+ * values.  Host buffer, synchronous. */
+int sgpu_band_load(sgpu_band *b, int layout, const double *values, double scale, int add);
+/* The same from device band values (layout 0), asynchronous. */
+int sgpu_band_load_device(sgpu_band *b, const double *d_values, double scale, int add);
+/* *info as sgpu_band_solve; the loaded values become the factors. */
+int sgpu_band_factor(sgpu_band *b, int32_t *info);
+/* x = A^-1 rhs with the last factorization; host buffers, synchronous. */
+int sgpu_band_solve_factored(sgpu_band *b, const double *rhs, double *x);
+/* The last factorization, each buffer may be NULL: u_rows n x (2 hbw + 1),
+ * U(k, k + d) at k (2 hbw + 1) + d (the reference's U rows, which reach
+ * 2 hbw past the diagonal after its dense fallback); l_cols n x hbw, the
+ * multiplier of step k for the row at position i (after step k's swap) at
+ * k hbw + i - k - 1; piv[k] the position swapped with k at step k. */
+int sgpu_band_factors(sgpu_band *b, double *u_rows, double *l_cols, int32_t *piv);
 
 /* ------------------------------------------------------------------
  * Transient nonlinear analysis of 2D forceBeamColumn fiber frames.

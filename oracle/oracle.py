@@ -501,6 +501,20 @@ def band_solve(band: np.ndarray, hbw: int, rhs: np.ndarray):
     return bool(ok), x, bool(fb.value)
 
 
+def band_factor_rows(band: np.ndarray, hbw: int):
+    """U rows of the oracle's BandGeneral factorization, n x (2 hbw + 1) with
+    the diagonal first.  Returns (ok, U, dense_fallback)."""
+    lib = load()
+    lib.or_band_factor_rows.restype = c_int
+    lib.or_band_factor_rows.argtypes = [c_int, c_int, POINTER(c_double), POINTER(c_double), POINTER(c_int)]
+    b = np.ascontiguousarray(band, dtype=np.float64)
+    n = b.shape[0]
+    u = np.zeros((n, 2 * hbw + 1))
+    fb = c_int()
+    ok = lib.or_band_factor_rows(n, hbw, _d_out(b), _d_out(u), ctypes.byref(fb))
+    return bool(ok), u, bool(fb.value)
+
+
 def path_eval(values, dt: float, t: float, factor: float = 1.0, start_time: float = 0.0,
               use_last: bool = False) -> float:
     """The oracle's Path time series (time_series.mojo:71-96) at time t."""

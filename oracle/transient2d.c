@@ -225,6 +225,40 @@ int or_band_solve(int n, int hbw, const double *band, const double *rhs, double 
     return ok;
 }
 
+/* Test entry: the factorization's U rows, n x (2 hbw + 1) with the diagonal
+ * first (U(k, k + d) at k (2 hbw + 1) + d): from the band when the band LU
+ * ran to the end (d <= hbw), from the dense LU's rows when it fell back
+ * (lsb.mojo:652-657, :175-214; d <= 2 hbw).  Returns 1 on success, 0 on a
+ * pivot below the tolerance. */
+int or_band_factor_rows(int n, int hbw, const double *band, double *u_rows, int *dense_fallback) {
+    band_t b;
+    memset(&b, 0, sizeof(b));
+    b.n = n;
+    b.hbw = hbw;
+    b.w = 2 * hbw + 1;
+    b.band = malloc(sizeof(double) * (size_t)n * b.w);
+    b.dense = calloc((size_t)n * n, sizeof(double));
+    b.piv = calloc((size_t)n, sizeof(int));
+    b.work = calloc((size_t)n, sizeof(double));
+    memcpy(b.band, band, sizeof(double) * (size_t)n * b.w);
+    int ok = band_factor(&b);
+    memset(u_rows, 0, sizeof(double) * (size_t)n * b.w);
+    if (ok)
+        for (int k = 0; k < n; ++k)
+            for (int d = 0; d < b.w && k + d < n; ++d) {
+                if (b.dense_fallback)
+                    u_rows[(size_t)k * b.w + d] = b.dense[(size_t)k * n + k + d];
+                else if (d <= hbw)
+                    u_rows[(size_t)k * b.w + d] = b.band[band_slot(hbw, k, k + d)];
+            }
+    if (dense_fallback) *dense_fallback = b.dense_fallback;
+    free(b.band);
+    free(b.dense);
+    free(b.piv);
+    free(b.work);
+    return ok;
+}
+
 /* Test entry: the Path series of a case record evaluated at t. */
 double or_path_eval_raw(int n, const double *values, double dt, double factor, double start_time, int use_last,
                         double t) {

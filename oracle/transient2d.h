@@ -42,6 +42,7 @@ double or_path_eval(const or_tr2d_case *c, double t);
 double or_path_eval_raw(int n, const double *values, double dt, double factor, double start_time, int use_last,
                         double t);
 int or_band_solve(int n, int hbw, const double *band, const double *rhs, double *x, int *dense_fallback);
+int or_band_factor_rows(int n, int hbw, const double *band, double *u_rows, int *dense_fallback);
 int or_tr2d_run(const or_tr2d_case *c, or_tr2d_out *out);
 
 #endif

@@ -100,6 +100,14 @@ EXPORTED = (
     "sgpu_frame3d_run_records",
     "sgpu_frame3d_recorded_record",
     "sgpu_frame3d_envelopes_record",
+    "sgpu_frame3d_set_band_solver",
+    "sgpu_band_create",
+    "sgpu_band_destroy",
+    "sgpu_band_load",
+    "sgpu_band_load_device",
+    "sgpu_band_factor",
+    "sgpu_band_solve_factored",
+    "sgpu_band_factors",
 )
 
 # Kernel ids of sgpu_profile_read.

@@ -22,10 +22,10 @@
 //   * C v in the residual is alphaM M v + element-level (k_hat v) and
 //     (k_hat_init v) gathered per DOF -- the full matrices, as the
 //     reference's dense C_ff row dot products use;
-//   * the band LU (no pivoting) and both substitutions run in one
-//     workgroup; where the reference would switch to its dense partial-pivot
-//     LU (a sub-diagonal entry outweighs the pivot) the band elimination
-//     continues, and the run counts those factorizations.
+//   * the band LU and both substitutions run in one workgroup per record
+//     with the reference's row swaps (band_lu.hip): where the reference
+//     moves to its dense partial-pivot LU, the band LU pivots and gives the
+//     same factors, and the run counts those factorizations.
 // The host loop launches the kernels of a Newton iteration and reads back
 // one residual norm per iteration.
 #include <hip/hip_runtime.h>
@@ -39,6 +39,13 @@
 #include "ctx.h"
 
 namespace sgpu {
+
+// band_lu.hip: the factorization with the reference's row swaps
+int launch_band_piv(sgpu_ctx *ctx, int mode, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
+                    double *d_status, double *d_L, int32_t *d_ipiv, double *d_work, const int32_t *recs,
+                    int count);
+bool band_piv_reg_fits(int n, int hbw);
+size_t band_piv_work_doubles(int n, int hbw);
 
 constexpr int kF3MaxRet = 12;       // free retained DOFs one element reaches
 constexpr int kF3SolveThreads = 1024;
@@ -76,6 +83,8 @@ struct Frame3dHost {
            *d_fhat = nullptr, *d_yk = nullptr, *d_yi = nullptr;
     int32_t *d_status = nullptr;
     double *d_band = nullptr;
+    double *d_bwork = nullptr;  // R x band_piv_work_doubles when the register solve cannot hold the band
+    int solver = 0;             // 0: the reference's row swaps (band LU / dense fallback); 1: no row swaps
     double *d_scalar = nullptr;  // [0] residual norm^2, [1] max element status, [2] solve status
     // recorder: chosen DOFs after every converged step, kept on the device
     int n_rec = 0, rec_cap = 0;
@@ -890,7 +899,11 @@ double path_at(const std::vector<double> &v, double dt, double factor, double t)
 // recs == nullptr: one system at the given pointers; else `count` systems,
 // record r's band, right-hand side, solution and status at its slices.
 int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
-                      double *d_status, const int32_t *recs = nullptr, int count = 1) {
+                      double *d_status, const int32_t *recs = nullptr, int count = 1, double *d_work = nullptr) {
+    if (variant == 0 || variant == 5 || variant == 6)  // the reference's row swaps
+        return launch_band_piv(ctx, variant == 0 ? 0 : variant - 4, n, hbw, d_band, d_rhs, d_x, d_status, nullptr,
+                               nullptr, d_work, recs, count);
+    if (variant == 7) variant = 0;  // no row swaps, kernel picked by size
     const size_t blk_lds = sizeof(double) * ((size_t)kF3BlkRing + n);
     const int P = hbw + 1, Bc = (P + 3) / 4;
     const bool fits8 = ((P + 7) / 8) * Bc <= 512 - 64, fits4 = ((P + 3) / 4) * Bc <= 1024 - 64;
@@ -933,18 +946,20 @@ using namespace sgpu;
 
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info) {
-    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 4)
+    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 7)
         return SGPU_ERR_ARG;
     HIP_OK(ctx, hipSetDevice(ctx->device));
     const size_t slots = (size_t)n * (2 * hbw + 1);
+    const bool global_piv = variant == 6 || (variant == 0 && !band_piv_reg_fits(n, hbw));
+    const size_t work = global_piv ? band_piv_work_doubles(n, hbw) : 0;
     double *d = nullptr;
-    HIP_OK(ctx, hipMalloc((void **)&d, sizeof(double) * (slots + 2 * (size_t)n + 1)));
-    double *d_rhs = d + slots, *d_x = d_rhs + n, *d_st = d_x + n;
+    HIP_OK(ctx, hipMalloc((void **)&d, sizeof(double) * (slots + 2 * (size_t)n + 4 + work)));
+    double *d_rhs = d + slots, *d_x = d_rhs + n, *d_st = d_x + n, *d_work = global_piv ? d_st + 4 : nullptr;
     int rc = SGPU_OK;
     if (hipMemcpy(d, band, sizeof(double) * slots, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_rhs, rhs, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
         rc = fail(ctx, SGPU_ERR_HIP, "band upload failed");
-    } else if ((rc = launch_band_solve(ctx, variant, n, hbw, d, d_rhs, d_x, d_st)) == SGPU_OK) {
+    } else if ((rc = launch_band_solve(ctx, variant, n, hbw, d, d_rhs, d_x, d_st, nullptr, 1, d_work)) == SGPU_OK) {
         double st = 0.0;
         if (hipMemcpyAsync(x, d_x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             hipMemcpyAsync(&st, d_st, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
@@ -1213,6 +1228,8 @@ int f3_allocate(sgpu_ctx *ctx, Frame3dHost &h) {
         (rc = dalloc(ctx, &h.d_list_ids, R * n_el)) || (rc = dalloc(ctx, &h.d_row_of, R)) ||
         (rc = dalloc(ctx, &h.d_ag, 3 * R)))
         return rc;
+    if (!band_piv_reg_fits(h.n, h.hbw) && (rc = dalloc(ctx, &h.d_bwork, R * band_piv_work_doubles(h.n, h.hbw))))
+        return rc;
     for (double **p : vecs) HIP_OK(ctx, hipMemset(*p, 0, sizeof(double) * R * n));
     HIP_OK(ctx, hipMemset(h.d_u, 0, sizeof(double) * R * h.total));
     HIP_OK(ctx, hipMemset(h.d_kinit_hat, 0, sizeof(double) * R * n_el * 144));
@@ -1350,7 +1367,8 @@ int f3_run(sgpu_ctx *ctx, Frame3dHost &h, int steps, double *u_hist, int32_t *it
                                h.d_kinit_band, h.d_band, 0, a0, a1, h.alpha_m, h.beta_k, h.beta_k_init, has_damping);
             {
                 ProfScope ps(ctx, SGPU_KERNEL_F3_SOLVE);
-                if ((rc = launch_band_solve(ctx, 0, n, h.hbw, h.d_band, h.d_R, h.d_du, h.d_scalar + 2, h.d_list, nu)))
+                if ((rc = launch_band_solve(ctx, h.solver == 0 ? 0 : 7, n, h.hbw, h.d_band, h.d_R, h.d_du,
+                                            h.d_scalar + 2, h.d_list, nu, h.d_bwork)))
                     return rc;
             }
             hipLaunchKernelGGL(f3_update_kernel, grid2(n, nu), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_du);
@@ -1440,6 +1458,13 @@ int sgpu_frame3d_run_records(sgpu_ctx *ctx, int steps, int32_t *iters, int32_t *
         status[r] = h.rec_status[r];
     }
     return rc;
+}
+
+int sgpu_frame3d_set_band_solver(sgpu_ctx *ctx, int solver) {
+    if (!ctx || (solver != 0 && solver != 1)) return SGPU_ERR_ARG;
+    if (!ctx->frame3d) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
+    ctx->frame3d->solver = solver;
+    return SGPU_OK;
 }
 
 int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band_out) {

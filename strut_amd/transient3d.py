@@ -70,6 +70,8 @@ def _declare(lib):
     lib.sgpu_frame3d_envelopes_record.restype = c_int
     lib.sgpu_frame3d_envelopes_record.argtypes = [c_void_p, c_int32, c_int, POINTER(c_int32), POINTER(c_int32),
                                                   POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+    lib.sgpu_frame3d_set_band_solver.restype = c_int
+    lib.sgpu_frame3d_set_band_solver.argtypes = [c_void_p, c_int]
     lib._f3_declared = True
 
 
@@ -78,10 +80,12 @@ class Frame3dRun:
     free map as the reference does; 1 keeps every coupling of the collapsed
     system."""
 
-    def __init__(self, fc: Frame3dCase, device: int = 0, hbw_rule: int = 0, records=None):
+    def __init__(self, fc: Frame3dCase, device: int = 0, hbw_rule: int = 0, records=None, band_solver: int = 0):
         """records: further ground-motion records of the same frame, each a
         list of Pattern (one per pattern of the case, same directions); the
-        case's own patterns are record 0."""
+        case's own patterns are record 0.  band_solver 0 makes the
+        reference's row swaps (its band LU and dense partial-pivot
+        fallback); 1 eliminates without swaps (for comparison)."""
         self.fc = fc
         self.rt = SectionRuntime(fc.materials, device=device)
         _declare(self.rt.lib)
@@ -133,6 +137,7 @@ class Frame3dRun:
         self.rt._check(self.rt.lib.sgpu_frame3d_create(self.rt.ctx, ctypes.byref(d), ctypes.byref(hbw),
                                                        ctypes.byref(nf)))
         self.hbw, self.free_count = hbw.value, nf.value
+        self.rt._check(self.rt.lib.sgpu_frame3d_set_band_solver(self.rt.ctx, band_solver))
         for rec_ids, rec in zip(all_ids[1:], extra):
             vals = [np.ascontiguousarray(p.values, dtype=np.float64) for p in rec]
             self._keep.setdefault("rec", []).append((rec_ids, vals))

@@ -31,7 +31,7 @@ SCHEDULES = [_native.SCHEDULE_ROUNDS, _native.SCHEDULE_PERSISTENT, _native.SCHED
 
 @pytest.mark.parametrize("schedule", SCHEDULES, ids=["rounds", "persistent", "queue", "auto"])
 def test_gpu_config0_plain_matches_oracle(schedule):
-    fc = load_frame2d_case(cases.config0_case(steps=400, seed=0))
+    fc = load_frame2d_case(cases.config0_case(steps=1000, seed=0))
     ref = oracle.run_transient2d(fc)
     assert ref["status"] == 0
     fb = FrameBatch(fc.materials)
@@ -42,8 +42,8 @@ def test_gpu_config0_plain_matches_oracle(schedule):
     fb.run(rounds=64)
     got = fb.result(i)
     fb.close()
-    print(f"config0 400 steps: status {got['status']}, iterations {got['iters'].sum()} vs {ref['iters'].sum()}")
-    assert got["status"] == 0 and got["steps_done"] == 400
+    print(f"config0 1000 steps: status {got['status']}, iterations {got['iters'].sum()} vs {ref['iters'].sum()}")
+    assert got["status"] == 0 and got["steps_done"] == 1000
     np.testing.assert_array_equal(got["iters"], ref["iters"])
     err = _rel_err(got["u"], ref["u"])
     print(f"config0 max relative displacement error {err:.3e}")

@@ -4,12 +4,13 @@ seeded cases: RC frames with forceBeamColumn3d columns and dispBeamColumn3d
 beams, rigid floor diaphragms under the Transformation handler and two
 simultaneous horizontal records.  The bar: the same status, the same Newton
 iteration count at every step, and every displacement within 1e-8 of the
-history's peak.  Where the reference's band LU hands a factorization to its
-dense partial-pivot LU the device keeps the band elimination (the frames'
-effective tangents are symmetric with a dominant diagonal): the solutions
-agree within rounding, which the displacement bar checks.  Both band rules run: the reference's (element free map,
-whose truncation of the collapsed tangent costs extra iterations) and the
-full collapsed band."""
+history's peak.  The device's band LU makes the reference's row swaps
+(where the reference hands a factorization to its dense partial-pivot LU,
+the device pivots within the band and gets the same factors), and the count
+of such factorizations must equal the oracle's; the elimination without
+swaps, kept as an opt-in, is checked on one frame too.  Both band rules
+run: the reference's (element free map, whose truncation of the collapsed
+tangent costs extra iterations) and the full collapsed band."""
 
 import numpy as np
 import pytest
@@ -22,9 +23,9 @@ from strut_amd.transient3d import Frame3dRun
 pytestmark = pytest.mark.gpu
 
 
-def _compare(fc, hbw_rule, steps):
+def _compare(fc, hbw_rule, steps, band_solver=0):
     ref = oracle.run_transient3d(fc, hbw_rule=hbw_rule, steps=steps)
-    dev = Frame3dRun(fc, hbw_rule=hbw_rule)
+    dev = Frame3dRun(fc, hbw_rule=hbw_rule, band_solver=band_solver)
     assert dev.hbw == ref["hbw"] and dev.free_count == ref["free_count"]
     got = dev.run(steps)
     dev.close()
@@ -33,6 +34,7 @@ def _compare(fc, hbw_rule, steps):
           f"{ref['dense_fallbacks']}", flush=True)
     assert got["status"] == ref["status"]
     assert got["steps_done"] == ref["steps_done"]
+    assert got["dense_fallbacks"] == ref["dense_fallbacks"]
     k = ref["steps_done"]
     np.testing.assert_array_equal(got["iters"][:k], ref["iters"][:k])
     scale = np.abs(ref["u"][:k]).max()
@@ -70,6 +72,26 @@ def test_gpu_frame_without_constraints_matches_oracle():
     case = cases.rc_frame3d_case(2, 1, 1, steps=60, dt=0.005, n_core=4, n_bars=4, diaphragms=False,
                                  bidirectional=False, pgas_g=(0.6, 0.0))
     _compare(load_frame3d_case(case), 0, 60)
+
+
+def test_gpu_band_elimination_without_swaps_opt_in_matches_oracle():
+    """The opt-in solver that eliminates without row swaps (it counts where
+    the reference would swap): on this frame's symmetric, diagonally strong
+    tangents it follows the oracle within the displacement bar."""
+    case = cases.rc_frame3d_case(2, 2, 1, steps=60, dt=0.005, n_core=4, n_bars=4, pgas_g=(0.5, 0.35))
+    _compare(load_frame3d_case(case), 0, 60, band_solver=1)
+
+
+def test_gpu_config1_full_size_first_steps_match_oracle():
+    """Configuration 1 at its own size (20 stories x 5 x 5 bays, 1920
+    elements, 268-fiber sections, rigid diaphragms, both records, the
+    reference's band rule: 2220 free DOFs, half bandwidth 110): the first
+    three steps against the oracle's driver (dense K, collapse, band LU with
+    the dense fallback), identical iteration counts, dense-fallback count
+    and displacements within 1e-8 of the peak."""
+    fc = load_frame3d_case(cases.config1_case(steps=3))
+    ref = _compare(fc, 0, 3)
+    assert ref["hbw"] == 110 and ref["free_count"] == 2220 and ref["status"] == 0
 
 
 def _spd_band(n, hbw, seed, dominant):
@@ -191,3 +213,63 @@ def test_gpu_records_run_together_match_each_oracle_run():
         _, want = oracle.envelopes(hist, k, fi, fj, dx)
         np.testing.assert_array_equal(drift, want)
     run.close()
+
+
+def test_gpu_records_that_stop_early_leave_the_others_unchanged():
+    """Four records together, two of them scaled until they stop (the
+    reference aborts on an element's local failure, and on Newton
+    non-convergence).  Every record gives the same bits as the same record
+    run alone on the device (statuses, steps done, iteration counts and
+    recorded displacements), so a record that stops leaves the live list
+    without disturbing the others' lockstep steps, commits or recorder rows.
+    Against the oracle's run of each record alone: the same status and stop
+    step for all four; identical counts and displacements within 1e-8 for
+    the three that do not end in stalled Newton steps.  The record that
+    stops on Newton non-convergence (scaled 40x) drifts from the oracle by
+    about 3e-5 in its last steps, which depend on the last ulp of libm's
+    pow in the reference itself."""
+    from strut_amd.frame3d import Pattern
+
+    steps = 60
+    base = cases.rc_frame3d_case(2, 1, 1, steps=steps, dt=0.005, n_core=4, n_bars=4, pgas_g=(0.5, 0.35))
+    fc = load_frame3d_case(base)
+    recs, fcs = [], []
+    # scaled 60x: an element's local failure at step 53; 40x: Newton
+    # non-convergence at step 59 (the oracle's runs)
+    for seeds, pgas, scale in (((5, 6), (0.6, 0.3), 60.0), ((7, 8), (0.2, 0.45), 1.0), ((9, 10), (0.6, 0.3), 40.0)):
+        c = cases.rc_frame3d_case(2, 1, 1, steps=steps, dt=0.005, n_core=4, n_bars=4, seeds=seeds, pgas_g=pgas)
+        f = load_frame3d_case(c)
+        f.patterns = [Pattern(p.direction, p.values, p.dt, p.factor * scale) for p in f.patterns]
+        fcs.append(f)
+        recs.append(list(f.patterns))
+    full = np.arange(fc.total_dofs, dtype=np.int32)
+    run = Frame3dRun(fc, records=recs)
+    run.set_recorder(full, steps)
+    got = run.run_records(steps)
+    hists = [run.recorded(r) for r in range(4)]
+    run.close()
+    stopped = 0
+    for r, f in enumerate([fc] + fcs):
+        alone = Frame3dRun(f)
+        alone.set_recorder(full, steps)
+        one = alone.run(steps, record_history=False)
+        one_hist = alone.recorded(0)
+        alone.close()
+        ref = oracle.run_transient3d(f)
+        k = ref["steps_done"]
+        print(f"record {r}: status {got['status'][r]}/{one['status']}/{ref['status']} steps "
+              f"{got['steps_done'][r]}/{one['steps_done']}/{k} iters {got['iters'][r].sum()}/{one['iters'].sum()}/"
+              f"{ref['iters'].sum()}", flush=True)
+        assert got["status"][r] == one["status"] and got["steps_done"][r] == one["steps_done"]
+        np.testing.assert_array_equal(got["iters"][r], one["iters"])
+        np.testing.assert_array_equal(hists[r], one_hist)
+        assert got["status"][r] == ref["status"] and got["steps_done"][r] == k
+        stopped += ref["status"] != 0
+        if ref["status"] > 0:
+            continue  # ends in stalled Newton steps: the same stop is the bar
+        n_it = k  # an element failure's last step has no oracle count
+        np.testing.assert_array_equal(got["iters"][r][:n_it], ref["iters"][:n_it])
+        if k:
+            scale = np.abs(ref["u"][:k]).max()
+            assert np.abs(hists[r] - ref["u"][:k]).max() <= 1e-8 * scale
+    assert stopped == 2 and got["status"][1] < 0 and got["status"][3] > 0
