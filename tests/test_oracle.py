@@ -102,9 +102,19 @@ def test_concrete02_unload_reload_returns_to_envelope():
     path = list(-np.linspace(1e-4, 0.004, 20)) + list(-np.linspace(0.0039, 0.001, 10)) + \
         list(-np.linspace(0.0011, 0.0045, 20))
     sig, tan = oracle.material_path(CONCRETE02, p, np.array(path))
-    # first unloading step starts with the initial modulus or the clamp slopes
+    # the first unloading step: the elastic step from the envelope point,
+    # held below the line 0.5 er (eps - ept) through the zero-stress strain
+    # of the secant toward the focal point (the full closed form and its
+    # planted slips: tests/test_oracle_hysteresis.py)
+    fcu, epscu, rat = p[2], p[3], p[4]
     ec0 = 2 * fc / epsc0
-    assert tan[20] in (ec0,) or tan[20] > 0
+    eps_f = (fcu - rat * ec0 * epscu) / (ec0 - rat * ec0)
+    er = (sig[19] - ec0 * eps_f) / (path[19] - eps_f)
+    ept = path[19] - sig[19] / er
+    trial = sig[19] + ec0 * (path[20] - path[19])
+    want = min(trial, 0.5 * er * (path[20] - ept))
+    assert sig[20] == pytest.approx(want, rel=1e-13)
+    assert tan[20] == (ec0 if trial < 0.5 * er * (path[20] - ept) else 0.5 * er)
     # reloading past the previous minimum rejoins the envelope
     s_env, _ = oracle.material_path(CONCRETE02, p, np.array([path[-1]]))
     assert sig[-1] == s_env[0]
