@@ -12,11 +12,13 @@
 #include "../../include/strut_gpu.h"
 #include "strut_gpu_types.h"
 
+struct sgpu_ctx;
 namespace sgpu {
 struct FrameHost;
 void frame2d_host_free(FrameHost *fh);
 struct Elem3dHost;
 void elem3d_host_free(Elem3dHost *h);
+int elem3d_count(const sgpu_ctx *ctx);  // sgpu_elem3d elements of the context
 struct MpcHost;
 void mpc_host_free(MpcHost *h);
 struct Frame3dHost;

@@ -25,6 +25,8 @@ struct Elem3dHost {
     double *d_u = nullptr, *d_k = nullptr, *d_f = nullptr;
 };
 
+int elem3d_count(const sgpu_ctx *ctx) { return ctx && ctx->elems3d ? (int)ctx->elems3d->elems.size() : 0; }
+
 void elem3d_host_free(Elem3dHost *h) {
     if (!h) return;
     for (void *p : {(void *)h->d_ids, (void *)h->d_status, (void *)h->d_u, (void *)h->d_k, (void *)h->d_f})

@@ -62,6 +62,8 @@ struct Frame3dHost {
     std::vector<std::vector<double>> rec_ts;           // R x n_patterns series
     std::vector<double> rec_ts_dt, rec_ts_factor;      // R x n_patterns
     std::vector<int32_t> rec_steps, rec_status;        // completed steps; 0 running, > 0 stopped, < 0 failed
+    std::vector<uint8_t> elem_used;                    // per sgpu_elem3d element: claimed by a record
+    std::vector<int32_t> list_on_device;               // the record list the kernels read now
     bool allocated = false, initialized = false;
     int32_t *d_list = nullptr, *d_list_ids = nullptr, *d_row_of = nullptr;
     double *d_ag = nullptr;  // R x 3 ground accelerations of the step
@@ -898,6 +900,24 @@ double path_at(const std::vector<double> &v, double dt, double factor, double t)
 // half bandwidth fits it (variant 0 or 2), else the global-memory one.
 // recs == nullptr: one system at the given pointers; else `count` systems,
 // record r's band, right-hand side, solution and status at its slices.
+// A record's element list: every id an sgpu_elem3d element, none named twice
+// within the list or by an earlier record (two workgroups writing one
+// element's section states would race).  Claims the ids on success.
+int claim_elements(sgpu_ctx *ctx, Frame3dHost &h, const int32_t *ids, int n_el) {
+    const int count = elem3d_count(ctx);
+    if ((int)h.elem_used.size() != count) h.elem_used.resize(count, 0);
+    std::vector<uint8_t> mine(count, 0);
+    for (int e = 0; e < n_el; ++e) {
+        const int id = ids[e];
+        if (id < 0 || id >= count) return fail(ctx, SGPU_ERR_ARG, "element %d: id %d is not an sgpu_elem3d element", e, id);
+        if (mine[id] || h.elem_used[id])
+            return fail(ctx, SGPU_ERR_ARG, "element %d: id %d is already used by this or another record", e, id);
+        mine[id] = 1;
+    }
+    for (int i = 0; i < count; ++i) h.elem_used[i] |= mine[i];
+    return SGPU_OK;
+}
+
 int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
                       double *d_status, const int32_t *recs = nullptr, int count = 1, double *d_work = nullptr) {
     if (variant == 0 || variant == 5 || variant == 6)  // the reference's row swaps
@@ -994,6 +1014,11 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
         for (int a = 0; a < 2; ++a)
             if (d->elem_nodes[2 * e + a] < 0 || d->elem_nodes[2 * e + a] >= nn)
                 return fail(ctx, SGPU_ERR_ARG, "element %d: node index out of range", e);
+    {  // record 0's element ids, checked before anything is built (claimed below)
+        Frame3dHost probe;
+        int rc0 = claim_elements(ctx, probe, d->elem_ids, n_el);
+        if (rc0) return rc0;
+    }
     // loader.mojo:2378-2388, :2443-2450: free DOFs in plain order, slaves out
     std::vector<int32_t> free_idx(total, -1), free_list;
     for (int i = 0; i < total; ++i) {
@@ -1195,6 +1220,8 @@ int sgpu_frame3d_add_record(sgpu_ctx *ctx, const int32_t *elem_ids, const double
     for (int p = 0; p < h->n_patterns; ++p)
         if (!ts_values || !ts_values[p] || !ts_n || ts_n[p] <= 0 || !ts_dt || ts_dt[p] <= 0.0 || !ts_factor)
             return fail(ctx, SGPU_ERR_ARG, "pattern %d: bad Path series", p);
+    int rc = claim_elements(ctx, *h, elem_ids, h->n_el);
+    if (rc) return rc;
     h->rec_elem_ids.insert(h->rec_elem_ids.end(), elem_ids, elem_ids + h->n_el);
     for (int p = 0; p < h->n_patterns; ++p) {
         h->rec_ts.emplace_back(ts_values[p], ts_values[p] + ts_n[p]);
@@ -1241,6 +1268,7 @@ int f3_allocate(sgpu_ctx *ctx, Frame3dHost &h) {
 // the listed records become the kernels' record list (and their element
 // ids, in list order, the element pass's)
 int f3_set_list(sgpu_ctx *ctx, Frame3dHost &h, const std::vector<int32_t> &list) {
+    if (list == h.list_on_device) return SGPU_OK;  // already the kernels' list
     std::vector<int32_t> ids;
     ids.reserve(list.size() * h.n_el);
     for (int r : list) ids.insert(ids.end(), h.rec_elem_ids.begin() + (size_t)r * h.n_el,
@@ -1250,6 +1278,7 @@ int f3_set_list(sgpu_ctx *ctx, Frame3dHost &h, const std::vector<int32_t> &list)
     HIP_OK(ctx, hipMemcpyAsync(h.d_list_ids, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice,
                                ctx->stream));
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
+    h.list_on_device = list;
     return SGPU_OK;
 }
 

@@ -112,6 +112,8 @@ def load_frame2d_case(case: dict, steps: int | None = None) -> Frame2dCase:
                          ("test_type", ("NormUnbalance",))):
         if an.get(key, allowed[0]) not in allowed:
             raise ValueError(f"analysis {key} {an.get(key)} is outside the fiber-frame path")
+    if "solver_chain" in an:  # the reference retries steps over the chain (transient_nonlinear.mojo:758-827)
+        raise ValueError("analysis solver_chain is outside the fiber-frame path")
     integ = an.get("integrator", {})
     ray = case.get("rayleigh", {})
     if float(ray.get("betaKComm", 0.0)) != 0.0:

@@ -142,6 +142,8 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
                          ("algorithm", ("Newton",)), ("test_type", ("NormUnbalance",))):
         if an.get(key, allowed[0]) not in allowed:
             raise ValueError(f"analysis {key} {an.get(key)} is outside the 3D fiber-frame path")
+    if "solver_chain" in an:  # the reference retries steps over the chain (transient_nonlinear.mojo:758-827)
+        raise ValueError("analysis solver_chain is outside the 3D fiber-frame path")
     mps = case.get("mp_constraints", [])
     if mps and an.get("constraints", "Plain") != "Transformation":
         raise ValueError("mp_constraints need the Transformation constraint handler on this path")

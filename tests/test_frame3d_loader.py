@@ -67,6 +67,20 @@ def test_unsupported_inputs_raise():
     bad["recorders"].append({"type": "drift", "i_node": 1, "j_node": 2, "dof": 1, "perp_dirn": 3})  # same z
     with pytest.raises(ValueError, match="denominator"):
         load_frame3d_case(bad)
+    bad = copy.deepcopy(c)
+    bad["analysis"]["solver_chain"] = [{"algorithm": "Newton"}, {"algorithm": "ModifiedNewton"}]
+    with pytest.raises(ValueError, match="solver_chain"):
+        load_frame3d_case(bad)
+
+
+def test_2d_loader_refuses_a_solver_chain():
+    from strut_amd import cases
+    from strut_amd.frame2d import load_frame2d_case
+
+    c = cases.config0_case(steps=100)
+    c["analysis"]["solver_chain"] = [{"algorithm": "Newton"}]
+    with pytest.raises(ValueError, match="solver_chain"):
+        load_frame2d_case(c)
 
 
 def test_one_pattern_is_the_reference_schema():

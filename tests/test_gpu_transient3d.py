@@ -166,6 +166,20 @@ def test_gpu_frame3d_entry_points_refuse_bad_calls():
     d.n_nodes = 1
     with pytest.raises(StrutGpuError, match="already holds"):
         run.rt._check(run.rt.lib.sgpu_frame3d_create(run.rt.ctx, ctypes.byref(d), None, None))
+    # a record naming elements record 0 owns, or ids that are no elements
+    from ctypes import POINTER, c_double, c_int32
+    from strut_amd._native import dptr, iptr
+    ts = [np.zeros(10) for _ in fc.patterns]
+    tv = (POINTER(c_double) * 3)(*[dptr(v) for v in ts])
+    tn = (c_int32 * 3)(*[10] * len(ts))
+    tdt = (c_double * 3)(*[0.005] * len(ts))
+    tf = (c_double * 3)(*[1.0] * len(ts))
+    out = c_int32()
+    for ids, msg in ((np.array(run._keep["ids"], dtype=np.int32), "already used"),
+                     (np.full(fc.n_elem, 10**6, dtype=np.int32), "not an sgpu_elem3d element")):
+        with pytest.raises(StrutGpuError, match=msg):
+            run.rt._check(run.rt.lib.sgpu_frame3d_add_record(run.rt.ctx, iptr(ids), tv, tn, tdt, tf,
+                                                             ctypes.byref(out)))
     run.set_recorder([0, 6], 4)
     with pytest.raises(StrutGpuError, match="already set"):
         run.set_recorder([0], 4)
