@@ -114,6 +114,9 @@ def load_frame2d_case(case: dict, steps: int | None = None) -> Frame2dCase:
             raise ValueError(f"analysis {key} {an.get(key)} is outside the fiber-frame path")
     if "solver_chain" in an:  # the reference retries steps over the chain (transient_nonlinear.mojo:758-827)
         raise ValueError("analysis solver_chain is outside the fiber-frame path")
+    for key in ("loads", "element_loads", "dampings"):  # nodal / element loads and damping records
+        if case.get(key):
+            raise ValueError(f"{key} are outside the fiber-frame path")
     integ = an.get("integrator", {})
     ray = case.get("rayleigh", {})
     if float(ray.get("betaKComm", 0.0)) != 0.0:

@@ -144,6 +144,9 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
             raise ValueError(f"analysis {key} {an.get(key)} is outside the 3D fiber-frame path")
     if "solver_chain" in an:  # the reference retries steps over the chain (transient_nonlinear.mojo:758-827)
         raise ValueError("analysis solver_chain is outside the 3D fiber-frame path")
+    for key in ("loads", "element_loads", "dampings"):  # nodal / element loads and damping records
+        if case.get(key):
+            raise ValueError(f"{key} are outside the 3D fiber-frame path")
     mps = case.get("mp_constraints", [])
     if mps and an.get("constraints", "Plain") != "Transformation":
         raise ValueError("mp_constraints need the Transformation constraint handler on this path")
