@@ -16,6 +16,7 @@ edge is 40 mm.
 
 from __future__ import annotations
 
+import cmath
 import math
 from typing import Dict, List, Tuple
 
@@ -147,21 +148,85 @@ def gaussian_noise(seed: int, n: int):
     return out
 
 
+def butterworth_bandpass_sos(order: int, f_lo: float, f_hi: float, fs: float):
+    """Digital Butterworth band-pass of the given prototype order as
+    second-order sections (b0, b1, b2, a1, a2), a0 = 1: the analog
+    prototype's left-half-plane poles exp(i pi (2k + N + 1) / 2N), the
+    low-pass to band-pass transform s -> (s^2 + w1 w2) / ((w2 - w1) s) with
+    edges pre-warped to w = 2 fs tan(pi f / fs), then the bilinear
+    transform z = (2 fs + s) / (2 fs - s).  The N analog zeros at s = 0 go
+    to z = 1 and the N at infinity to z = -1, one of each per section; the
+    gain sits in the first section, and sections are ordered by pole
+    radius, the pair nearest the unit circle last."""
+    fs2 = 2.0 * fs
+    w1 = fs2 * math.tan(math.pi * f_lo / fs)
+    w2 = fs2 * math.tan(math.pi * f_hi / fs)
+    bw, w0sq = w2 - w1, w1 * w2
+    poles = []
+    for k in range(order):
+        half = cmath.exp(1j * math.pi * (2 * k + order + 1) / (2 * order)) * (0.5 * bw)
+        disc = cmath.sqrt(half * half - w0sq)
+        poles += [half + disc, half - disc]
+    gain = complex(bw ** order * fs2 ** order)
+    for p in poles:
+        gain /= fs2 - p
+    zp = [(fs2 + p) / (fs2 - p) for p in poles]
+    upper = sorted((z for z in zp if z.imag > 0.0), key=abs)
+    if len(upper) != order:
+        raise ValueError("band edges too close for a conjugate-pair design")
+    sos = []
+    for i, z in enumerate(upper):
+        g = gain.real if i == 0 else 1.0
+        sos.append((g, 0.0, -g, -2.0 * z.real, z.real * z.real + z.imag * z.imag))
+    return sos
+
+
+def _sos_pass(sos, x, x0):
+    """One causal pass through the sections (transposed direct form II),
+    each section starting in its steady state for a constant input equal
+    to x0 (the filtfilt initial conditions of Gustafsson 1996)."""
+    y = list(x)
+    v = x0
+    for b0, b1, b2, a1, a2 in sos:
+        y_ss = v * (b0 + b1 + b2) / (1.0 + a1 + a2)
+        z2 = b2 * v - a2 * y_ss
+        z1 = b1 * v - a1 * y_ss + z2
+        for i, xi in enumerate(y):
+            yi = b0 * xi + z1
+            z1 = b1 * xi - a1 * yi + z2
+            z2 = b2 * xi - a2 * yi
+            y[i] = yi
+        v = y_ss
+    return y
+
+
+def sos_filtfilt(sos, x):
+    """Zero-phase filtering: odd extension by 3 (2 sections + 1) samples at
+    each end (at most len - 1), a forward and a backward pass, the
+    extension cut off again."""
+    x = [float(v) for v in x]
+    n = len(x)
+    pad = min(3 * (2 * len(sos) + 1), n - 1)
+    ext = [2.0 * x[0] - x[k] for k in range(pad, 0, -1)] + x + [2.0 * x[-1] - x[n - 2 - k] for k in range(pad)]
+    fwd = _sos_pass(sos, ext, ext[0])
+    back = _sos_pass(sos, fwd[::-1], fwd[-1])[::-1]
+    return back[pad:pad + n]
+
+
 def ground_motion(seed: int, dt: float, n_steps: int, pga_g: float, f_lo: float = 0.1, f_hi: float = 25.0,
                   rise: float = 2.0, strong: float = 6.0, decay: float = 2.0):
     """Band-limited white noise (4th-order Butterworth band-pass applied
     forward and backward) under a trapezoidal envelope, scaled to a peak
     ground acceleration of pga_g * 9.81 m/s^2.  Returns n_steps + 1 samples
-    at spacing dt starting at t = 0."""
+    at spacing dt starting at t = 0.  Python floats and the math module
+    only, so the record does not depend on a numerical library's version."""
     import numpy as np
-    from scipy import signal
 
     n = n_steps + 1
     raw = gaussian_noise(seed, n)
     fs = 1.0 / dt
     hi = min(f_hi, 0.45 * fs)
-    sos = signal.butter(4, [f_lo, hi], btype="bandpass", fs=fs, output="sos")
-    band = signal.sosfiltfilt(sos, raw)
+    band = np.array(sos_filtfilt(butterworth_bandpass_sos(4, f_lo, hi, fs), raw))
     t = np.arange(n) * dt
     env = np.clip(np.minimum(t / rise, (rise + strong + decay - t) / decay), 0.0, 1.0)
     acc = band * env
@@ -501,7 +566,7 @@ def rc_frame3d_case(stories: int, bays_x: int, bays_y: int, steps: int, dt: floa
     for tag, (seed, pga, direction) in enumerate(zip(seeds, pgas_g, (1, 2)), start=1):
         if not bidirectional and tag > 1:
             break
-        acc = ground_motion(seed, dt, max(steps, 100), pga, strong=strong)  # the band-pass needs 100 samples
+        acc = ground_motion(seed, dt, steps, pga, strong=strong)
         series.append({"type": "Path", "tag": tag, "dt": dt, "values": [float(v) for v in acc], "factor": 1.0})
         patterns.append({"type": "UniformExcitation", "tag": tag, "direction": direction, "accel": tag})
     case = {
