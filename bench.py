@@ -25,11 +25,23 @@ Two legs, one JSON line.
 3. The uneven ensemble (``config.ensemble``): the three frame variants
    (4 x 3, 8 x 4 and 12 x 6 bays, sections as above) x N ground motions from
    the ensemble generator (corner frequency 2-15 Hz, strong-motion duration
-   5-20 s, PGA on the log grid 0.1-1.5 g), each record capped at
-   --ens-steps steps, all in one FrameBatch (autotuned schedule), against
-   the same records solved one by one with the plain path (four records of
-   each frame timed alone; the one-by-one time is each frame's completed
-   steps over the steps per second of its timed single records).
+   5-20 s, PGA on the log grid 0.1-1.5 g, 1000-3000 steps; --ens-steps caps
+   them), all in one FrameBatch (autotuned schedule), against the same
+   records solved one by one with the plain path (four records of each
+   frame timed alone; the one-by-one time is each record's Newton
+   iterations at its frame's timed seconds per iteration).  Its GM-steps/s
+   as a fraction of the HBM roofline counts the device's own section
+   evaluations; the CPU baseline is the oracle in P pinned processes; the
+   high-PGA records the device stopped earliest are re-run on the oracle
+   and their stops compared.
+
+GM-step roofline fraction of every GM leg: (section-evaluation bytes +
+commit bytes + band bytes) / wall time / 8 TB/s, with 80 B (Concrete02) or
+168 B (Steel02) per fiber evaluation and per fiber commit, and
+n (2 hbw + 1) 8 B x 2 per band factorization plus x 1 per solve.
+
+The single-core CPU baselines run one after another in one worker process
+pinned to one CPU (as taskset -c would), after the GPU legs.
 
 4. The learned Newton predictor (``config.learned``): on the 4 x 3 frame, a
    network trained on the device from the pairs the solver records on one
@@ -83,6 +95,111 @@ def algorithmic_bytes(mats, cells, counts, aux: bool) -> int:
             if (ty in AUX_TYPES) == aux:
                 total += BYTES_PER_UPDATE.get(ty, 0) * n
     return total
+
+
+def frame_bytes(fc) -> dict:
+    """Algorithmic HBM bytes of a 2D frame's GM-step work: per section point
+    evaluation (the frame's sections weighted by their integration points),
+    per commit of every section after a converged step, and per band
+    factorization (read + write) and solve (read)."""
+    mats = fc.materials
+    per_sec = {}
+    for sid in fc.section_ids:
+        per_sec[int(sid)] = sum(BYTES_PER_UPDATE.get(int(mats.defs[m].mat_type), 0) for m in fc.section_cells[sid]["mat"])
+    nip = [int(n) for n in fc.elem_nip]
+    commit = sum(n * per_sec[int(s)] for n, s in zip(nip, fc.elem_section))
+    return {"per_section_eval": commit / max(sum(nip), 1), "per_commit": commit}
+
+
+def gm_roofline(fbytes: dict, n_free: int, hbw: int, sec_evals: int, steps: int, iters: int,
+                seconds: float) -> dict:
+    """GM-steps/s as a fraction of the HBM roofline: (section-evaluation +
+    commit + band bytes) / wall time / peak.  A step's converged iteration
+    factors nothing (the residual test comes first), so factorizations =
+    iterations - converged steps."""
+    band = n_free * (2 * hbw + 1) * 8
+    factorizations = max(iters - steps, 0)
+    total = sec_evals * fbytes["per_section_eval"] + steps * fbytes["per_commit"] + factorizations * band * 3
+    achieved = total / seconds / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": total, "section_evaluations": int(sec_evals),
+            "bytes_per_section_eval": fbytes["per_section_eval"], "commit_bytes_per_step": fbytes["per_commit"],
+            "band_factorizations": int(factorizations)}
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "allowed_cpus": len(os.sched_getaffinity(0)), "lscpu_model": model}
+
+
+_CPU_SLOT = None
+
+
+def _pin_worker(q) -> None:
+    """Pool initializer: this worker runs on one CPU of its own (taskset -c)."""
+    global _CPU_SLOT
+    _CPU_SLOT = q.get()
+    os.sched_setaffinity(0, {_CPU_SLOT})
+
+
+def cpu_pool(workers: int):
+    """Worker processes started fresh (no GPU state), each pinned to its own
+    CPU from this process's allowed set."""
+    import concurrent.futures
+    import multiprocessing as mp
+
+    ctx = mp.get_context("spawn")
+    cpus = sorted(os.sched_getaffinity(0))[:workers]
+    q = ctx.Queue()
+    for c in cpus:
+        q.put(c)
+    return concurrent.futures.ProcessPoolExecutor(max_workers=len(cpus), mp_context=ctx, initializer=_pin_worker,
+                                                  initargs=(q,)), len(cpus)
+
+
+def _cpu_task(name: str, *a):
+    """Run one CPU-baseline or oracle-check task in a pinned worker."""
+    if name == "sections":
+        mats, cells, _, _ = build_population()
+        out = cpu_baseline(mats, cells, *a)
+    elif name == "gm":
+        out = gm_cpu_baseline(*a)
+    elif name == "frame3d":
+        out = frame3d_cpu_baseline(*a)
+    elif name == "ensemble_record":
+        out = _oracle_record(*a)
+    else:
+        raise ValueError(name)
+    out["cpu"] = _CPU_SLOT
+    return out
+
+
+def _oracle_record(k: int, seed: int, max_steps: int, step_cap: int) -> dict:
+    """The oracle on one record of the uneven ensemble (its first step_cap
+    steps when step_cap > 0)."""
+    from oracle import oracle
+    from strut_amd import cases
+
+    s, b = cases.CONFIG2_FRAMES[k]
+    fc = _motion_case(cases.config2_frame_case(s, b, steps=100), seed, max_steps)
+    if step_cap > 0:
+        fc.steps = min(fc.steps, step_cap)
+    t0 = time.perf_counter()
+    r = oracle.run_transient2d(fc, record_history=False)
+    dt = time.perf_counter() - t0
+    n = int(r["steps_done"])
+    return {"frame": k, "seed": seed, "steps": fc.steps, "status": int(r["status"]), "steps_done": n,
+            "iters": [int(v) for v in r["iters"][:n]],
+            "newton_iters": int(r["iters"][:fc.steps].sum()), "section_evals": int(r["section_evals"]),
+            "seconds": dt}
 
 
 def cpu_baseline(mats, cells, budget_s: float = 12.0) -> dict:
@@ -244,12 +361,16 @@ def gm_leg(n_batch: int, steps: int) -> dict:
         fb.rt.profile_enable(False)
         sched = fb.schedule_info()
         res = [fb.result(i) for i in ids]
+        evals = int(fb.section_evals().sum())
+        info = fb.info(fid)
+        fb.close()
         done = sum(r["steps_done"] for r in res)
         iters = sum(int(r["iters"][:r["steps_done"]].sum()) for r in res)
+        all_iters = sum(int(r["iters"].sum()) for r in res)
         out[label] = {"records": len(ms), "gm_steps": int(done), "seconds": dt, "gm_steps_per_s": done / dt,
                       "kernel_ms": kms, "launches": kn, "newton_iters": iters, "schedule": sched,
-                      "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0))}
-        fb.close()
+                      "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0)),
+                      "roofline": gm_roofline(frame_bytes(fc), info["free"], info["hbw"], evals, done, all_iters, dt)}
     out["batched_vs_plain"] = out["batched"]["gm_steps_per_s"] / out["plain"]["gm_steps_per_s"]
     return out
 
@@ -368,39 +489,115 @@ def _run_batch(fcs_by_frame, record=False, predictor=None, record_all=False, sch
     return fb, fids, ids, time.perf_counter() - t0
 
 
-def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4) -> dict:
-    """The uneven three-frame ensemble batched vs its records one by one."""
+def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4, cpu_workers: int = 16,
+                 cpu_steps: int = 300, check_records: int = 3, cpu: bool = True) -> dict:
+    """The uneven three-frame ensemble batched vs its records one by one on
+    the same GPU, its GM-steps/s roofline fraction, the oracle in P pinned
+    processes as the CPU throughput baseline, and high-PGA records' stops
+    checked against the oracle."""
     from strut_amd import _native, cases
 
     frames = [cases.config2_frame_case(s, b, steps=100) for s, b in cases.CONFIG2_FRAMES]
-    by_frame = {k: [_motion_case(f, 100000 + 1000 * k + j, max_steps) for j in range(n_motions)]
-                for k, f in enumerate(frames)}
-    fb, _, ids, sec = _run_batch(by_frame)
+    seeds = {k: [100000 + 1000 * k + j for j in range(n_motions)] for k in range(len(frames))}
+    by_frame = {k: [_motion_case(f, sd, max_steps) for sd in seeds[k]] for k, f in enumerate(frames)}
+    fb, fids, ids, sec = _run_batch(by_frame)
     res = [fb.result(i) for _, i in ids]
+    evals = fb.section_evals()
+    infos = {k: fb.info(fid) for k, fid in fids.items()}
     sched = fb.schedule_info()
     fb.close()
-    done = [r["steps_done"] for r in res]
+    done = [int(r["steps_done"]) for r in res]
+    iters = [int(r["iters"].sum()) for r in res]
+    # roofline: bytes of every instance over the batched wall time
+    total_bytes = 0.0
+    per_frame_bytes = {k: frame_bytes(by_frame[k][0]) for k in by_frame}
+    for (k, i), d, it in zip(ids, done, iters):
+        rl = gm_roofline(per_frame_bytes[k], infos[k]["free"], infos[k]["hbw"], int(evals[i]), d, it, 1.0)
+        total_bytes += rl["algorithmic_bytes"]
+    achieved = total_bytes / sec / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": total_bytes,
+                "section_evaluations": int(evals.sum())}
     # plain path, records one by one: n_single records of each frame alone
-    # (every n_motions / n_single-th motion, the same step cap); each frame's
-    # one-by-one time is its batched steps over the sample's steps per second
-    single = {}
+    # (every n_motions / n_single-th motion); each frame's one-by-one time is
+    # its batched Newton iterations times the sample's seconds per iteration
+    single, single_it, sample = {}, {}, {}
     for k, f in enumerate(frames):
-        sec1 = steps1 = 0.0
-        for j in range(0, n_motions, max(1, n_motions // n_single))[:n_single]:
-            fb1, _, ids1, s1 = _run_batch({k: [_motion_case(f, 100000 + 1000 * k + j, max_steps)]},
-                                          schedule=_native.SCHEDULE_ROUNDS)
-            steps1 += fb1.result(ids1[0][1])["steps_done"]
+        sec1 = steps1 = it1 = 0.0
+        picks = list(range(0, n_motions, max(1, n_motions // n_single)))[:n_single]
+        sample[k] = [seeds[k][j] for j in picks]
+        for j in picks:
+            fb1, _, ids1, s1 = _run_batch({k: [by_frame[k][j]]}, schedule=_native.SCHEDULE_ROUNDS)
+            r1 = fb1.result(ids1[0][1])
+            steps1 += r1["steps_done"]
+            it1 += int(r1["iters"].sum())
             sec1 += s1
             fb1.close()
         single[k] = steps1 / sec1
-    one_by_one = sum(d / single[k] for (k, _), d in zip(ids, done))
-    return {"records": len(ids), "frames": [f"{s}x{b}" for s, b in cases.CONFIG2_FRAMES], "max_steps": max_steps,
-            "gm_steps": int(sum(done)), "seconds": sec, "gm_steps_per_s": sum(done) / sec,
-            "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0)),
-            "plain_single_record_gm_steps_per_s": {f"{s}x{b}": single[k]
-                                                   for k, (s, b) in enumerate(cases.CONFIG2_FRAMES)},
-            "single_records_timed_per_frame": n_single,
-            "one_by_one_seconds_est": one_by_one, "batched_vs_one_by_one": one_by_one / sec, "schedule": sched}
+        single_it[k] = sec1 / it1
+    by_steps = sum(d / single[k] for (k, _), d in zip(ids, done))
+    by_iters = sum(it * single_it[k] for (k, _), it in zip(ids, iters))
+    names = [f"{s}x{b}" for s, b in cases.CONFIG2_FRAMES]
+    out = {"records": len(ids), "frames": names, "max_steps": max_steps,
+           "record_steps": {"min": int(min(r.steps for fl in by_frame.values() for r in fl)),
+                            "max": int(max(r.steps for fl in by_frame.values() for r in fl))},
+           "gm_steps": int(sum(done)), "newton_iters": int(sum(iters)), "seconds": sec,
+           "gm_steps_per_s": sum(done) / sec, "roofline": roofline,
+           "stopped_unconverged": int(sum(1 for r in res if r["status"] != 0)),
+           "plain_single_record_gm_steps_per_s": {names[k]: single[k] for k in single},
+           "plain_single_record_ms_per_newton_iter": {names[k]: 1e3 * single_it[k] for k in single_it},
+           "single_records_timed_per_frame": n_single,
+           "one_by_one_seconds_est": by_iters, "batched_vs_one_by_one": by_iters / sec,
+           "one_by_one_seconds_est_by_steps": by_steps, "batched_vs_one_by_one_by_steps": by_steps / sec,
+           "schedule": sched}
+    if not cpu:
+        return out
+    # CPU: the oracle in P pinned processes -- per-frame seconds per Newton
+    # iteration from the timed sample records' first cpu_steps steps, and
+    # full oracle runs of high-PGA records that the device stopped early
+    status, pos = {}, 0
+    for k in by_frame:  # ids hold each frame's records in seed order, frame after frame
+        for sd in seeds[k]:
+            assert ids[pos][0] == k
+            status[(k, sd)] = (res[pos], int(evals[ids[pos][1]]))
+            pos += 1
+    checks = []
+    for k in by_frame:  # per frame, the high-PGA records the device stopped earliest (bounded oracle time)
+        cand = [(status[(k, sd)][0]["steps_done"], sd) for sd in seeds[k]
+                if cases.config2_motion_params(sd)["pga_g"] >= 0.9 and status[(k, sd)][0]["status"] != 0]
+        checks += [(k, sd) for _, sd in sorted(cand)[:check_records]]
+    pool, P = cpu_pool(cpu_workers)
+    t0 = time.perf_counter()
+    with pool:
+        timing = [pool.submit(_cpu_task, "ensemble_record", k, sd, max_steps, cpu_steps)
+                  for k in by_frame for sd in sample[k]]
+        checked = [pool.submit(_cpu_task, "ensemble_record", k, sd, max_steps, 0) for k, sd in checks]
+        timing = [f.result() for f in timing]
+        checked = [f.result() for f in checked]
+    wall = time.perf_counter() - t0
+    cpu_it = {k: sum(t["seconds"] for t in timing if t["frame"] == k) /
+              max(sum(t["newton_iters"] for t in timing if t["frame"] == k), 1) for k in by_frame}
+    cpu_serial = sum(it * cpu_it[k] for (k, _), it in zip(ids, iters))
+    out["cpu_baseline"] = {
+        "value": sum(done) / (cpu_serial / P), "unit": "GM-steps/s", "cores": P, "kind": "port",
+        "sample": f"oracle C restatement in {P} processes, one pinned per CPU: the first {cpu_steps} steps of the "
+                  f"{n_single} timed records of each frame give each frame's seconds per Newton iteration "
+                  f"({', '.join(f'{names[k]} {1e3 * cpu_it[k]:.3g} ms' for k in cpu_it)}); the ensemble's CPU time "
+                  f"is its Newton iterations at those rates over {P} processes ({cpu_serial:.0f} core-seconds)",
+        "sample_wall_seconds": wall, **host_cpu()}
+    out["batched_vs_cpu_processes"] = out["gm_steps_per_s"] / out["cpu_baseline"]["value"]
+    rows = []
+    for c in checked:
+        r, ev = status[(c["frame"], c["seed"])]
+        n = c["steps_done"]
+        rows.append({"frame": names[c["frame"]], "seed": c["seed"],
+                     "pga_g": cases.config2_motion_params(c["seed"])["pga_g"],
+                     "status_gpu": int(r["status"]), "status_oracle": c["status"],
+                     "steps_done_gpu": int(r["steps_done"]), "steps_done_oracle": n,
+                     "same_iterations": [int(v) for v in r["iters"][:n]] == c["iters"],
+                     "same_section_evaluations": ev == c["section_evals"]})
+    out["high_pga_stop_checks"] = rows
+    return out
 
 
 def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
@@ -424,11 +621,17 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
     del x, y
     runs = {}
     # both runs on the work queue, so that they differ only in the predictor
+    roof = {}
     for label, pred in (("plain", None), ("learned", {0: net})):
-        fb, _, ids, sec = _run_batch(held, predictor=pred, record_all=True, schedule=_native.SCHEDULE_QUEUE)
+        fb, hf, ids, sec = _run_batch(held, predictor=pred, record_all=True, schedule=_native.SCHEDULE_QUEUE)
         runs[label] = ([fb.result(i) for _, i in ids], [fb.inst_stats(i) for _, i in ids], sec)
+        info, ev = fb.info(hf[0]), int(fb.section_evals().sum())
         fb.close()
+        rr = runs[label][0]
+        roof[label] = gm_roofline(frame_bytes(held[0][0]), info["free"], info["hbw"], ev,
+                                  sum(int(r["steps_done"]) for r in rr), sum(int(r["iters"].sum()) for r in rr), sec)
     net.close()
+    out["roofline"] = roof
     (rp, _, sp), (rl, stl, sl) = runs["plain"], runs["learned"]
     same = [a["status"] == b["status"] and a["steps_done"] == b["steps_done"] for a, b in zip(rp, rl)]
     it_p = sum(int(a["iters"][:a["steps_done"]].sum()) for a in rp)
@@ -478,14 +681,17 @@ def main() -> int:
     ap.add_argument("--gm-steps", type=int, default=1000)
     ap.add_argument("--ens-motions", type=int, default=256,
                     help="motions per frame in the ensemble leg (0 = skip; 256 is the ensemble configuration's count)")
-    ap.add_argument("--ens-steps", type=int, default=1000)
+    ap.add_argument("--ens-steps", type=int, default=3000,
+                    help="step cap of the ensemble records (3000: none; their lengths are 1000-3000 steps)")
+    ap.add_argument("--cpu-workers", type=int, default=16,
+                    help="pinned oracle processes of the ensemble's CPU baseline (at most the allowed CPUs)")
     ap.add_argument("--learned-train", type=int, default=256, help="training records of the learned leg (0 = skip)")
     ap.add_argument("--learned-test", type=int, default=64)
     ap.add_argument("--learned-steps", type=int, default=800)
     ap.add_argument("--learned-epochs", type=int, default=30)
     ap.add_argument("--f3-steps", type=int, default=10, help="timed steps of the 3D frame transient leg (0 = skip)")
     ap.add_argument("--f3-warmup", type=int, default=2)
-    ap.add_argument("--f3-cpu-steps", type=int, default=2, help="steps of the 3D frame leg's CPU baseline sample")
+    ap.add_argument("--f3-cpu-steps", type=int, default=6, help="steps of the 3D frame leg's CPU baseline sample")
     ap.add_argument("--f3-records", type=int, default=32, help="record pairs run together in the 3D frame leg (1 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -511,8 +717,6 @@ def main() -> int:
     if args.gm_batch > 0:
         gm = gm_leg(args.gm_batch, args.gm_steps)
         _progress(f"GM-steps leg: batched/plain {gm['batched_vs_plain']:.3g}")
-        if not args.no_cpu_baseline:
-            gm["cpu_baseline"] = gm_cpu_baseline(args.gm_steps)
         config["gm_steps"] = {"workload": f"2D RC frame 3 stories x 1 bay, forceBeamColumn 5 IPs, {args.gm_steps} "
                                           "steps dt 0.01 s, PGA 0.4 g seeded ground motions (seeds 0..N-1)",
                               **gm}
@@ -523,19 +727,35 @@ def main() -> int:
             _progress(f"3D frame leg, {args.f3_records} records together: "
                       f"{f3['records_together']['record_steps_per_s']:.3g} record-steps/s "
                       f"({f3['records_together']['vs_one_record']:.3g}x one record)")
-        if not args.no_cpu_baseline:
-            f3["cpu_baseline"] = frame3d_cpu_baseline(args.f3_cpu_steps)
-            _progress(f"3D frame leg CPU baseline: {f3['cpu_baseline']['value']:.3g} steps/s")
         config["frame3d"] = {"workload": "3D RC frame 20 stories x 5 x 5 bays, forceBeamColumn3d columns (PDelta) and "
                                          "dispBeamColumn3d beams, 5 IPs, 268-fiber sections, rigid diaphragms "
                                          "(Transformation), two seeded records 0.5 g / 0.35 g, dt 0.005 s", **f3}
     if args.ens_motions > 0:
-        config["ensemble"] = ensemble_leg(args.ens_motions, args.ens_steps)
-        _progress(f"ensemble leg: batched/one-by-one {config['ensemble']['batched_vs_one_by_one']:.3g}")
+        config["ensemble"] = ensemble_leg(args.ens_motions, args.ens_steps, cpu_workers=args.cpu_workers,
+                                          cpu=not args.no_cpu_baseline)
+        e = config["ensemble"]
+        _progress(f"ensemble leg: {e['gm_steps_per_s']:.4g} GM-steps/s, roofline {e['roofline']['frac']:.3g}, "
+                  f"batched/one-by-one {e['batched_vs_one_by_one']:.3g} (by steps "
+                  f"{e['batched_vs_one_by_one_by_steps']:.3g})")
+        if "cpu_baseline" in e:
+            _progress(f"ensemble CPU baseline: {e['cpu_baseline']['value']:.4g} GM-steps/s on "
+                      f"{e['cpu_baseline']['cores']} processes; high-PGA checks {e['high_pga_stop_checks']}")
     if args.learned_train > 0:
         config["learned"] = learned_leg(args.learned_train, args.learned_test, args.learned_steps,
                                         args.learned_epochs)
         _progress(f"learned leg: iteration reduction {config['learned']['newton_iter_reduction']:.3g}")
+    cpu_main = None
+    if not args.no_cpu_baseline:  # the single-core baselines, one after another in one pinned process
+        pool, _ = cpu_pool(1)
+        with pool:
+            cpu_main = pool.submit(_cpu_task, "sections").result()
+            _progress(f"section CPU baseline: {cpu_main['value']:.4g} fiber updates/s")
+            if "gm_steps" in config:
+                config["gm_steps"]["cpu_baseline"] = pool.submit(_cpu_task, "gm", args.gm_steps).result()
+            if "frame3d" in config:
+                config["frame3d"]["cpu_baseline"] = pool.submit(_cpu_task, "frame3d", args.f3_cpu_steps).result()
+                _progress(f"3D frame leg CPU baseline: {config['frame3d']['cpu_baseline']['value']:.3g} steps/s")
+        cpu_main.update(host_cpu())
     out = {
         "metric": "fiber updates/s",
         "value": sec["value"],
@@ -552,8 +772,8 @@ def main() -> int:
         "config": config,
         "roofline": sec["roofline"],
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sec["mats"], sec["cells"])
+    if cpu_main is not None:
+        out["cpu_baseline"] = cpu_main
     print(json.dumps(out))
     return 0
 

@@ -597,6 +597,13 @@ int sgpu_frame2d_set_predictor(sgpu_ctx *ctx, int32_t frame_id, const sgpu_mlp *
 /* out4 = {Newton iterations, steps started from a proposal, proposals
  * replaced by the Newmark predictor, dense-LU fallbacks} of an instance. */
 int sgpu_frame2d_inst_stats(sgpu_ctx *ctx, int32_t instance, int32_t *out4);
+/* Section point evaluations of instances [first, first + count) so far,
+ * counted where the reference's runtime metrics count them (a batch of an
+ * element's integration points in the predictor and each local iteration,
+ * force_beam_column2d.mojo:910 and :1082; the points of a dispBeamColumn2d
+ * state determination) plus the re-evaluation at the committed
+ * deformations after a failed local iteration. */
+int sgpu_frame2d_section_evals(sgpu_ctx *ctx, int32_t first, int32_t count, int64_t *out);
 
 /* ------------------------------------------------------------------
  * Per-kernel timing.  Mirrors the phase timers of Strut's --profile output

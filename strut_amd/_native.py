@@ -77,6 +77,7 @@ EXPORTED = (
     "sgpu_frame2d_samples",
     "sgpu_frame2d_set_predictor",
     "sgpu_frame2d_inst_stats",
+    "sgpu_frame2d_section_evals",
     "sgpu_frame2d_launch_queue",
     "sgpu_elem3d_add",
     "sgpu_elem3d_state",

@@ -72,6 +72,7 @@ struct ElemCtx {
     const MatDev *mats;  // material table (LDS copy when it fits)
     double *red;         // this wavefront's reduction rows
     double *last;        // this element's last evaluated (eps0, kappa) per integration point
+    unsigned long long *sec_evals;  // the instance's section-evaluation count
 };
 
 // fiber_section2d_batch_eval_same_def (fiber2d.mojo:3042-3086) for the
@@ -106,6 +107,7 @@ __device__ bool eval_sections(const ElemCtx &E, const double *pe, const double *
             }
         }
     }
+    if (threadIdx.x == 0) atomicAdd(E.sec_evals, (unsigned long long)nip);
     __syncthreads();
     bool ok = true;
     for (int ip = 0; ip < nip; ++ip)
@@ -324,10 +326,11 @@ __device__ __forceinline__ double kglobal(const double *kb, double inv_L, double
 // force_beam_column2d.mojo:1418-1919 for one element, executed by one whole
 // wavefront; lanes 0..35 store k_global, lanes 36..41 f_global.  Returns
 // false when the reference would abort (no recoverable tangent).
-__device__ bool fbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, double *q_global,
-                              const double *uf, double *out42, ElemShared *sh, double *q, FbcScratch &sc,
-                              const MatDev *mats, double *red) {
-    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e], sh, mats, red, A.last + I.last_base + 2 * kMaxNipDev * e};
+__device__ bool fbc2d_element(const Frame2dArgs &A, int inst, const InstDev &I, const ElemDev &el, int e,
+                              double *q_global, const double *uf, double *out42, ElemShared *sh, double *q,
+                              FbcScratch &sc, const MatDev *mats, double *red) {
+    ElemCtx E{&A, &el, A.elem_inst[I.elem_inst_base + e], sh, mats, red, A.last + I.last_base + 2 * kMaxNipDev * e,
+              A.sec_evals + inst};
     const int nip = el.nip;
     const int q_len = 3 + 2 * nip + 3;
     // the wavefront's private copy of the element state (uniform in all wavefronts)
@@ -511,9 +514,11 @@ __device__ __forceinline__ double dbc_curvature_shape(int idx, double inv_L, dou
 
 static_assert(sizeof(FbcScratch) >= 42 * sizeof(double), "dispBeamColumn2d sums its outputs in an FbcScratch");
 
-__device__ void dbc2d_element(const Frame2dArgs &A, const InstDev &I, const ElemDev &el, int e, const double *uf,
-                              double *out42, ElemShared *sh, const MatDev *mats, double *red, double *tmp) {
+__device__ void dbc2d_element(const Frame2dArgs &A, int inst, const InstDev &I, const ElemDev &el, int e,
+                              const double *uf, double *out42, ElemShared *sh, const MatDev *mats, double *red,
+                              double *tmp) {
     const int lane = threadIdx.x & 63, nip = el.nip, n_waves = blockDim.x >> 6;
+    if (threadIdx.x == 0) atomicAdd(A.sec_evals + inst, (unsigned long long)nip);
     const double L = el.L, c = el.c, s = el.s;
     const double inv_L = 1.0 / L;
     const double inv_L_sq = inv_L * inv_L;
@@ -649,11 +654,11 @@ __device__ __forceinline__ void element_state(const Frame2dArgs &A, int inst, in
         __syncthreads();
     }
     if (el.type == 1) {
-        dbc2d_element(A, I, el, e, A.vec + I.vec_base, A.scratch + I.scratch_base + 42 * e, &L.sh, mats, L.wred[w],
-                      reinterpret_cast<double *>(&L.scr[0]));
+        dbc2d_element(A, inst, I, el, e, A.vec + I.vec_base, A.scratch + I.scratch_base + 42 * e, &L.sh, mats,
+                      L.wred[w], reinterpret_cast<double *>(&L.scr[0]));
         return;
     }
-    const bool ok = fbc2d_element(A, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
+    const bool ok = fbc2d_element(A, inst, I, el, e, A.q + I.q_base + el.q_off, A.vec + I.vec_base,
                                   A.scratch + I.scratch_base + 42 * e, &L.sh, L.qbuf[w], L.scr[w], mats, L.wred[w]);
     if (!ok && threadIdx.x == 0) A.eflag[inst] = 1;
 }

@@ -50,6 +50,7 @@ struct FrameHost {
     size_t lds = 0;
     int max_steps = 0;
     InstState *d_state = nullptr;
+    unsigned long long *d_sec_evals = nullptr;  // per instance
     double *d_hist = nullptr;
     int32_t *d_iters = nullptr;
     std::vector<int64_t> hist_base, iters_base;
@@ -421,6 +422,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     if ((rc = upload(ctx, &d_ef, std::vector<int32_t>(fh.insts.size(), 0)))) return rc;
     std::vector<InstState> st(fh.insts.size(), InstState{});
     if ((rc = upload(ctx, &fh.d_state, st))) return rc;
+    if ((rc = upload(ctx, &fh.d_sec_evals, std::vector<unsigned long long>(fh.insts.size(), 0ull)))) return rc;
     std::vector<double> zq(q, 0.0), zv(vec, 0.0);
     double *d_q, *d_vec, *d_kinit, *d_scratch, *d_dense, *d_last;
     int32_t *d_piv;
@@ -451,6 +453,7 @@ int sgpu_frame2d_finalize(sgpu_ctx *ctx) {
     A.pair_inst = d_pi;
     A.pair_elem = d_pe;
     A.eflag = d_ef;
+    A.sec_evals = fh.d_sec_evals;
     A.ag = d_ag;
     A.q = d_q;
     A.vec = d_vec;
@@ -843,6 +846,21 @@ int sgpu_frame2d_inst_stats(sgpu_ctx *ctx, int32_t instance, int32_t *out4) {
     out4[1] = st.proposals;
     out4[2] = st.fallbacks;
     out4[3] = st.dense_fallbacks;
+    return SGPU_OK;
+}
+
+int sgpu_frame2d_section_evals(sgpu_ctx *ctx, int32_t first, int32_t count, int64_t *out) {
+    if (!ctx || count < 0 || (count > 0 && !out)) return SGPU_ERR_ARG;
+    FrameHost &fh = frames_of(ctx);
+    if (!fh.finalized) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame2d_finalize has not run");
+    if (first < 0 || (int64_t)first + count > (int64_t)fh.insts.size())
+        return fail(ctx, SGPU_ERR_ARG, "instances [%d, %d) out of range", first, first + count);
+    if (count == 0) return SGPU_OK;
+    std::vector<unsigned long long> v((size_t)count);
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OK(ctx, hipMemcpy(v.data(), fh.d_sec_evals + first, sizeof(unsigned long long) * (size_t)count,
+                          hipMemcpyDeviceToHost));
+    for (int32_t k = 0; k < count; ++k) out[k] = (int64_t)v[(size_t)k];
     return SGPU_OK;
 }
 

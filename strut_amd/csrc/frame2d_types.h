@@ -102,6 +102,7 @@ struct Frame2dArgs {
     StateArrays c, t;
     const MlpDev *preds;  // per frame: learned predictor (n_in == 0: the plain Newmark predictor)
     float *samples;       // training-sample recorder, or null
+    unsigned long long *sec_evals;  // per instance: section point evaluations so far
 };
 
 // Work-queue schedule: a ring of tasks, (instance << 16) | (element + 1),

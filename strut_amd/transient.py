@@ -12,7 +12,7 @@ convergence.
 from __future__ import annotations
 
 import ctypes
-from ctypes import POINTER, c_double, c_int32, c_void_p
+from ctypes import POINTER, c_double, c_int32, c_int64, c_void_p
 from typing import List, Optional
 
 import numpy as np
@@ -63,6 +63,7 @@ def _declare(lib):
                                                 POINTER(ctypes.c_int64)]),
         "sgpu_frame2d_set_predictor": (ctypes.c_int, [c_void_p, c_int32, c_void_p]),
         "sgpu_frame2d_inst_stats": (ctypes.c_int, [c_void_p, c_int32, POINTER(c_int32)]),
+        "sgpu_frame2d_section_evals": (ctypes.c_int, [c_void_p, c_int32, c_int32, POINTER(c_int64)]),
         "sgpu_frame2d_launch_queue": (ctypes.c_int, [c_void_p, c_int32]),
         "sgpu_frame2d_envelopes": (ctypes.c_int, [c_void_p, c_int32, ctypes.c_int, POINTER(c_int32), POINTER(c_int32),
                                                   POINTER(c_double), POINTER(c_double), POINTER(c_double)]),
@@ -238,6 +239,13 @@ class FrameBatch:
         self._check(self.lib.sgpu_frame2d_inst_stats(self.rt.ctx, inst, iptr(out)))
         return {"iters": int(out[0]), "proposals": int(out[1]), "fallbacks": int(out[2]),
                 "dense_fallbacks": int(out[3])}
+
+    def section_evals(self) -> np.ndarray:
+        """Section point evaluations of every instance so far (int64)."""
+        out = np.zeros(max(len(self.insts), 1), dtype=np.int64)
+        self._check(self.lib.sgpu_frame2d_section_evals(self.rt.ctx, 0, len(self.insts),
+                                                         out.ctypes.data_as(POINTER(c_int64))))
+        return out[:len(self.insts)]
 
     def close(self) -> None:
         self.rt.close()

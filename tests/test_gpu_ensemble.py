@@ -57,14 +57,18 @@ def test_gpu_uneven_three_frame_ensemble_matches_oracle():
     fb.set_schedule(_native.SCHEDULE_AUTO)
     fb.run(rounds=16)
     print("schedule", fb.schedule_info())
+    sec_evals = fb.section_evals()
     for (k, fc), iid in zip(runs, ids):
         got = fb.result(iid)
         ref = oracle.run_transient2d(fc)
         n = ref["steps_done"]
         print(f"frame {k} steps {fc.steps}: status gpu {got['status']} oracle {ref['status']}, "
-              f"iterations {got['iters'].sum()} vs {ref['iters'].sum()}")
+              f"iterations {got['iters'].sum()} vs {ref['iters'].sum()}, "
+              f"section evaluations {sec_evals[iid]} vs {ref['section_evals']}")
         assert got["status"] == ref["status"] and got["steps_done"] == n
         np.testing.assert_array_equal(got["iters"][:n], ref["iters"][:n])
+        # the same local iterations in every element: the same section evaluations
+        assert sec_evals[iid] == ref["section_evals"]
         u_ref = ref["u"][:n][:, fc.record_dofs]
         scale = np.maximum(np.max(np.abs(u_ref), axis=0), 1e-12)
         assert float(np.max(np.abs(got["u"][:n] - u_ref) / scale)) <= REL

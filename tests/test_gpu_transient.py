@@ -41,10 +41,13 @@ def test_gpu_config0_plain_matches_oracle(schedule):
     fb.set_schedule(schedule)
     fb.run(rounds=64)
     got = fb.result(i)
+    sec_evals = int(fb.section_evals()[i])
     fb.close()
-    print(f"config0 1000 steps: status {got['status']}, iterations {got['iters'].sum()} vs {ref['iters'].sum()}")
+    print(f"config0 1000 steps: status {got['status']}, iterations {got['iters'].sum()} vs {ref['iters'].sum()}, "
+          f"section evaluations {sec_evals} vs {ref['section_evals']}")
     assert got["status"] == 0 and got["steps_done"] == 1000
     np.testing.assert_array_equal(got["iters"], ref["iters"])
+    assert sec_evals == ref["section_evals"]  # the same local iterations in every element
     err = _rel_err(got["u"], ref["u"])
     print(f"config0 max relative displacement error {err:.3e}")
     assert err <= REL
