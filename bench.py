@@ -490,17 +490,18 @@ def _run_batch(fcs_by_frame, record=False, predictor=None, record_all=False, sch
 
 
 def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4, cpu_workers: int = 16,
-                 cpu_steps: int = 300, check_records: int = 3, cpu: bool = True) -> dict:
+                 cpu_steps: int = 300, check_records: int = 3, cpu: bool = True, frame_ids=(0, 1, 2),
+                 schedule=None) -> dict:
     """The uneven three-frame ensemble batched vs its records one by one on
     the same GPU, its GM-steps/s roofline fraction, the oracle in P pinned
     processes as the CPU throughput baseline, and high-PGA records' stops
     checked against the oracle."""
     from strut_amd import _native, cases
 
-    frames = [cases.config2_frame_case(s, b, steps=100) for s, b in cases.CONFIG2_FRAMES]
-    seeds = {k: [100000 + 1000 * k + j for j in range(n_motions)] for k in range(len(frames))}
-    by_frame = {k: [_motion_case(f, sd, max_steps) for sd in seeds[k]] for k, f in enumerate(frames)}
-    fb, fids, ids, sec = _run_batch(by_frame)
+    frames = {k: cases.config2_frame_case(*cases.CONFIG2_FRAMES[k], steps=100) for k in frame_ids}
+    seeds = {k: [100000 + 1000 * k + j for j in range(n_motions)] for k in frames}
+    by_frame = {k: [_motion_case(f, sd, max_steps) for sd in seeds[k]] for k, f in frames.items()}
+    fb, fids, ids, sec = _run_batch(by_frame, schedule=schedule)
     res = [fb.result(i) for _, i in ids]
     evals = fb.section_evals()
     infos = {k: fb.info(fid) for k, fid in fids.items()}
@@ -522,7 +523,7 @@ def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4, cpu_workers:
     # (every n_motions / n_single-th motion); each frame's one-by-one time is
     # its batched Newton iterations times the sample's seconds per iteration
     single, single_it, sample = {}, {}, {}
-    for k, f in enumerate(frames):
+    for k in frames:
         sec1 = steps1 = it1 = 0.0
         picks = list(range(0, n_motions, max(1, n_motions // n_single)))[:n_single]
         sample[k] = [seeds[k][j] for j in picks]
@@ -538,7 +539,7 @@ def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4, cpu_workers:
     by_steps = sum(d / single[k] for (k, _), d in zip(ids, done))
     by_iters = sum(it * single_it[k] for (k, _), it in zip(ids, iters))
     names = [f"{s}x{b}" for s, b in cases.CONFIG2_FRAMES]
-    out = {"records": len(ids), "frames": names, "max_steps": max_steps,
+    out = {"records": len(ids), "frames": [names[k] for k in frames], "max_steps": max_steps,
            "record_steps": {"min": int(min(r.steps for fl in by_frame.values() for r in fl)),
                             "max": int(max(r.steps for fl in by_frame.values() for r in fl))},
            "gm_steps": int(sum(done)), "newton_iters": int(sum(iters)), "seconds": sec,
@@ -595,7 +596,7 @@ def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4, cpu_workers:
                      "status_gpu": int(r["status"]), "status_oracle": c["status"],
                      "steps_done_gpu": int(r["steps_done"]), "steps_done_oracle": n,
                      "same_iterations": [int(v) for v in r["iters"][:n]] == c["iters"],
-                     "same_section_evaluations": ev == c["section_evals"]})
+                     "section_evaluations_gpu": ev, "section_evaluations_oracle": c["section_evals"]})
     out["high_pga_stop_checks"] = rows
     return out
 
@@ -683,6 +684,8 @@ def main() -> int:
                     help="motions per frame in the ensemble leg (0 = skip; 256 is the ensemble configuration's count)")
     ap.add_argument("--ens-steps", type=int, default=3000,
                     help="step cap of the ensemble records (3000: none; their lengths are 1000-3000 steps)")
+    ap.add_argument("--ens-frames", default="0,1,2", help="frame variants of the ensemble (0 4x3, 1 8x4, 2 12x6)")
+    ap.add_argument("--ens-schedule", choices=("auto", "rounds", "queue"), default="auto")
     ap.add_argument("--cpu-workers", type=int, default=16,
                     help="pinned oracle processes of the ensemble's CPU baseline (at most the allowed CPUs)")
     ap.add_argument("--learned-train", type=int, default=256, help="training records of the learned leg (0 = skip)")
@@ -731,8 +734,13 @@ def main() -> int:
                                          "dispBeamColumn3d beams, 5 IPs, 268-fiber sections, rigid diaphragms "
                                          "(Transformation), two seeded records 0.5 g / 0.35 g, dt 0.005 s", **f3}
     if args.ens_motions > 0:
+        from strut_amd import _native
+        sched = {"auto": _native.SCHEDULE_AUTO, "rounds": _native.SCHEDULE_ROUNDS,
+                 "queue": _native.SCHEDULE_QUEUE}[args.ens_schedule]
         config["ensemble"] = ensemble_leg(args.ens_motions, args.ens_steps, cpu_workers=args.cpu_workers,
-                                          cpu=not args.no_cpu_baseline)
+                                          cpu=not args.no_cpu_baseline,
+                                          frame_ids=tuple(int(v) for v in args.ens_frames.split(",")),
+                                          schedule=sched)
         e = config["ensemble"]
         _progress(f"ensemble leg: {e['gm_steps_per_s']:.4g} GM-steps/s, roofline {e['roofline']['frac']:.3g}, "
                   f"batched/one-by-one {e['batched_vs_one_by_one']:.3g} (by steps "
