@@ -786,6 +786,7 @@ struct SysShared {
     double norm, diag;
     int stat;
     int judge;
+    int prow;  // pivot row of the dense fallback's current column
 };
 
 // One Newton iteration of instance `bid` by the whole block, after its
@@ -1118,7 +1119,16 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         return;
     }
     if (dense_mode) {
-        // lsb.mojo:608-628 then :175-214: dense LU with partial pivoting
+        // lsb.mojo:608-628 then :175-214: dense LU with partial pivoting.  The
+        // matrix is banded: column col is nonzero only in rows up to col + hbw
+        // (a row only moves down when it is swapped out of the pivot position,
+        // to at most the pivot column + hbw), and partial pivoting keeps every
+        // row within 2 hbw of the diagonal.  So the pivot scan, the multiplier
+        // divisions and the update visit those ranges only: the entries
+        // outside them stay the exact zeros they are in the reference, and
+        // every visited entry sees the reference's operations in its order.
+        // Rows are swapped whole (with the multipliers stored so far), as the
+        // reference swaps them.
         for (int p = tid; p < n * n; p += nt) {
             const int row = p / n, col = p % n;
             const int off = col - row + hbw;
@@ -1127,24 +1137,36 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
         for (int i = tid; i < n; i += nt) piv[i] = i;
         __syncthreads();
         for (int col = 0; col < n; ++col) {
+            const int rlast = col + hbw > n - 1 ? n - 1 : col + hbw;
             if (col > 0) {  // store the multipliers of column col - 1
                 const int cp = col - 1;
-                for (int row = cp + 1 + tid; row < n; row += nt)
-                    dense[row * n + cp] = dense[row * n + cp] / s_diag;
+                const int rl = cp + hbw > n - 1 ? n - 1 : cp + hbw;
+                for (int row = cp + 1 + tid; row <= rl; row += nt) dense[row * n + cp] = dense[row * n + cp] / s_diag;
             }
-            __syncthreads();
-            if (tid == 0) {
-                int prow = col;
-                double pabs = fabs(dense[col * n + col]);
-                for (int row = col + 1; row < n; ++row) {
+            if (tid < 64) {
+                // the first row of the largest |a_r,col| (the reference scans
+                // from the diagonal with a strict comparison)
+                double best = -1.0;
+                int brow = n;
+                for (int row = col + tid; row <= rlast; row += 64) {
                     const double cand = fabs(dense[row * n + col]);
-                    if (cand > pabs) {
-                        pabs = cand;
-                        prow = row;
+                    if (cand > best) {
+                        best = cand;
+                        brow = row;
                     }
                 }
-                s_stat = pabs <= 1.0e-14 ? kInstSingular : 0;
-                s_norm = (double)prow;  // reuse as the broadcast pivot row
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(best, off);
+                    const int orow = __shfl_xor(brow, off);
+                    if (ob > best || (ob == best && orow < brow)) {
+                        best = ob;
+                        brow = orow;
+                    }
+                }
+                if (tid == 0) {
+                    s_stat = best <= 1.0e-14 ? kInstSingular : 0;
+                    ss.prow = brow;
+                }
             }
             __syncthreads();
             if (s_stat != 0) {
@@ -1154,7 +1176,7 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
                 }
                 return;
             }
-            const int prow = (int)s_norm;
+            const int prow = ss.prow;
             if (prow != col) {
                 for (int j = tid; j < n; j += nt) {
                     const double tmp = dense[col * n + j];
@@ -1176,9 +1198,10 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
                 }
                 return;
             }
-            const int m = n - col - 1;
-            for (int p = tid; p < m * m; p += nt) {
-                const int row = col + 1 + p / m, j = col + 1 + p % m;
+            const int clast = col + 2 * hbw > n - 1 ? n - 1 : col + 2 * hbw;
+            const int mr = rlast - col, mc = clast - col;
+            for (int p = tid; p < mr * mc; p += nt) {
+                const int row = col + 1 + p / mc, j = col + 1 + p % mc;
                 const double factor = dense[row * n + col] / diag;
                 if (factor == 0.0) continue;
                 dense[row * n + j] -= factor * dense[col * n + j];
@@ -1196,14 +1219,21 @@ __device__ __forceinline__ void sys_round(const Frame2dArgs &A, int bid, double 
                 for (int i = j + 1 + lane; i < n; i += 64) work[i] -= dense[i * n + j] * yj;
                 wave_sync();
             }
-            if (lane == 0) {
-                for (int i = n - 1; i >= 0; --i) {
+            // backward: row i's products formed by the lanes (in `vt`), then
+            // subtracted by lane 0 in column order; U is zero beyond 2 hbw
+            for (int i = n - 1; i >= 0; --i) {
+                const int j1 = i + 2 * hbw > n - 1 ? n - 1 : i + 2 * hbw;
+                const int cnt = j1 - i;
+                for (int t = lane; t < cnt; t += 64) vt[t] = dense[i * n + i + 1 + t] * du[i + 1 + t];
+                wave_sync();
+                if (lane == 0) {
                     double sum = work[i];
-                    for (int j = i + 1; j < n; ++j) sum -= dense[i * n + j] * du[j];
+                    for (int t = 0; t < cnt; ++t) sum -= vt[t];
                     du[i] = sum / dense[i * n + i];
                 }
-                st.dense_fallbacks += 1;
+                wave_sync();
             }
+            if (lane == 0) st.dense_fallbacks += 1;
         }
         __syncthreads();
     } else {

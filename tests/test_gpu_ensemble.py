@@ -73,3 +73,47 @@ def test_gpu_uneven_three_frame_ensemble_matches_oracle():
         scale = np.maximum(np.max(np.abs(u_ref), axis=0), 1e-12)
         assert float(np.max(np.abs(got["u"][:n] - u_ref) / scale)) <= REL
     fb.close()
+
+
+def test_gpu_dense_fallback_records_match_oracle():
+    """Records whose BandGeneral systems need row swaps, where the reference
+    leaves its band LU for the dense LU with partial pivoting (13 such
+    factorizations in the 4 x 3 record, 3 and 2 in the 12 x 6 ones, over
+    their first 150 steps): the same fallback count, status, stop step,
+    Newton and section-evaluation counts as the oracle, displacements within
+    1e-8.  The 4 x 3 record stops on Newton non-convergence at step 147, so
+    its last (stalled) step is not compared."""
+    frames = {0: cases.config2_frame_case(4, 3, steps=100), 2: cases.config2_frame_case(12, 6, steps=100)}
+    picks = [(0, 100020), (2, 102000), (2, 102026)]
+    runs = []
+    for k, seed in picks:
+        m = dict(cases.config2_motion(seed), steps=150)  # the first 150 steps of the full-length record
+        runs.append((k, load_frame2d_case(_motion_case(frames[k], m))))
+    fb = FrameBatch(runs[0][1].materials)
+    fids, ids = {}, []
+    for k, fc in runs:
+        if k not in fids:
+            fids[k] = fb.add_frame(fc)
+        ids.append(fb.add_case_motion(fids[k], fc))
+    fb.finalize()
+    fb.set_schedule(_native.SCHEDULE_QUEUE)
+    fb.run(rounds=16)
+    sec_evals = fb.section_evals()
+    total_fallbacks = 0
+    for (k, fc), iid in zip(runs, ids):
+        got, stats = fb.result(iid), fb.inst_stats(iid)
+        ref = oracle.run_transient2d(fc)
+        n = ref["steps_done"]
+        print(f"frame {k}: status {got['status']}/{ref['status']} steps {got['steps_done']}/{n} dense fallbacks "
+              f"{stats['dense_fallbacks']}/{ref['dense_fallbacks']}", flush=True)
+        assert got["status"] == ref["status"] and got["steps_done"] == n
+        np.testing.assert_array_equal(got["iters"][:n], ref["iters"][:n])
+        u_ref = ref["u"][:n][:, fc.record_dofs]
+        scale = np.maximum(np.max(np.abs(u_ref), axis=0), 1e-12)
+        assert float(np.max(np.abs(got["u"][:n] - u_ref) / scale)) <= REL
+        if ref["status"] == 0:
+            assert stats["dense_fallbacks"] == ref["dense_fallbacks"]
+            assert sec_evals[iid] == ref["section_evals"]
+        total_fallbacks += stats["dense_fallbacks"]
+    fb.close()
+    assert total_fallbacks >= 10
