@@ -366,9 +366,76 @@ __device__ bool fbc2d_element(const Frame2dArgs &A, int inst, const InstDev &I, 
     // The zero-increment attempt (force_beam_column2d.mojo:1623-1650) and the
     // subdivision loop (:1651-1763) share one call site of try_increment: the
     // former is the scheme loop run once with a zero increment.
+    bool exact = false;
+    if (el.all_elastic) {
+        // every fiber elastic: the exact state (force_beam_column2d.mojo:608-732), tried before
+        // the iteration (:1577-1620) with the section's initial flexibility and no element loads
+        const double s00 = el.if00, s01 = el.if01, s10 = s01, s11 = el.if11;
+        double f[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double vl0 = 0.0, vl1 = 0.0, vl2 = 0.0;
+        for (int ip = 0; ip < nip; ++ip) {
+            const double xi = el.xi[ip], wL = el.w[ip] * L, b_mi = xi - 1.0, b_mj = xi;
+            f[0] += wL * s00;
+            f[1] += wL * s01 * b_mi;
+            f[2] += wL * s01 * b_mj;
+            f[3] += wL * b_mi * s10;
+            f[4] += wL * b_mi * s11 * b_mi;
+            f[5] += wL * b_mi * s11 * b_mj;
+            f[6] += wL * b_mj * s10;
+            f[7] += wL * b_mj * s11 * b_mi;
+            f[8] += wL * b_mj * s11 * b_mj;
+            vl0 += wL * (s00 * 0.0 + s01 * 0.0);
+            vl1 += wL * b_mi * (s10 * 0.0 + s11 * 0.0);
+            vl2 += wL * b_mj * (s10 * 0.0 + s11 * 0.0);
+        }
+        double k[9];
+        if (el.if_valid && invert3(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], k)) {
+            const double r0 = v0 - vl0, r1 = v1 - vl1, r2 = v2 - vl2;
+            const double eq0 = k[0] * r0 + k[1] * r1 + k[2] * r2;
+            const double eq1 = k[3] * r0 + k[4] * r1 + k[5] * r2;
+            const double eq2 = k[6] * r0 + k[7] * r1 + k[8] * r2;
+            for (int ip = 0; ip < nip; ++ip) {
+                const double xi = el.xi[ip], b_mi = xi - 1.0, b_mj = xi;
+                const double axial = eq0 + 0.0;
+                const double moment = b_mi * eq1 + b_mj * eq2 + 0.0;
+                q[3 + ip] = s00 * axial + s01 * moment;
+                q[3 + nip + ip] = s10 * axial + s11 * moment;
+            }
+            exact = true;  // the sections' trial states (:700-715) are set after the loop below
+            q[0] = eq0;
+            q[1] = eq1;
+            q[2] = eq2;
+            ab0 = v0;
+            ab1 = v1;
+            ab2 = v2;
+            q[boff] = ab0;
+            q[boff + 1] = ab1;
+            q[boff + 2] = ab2;
+            aq0 = eq0;
+            aq1 = eq1;
+            aq2 = eq2;
+            for (int ip = 0; ip < nip; ++ip) {
+                sc.vs_e[ip] = q[3 + ip];
+                sc.vs_k[ip] = q[3 + nip + ip];
+            }
+            best[0] = eq0;
+            best[1] = eq1;
+            best[2] = eq2;
+            best[3] = k[0];
+            best[4] = k[1];
+            best[5] = k[2];
+            best[6] = k[3];
+            best[7] = k[4];
+            best[8] = k[5];
+            best[9] = k[8];
+            rem0 = rem1 = rem2 = 0.0;
+            have_best = true;
+            converged = true;
+        }
+    }
     const bool zero_inc = max_abs3(rem0, rem1, rem2) <= 1.0e-16;
     if (zero_inc) at0 = at1 = at2 = 0.0;
-    while (true) {
+    while (!converged) {
         if (!zero_inc && (num_sub > 4 || ++guard > 4096)) break;
         const double tv0 = ab0 + at0, tv1 = ab1 + at1, tv2 = ab2 + at2;
         bool scheme_success = false;
@@ -421,11 +488,15 @@ __device__ bool fbc2d_element(const Frame2dArgs &A, int inst, const InstDev &I, 
         at2 /= 10.0;
         num_sub += 1;
     }
-    if (!converged) {
-        restore_predictor(q, nip, aq0, aq1, aq2, sc);
-        q[boff] = ab0;
-        q[boff + 1] = ab1;
-        q[boff + 2] = ab2;
+    if (!converged || exact) {
+        if (!converged) {
+            restore_predictor(q, nip, aq0, aq1, aq2, sc);
+            q[boff] = ab0;
+            q[boff + 1] = ab1;
+            q[boff + 2] = ab2;
+        }
+        // the committed deformations after a failed iteration, or the exact
+        // elastic state's deformations (held in vs_e / vs_k)
         (void)eval_sections(E, sc.vs_e, sc.vs_k);
     }
     if (!have_best) {

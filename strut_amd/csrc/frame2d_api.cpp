@@ -151,8 +151,7 @@ int sgpu_frame2d_create(sgpu_ctx *ctx, const sgpu_frame2d_desc *d, int32_t *fram
         bool any_nonlinear = false;
         for (size_t s = 0; s < sd.mat.size(); ++s)
             if (sd.mat[s] >= 0 && ctx->mats[sd.mat[s]].type != kElastic) any_nonlinear = true;
-        if (!any_nonlinear)
-            return fail(ctx, SGPU_ERR_ARG, "element %d: all-elastic sections are not on the device path", e);
+        el.all_elastic = any_nonlinear ? 0 : 1;
         const int nip = d->elem_nip[e];
         if (nip < 1 || nip > kMaxNipDev)
             return fail(ctx, SGPU_ERR_ARG, "element %d: 1..%d integration points on the device path", e, kMaxNipDev);

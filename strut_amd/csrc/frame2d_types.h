@@ -16,7 +16,7 @@ struct ElemDev {
     int32_t nip;
     int32_t def;        // fiber-section definition of its integration points
     int32_t type;       // 0 forceBeamColumn2d, 1 dispBeamColumn2d
-    int32_t pad_;
+    int32_t all_elastic;  // every fiber elastic: forceBeamColumn2d's exact state (force_beam_column2d.mojo:608-732)
     int32_t fmap[6];    // free-DOF index of the element DOFs, -1 if constrained
     double L, inv_L, c, s;  // hypot length, 1/L, and dx/L as dx*(1/L) (force_beam_column2d.mojo:383-408)
                             // or, for dispBeamColumn2d, as dx/L (disp_beam_column2d.mojo:137-140)

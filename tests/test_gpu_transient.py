@@ -105,3 +105,48 @@ def test_gpu_disp_beam_column_frames_match_oracle(column_type, beam_type):
     err = _rel_err(got["u"][:n], ref["u"][:n])
     print(f"max relative displacement error {err:.3e}")
     assert err <= REL
+
+
+@pytest.mark.parametrize("variant", ["all-elastic", "elastic-beams"])
+def test_gpu_all_elastic_sections_take_the_exact_state(variant):
+    """forceBeamColumn2d elements whose sections have only elastic fibers take
+    the reference's exact state (force_beam_column2d.mojo:608-732, tried
+    before the local iteration at :1577-1620): on the whole frame (a linear
+    problem: two assemblies per step) or on its beams only, next to RC
+    columns.  Identical Newton and section-evaluation counts, displacements
+    within 1e-8 of the oracle, whose exact state is pinned against an
+    independent linear Newmark solution."""
+    import copy
+
+    case = cases.config0_case(steps=300, seed=3)
+    if variant == "all-elastic":
+        case["materials"] = [{"id": m["id"], "type": "Elastic", "params": {"E": 3.0e7}} for m in case["materials"]]
+    else:
+        case["materials"].append({"id": 99, "type": "Elastic", "params": {"E": 3.0e7}})
+        beam = copy.deepcopy(next(s for s in case["sections"] if s["id"] == 2))
+        beam["id"] = 3
+        for part in beam["params"].get("patches", []) + beam["params"].get("layers", []):
+            part["material"] = 99
+        case["sections"].append(beam)
+        for e in case["elements"]:
+            if e["section"] == 2:
+                e["section"] = 3
+    fc = load_frame2d_case(case)
+    ref = oracle.run_transient2d(fc)
+    assert ref["status"] == 0
+    if variant == "all-elastic":
+        assert np.all(ref["iters"] == 2)
+    fb = FrameBatch(fc.materials)
+    f = fb.add_frame(fc, record_all=True)
+    i = fb.add_case_motion(f, fc)
+    fb.finalize()
+    fb.run(rounds=32)
+    got = fb.result(i)
+    sec_evals = int(fb.section_evals()[i])
+    fb.close()
+    print(f"{variant}: iterations {got['iters'].sum()} vs {ref['iters'].sum()}, section evaluations {sec_evals} vs "
+          f"{ref['section_evals']}, max relative error {_rel_err(got['u'], ref['u']):.2e}", flush=True)
+    assert got["status"] == 0 and got["steps_done"] == 300
+    np.testing.assert_array_equal(got["iters"], ref["iters"])
+    assert sec_evals == ref["section_evals"]
+    assert _rel_err(got["u"], ref["u"]) <= REL
