@@ -231,6 +231,54 @@ int sgpu_elem3d_commit(sgpu_ctx *ctx, int n, const int32_t *elem_ids);
 int sgpu_elem3d_basic_state(sgpu_ctx *ctx, int32_t elem_id, double *q);
 
 /* ------------------------------------------------------------------
+ * 2D fiber beam-column elements outside any frame: the element calls of
+ * the 2D frame assembly (stiffness_internal_frame2d.mojo:336-634) for a
+ * host that scatters them itself.  The device routines are the 2D frame
+ * path's (forceBeamColumn2d with its local iteration, subdivision, initial-
+ * tangent fallback and exact all-elastic state; dispBeamColumn2d), Linear
+ * transformation, no element loads.
+ * ------------------------------------------------------------------ */
+#define SGPU_MAX_NIP2D 5
+enum { SGPU_ELEM2D_FORCE = 0, SGPU_ELEM2D_DISP = 1 };
+
+/* type, 2D fiber-section definition, integration rule (points on [0, 1],
+ * weights summing to 1: beam_integration.mojo:224-241), end coordinates. */
+typedef struct {
+    int32_t type;
+    int32_t def_id;
+    int32_t nip;
+    int32_t pad_;
+    double xy1[2], xy2[2];
+    double xi[SGPU_MAX_NIP2D], w[SGPU_MAX_NIP2D];
+} sgpu_elem2d_desc;
+
+/* Register an element before sgpu_sections_finalize: allocates its nip
+ * section instances of def_id and a zero force-based basic state
+ * [q0..q2 | eps0[nip] | kappa[nip] | v0..v2] (loader.mojo:1760-1869). */
+int sgpu_elem2d_add(sgpu_ctx *ctx, const sgpu_elem2d_desc *desc, int32_t *elem_id);
+
+/* State determination of n distinct listed elements at their global end
+ * displacements u6[6 n]: global tangents k36[36 n] (row major) and
+ * resisting forces f6[6 n].  Replaces
+ * force_beam_column2d_fiber_global_tangent_and_internal
+ * (force_beam_column2d.mojo:1418-1919) and the dispBeamColumn2d state
+ * (disp_beam_column2d.mojo:114-254) in batch.  Force-based elements keep
+ * their trial basic state across calls (Newton iterations).  status[i] =
+ * SGPU_ERR_LOCAL_NONCONV where the reference aborts (no tangent could be
+ * recovered); the call then returns that code.  Host buffers, synchronous. */
+int sgpu_elem2d_state(sgpu_ctx *ctx, int n, const int32_t *elem_ids, const double *u6, double *k36, double *f6,
+                      int32_t *status);
+
+/* Commit the listed elements' sections at their last evaluated
+ * deformations (transient_nonlinear.mojo:3081-3101); with u6 (the
+ * converged end displacements, or NULL) a force-based element's basic
+ * deformation is re-synced from them (helpers.mojo:1832-1883). */
+int sgpu_elem2d_commit(sgpu_ctx *ctx, int n, const int32_t *elem_ids, const double *u6);
+
+/* The forceBeamColumn2d element's basic state (3 + 2 nip + 3 values). */
+int sgpu_elem2d_basic_state(sgpu_ctx *ctx, int32_t elem_id, double *q);
+
+/* ------------------------------------------------------------------
  * Multi-point constraints of the Transformation handler (rigidDiaphragm,
  * equalDOF).  DOF i of n expands as u_i = sum_t coeff_pool[t] u[dof_pool[t]]
  * over t in [row_offsets[i], row_offsets[i + 1]) -- the tables

@@ -78,6 +78,10 @@ EXPORTED = (
     "sgpu_frame2d_set_predictor",
     "sgpu_frame2d_inst_stats",
     "sgpu_frame2d_section_evals",
+    "sgpu_elem2d_add",
+    "sgpu_elem2d_state",
+    "sgpu_elem2d_commit",
+    "sgpu_elem2d_basic_state",
     "sgpu_frame2d_launch_queue",
     "sgpu_elem3d_add",
     "sgpu_elem3d_state",

@@ -23,6 +23,8 @@ struct MpcHost;
 void mpc_host_free(MpcHost *h);
 struct Frame3dHost;
 void frame3d_host_free(Frame3dHost *h);
+struct Elem2dHost;
+void elem2d_host_free(Elem2dHost *h);
 }
 
 namespace sgpu {
@@ -95,6 +97,8 @@ struct sgpu_ctx {
     sgpu::MpcHost *mpc = nullptr;
     // 3D frame transient (frame3d.hip)
     sgpu::Frame3dHost *frame3d = nullptr;
+    // 2D elements outside any frame (elem2d_api.cpp)
+    sgpu::Elem2dHost *elems2d = nullptr;
     SlotTables tables() const {
         SlotTables tb;
         tb.y = d_tab_y;

@@ -746,6 +746,64 @@ __global__ __launch_bounds__(kElemThreads) void frame2d_elem_kernel(Frame2dArgs 
     element_state(A, inst, e, L, mats, st.commit_pending != 0);
 }
 
+// Element-level state determination of listed 2D elements that belong to no
+// frame (the sgpu_elem2d entry points): one element per block, each at its
+// own six global end displacements u6 (element DOF k reads u6[6 i + k], fmap
+// being the identity), its 6 x 6 global tangent and 6 forces written to
+// out42[42 i ..].  status[i] = 1 where the reference aborts.
+__global__ __launch_bounds__(kElemThreads) void elem2d_state_kernel(Frame2dArgs A, const int32_t *ids,
+                                                                    const double *u6, double *out42,
+                                                                    int32_t *status) {
+    __shared__ ElemLds L;
+    const int i = blockIdx.x;
+    const int e = ids[i];
+    const MatDev *mats = stage_materials(A, L);
+    const InstDev I{};
+    const ElemDev &el = A.elems[e];
+    const int w = threadIdx.x >> 6;
+    double *out = out42 + 42 * (int64_t)i;
+    const double *u = u6 + 6 * (int64_t)i;
+    bool ok = true;
+    if (el.type == 1)
+        dbc2d_element(A, 0, I, el, e, u, out, &L.sh, mats, L.wred[w], reinterpret_cast<double *>(&L.scr[0]));
+    else
+        ok = fbc2d_element(A, 0, I, el, e, A.q + el.q_off, u, out, &L.sh, L.qbuf[w], L.scr[w], mats, L.wred[w]);
+    if (threadIdx.x == 0) status[i] = ok ? 0 : 1;
+}
+
+// Commit of listed 2D elements (transient_nonlinear.mojo:3081-3101): one
+// wavefront per integration point rebuilds the last evaluated trial state
+// into the committed arrays; with end displacements u6, a force-based
+// element's basic deformation is then re-synced from them as after a
+// converged step (helpers.mojo:1832-1883).
+__global__ __launch_bounds__(64 * kMaxNipDev) void elem2d_commit_kernel(Frame2dArgs A, const int32_t *ids,
+                                                                        const double *u6) {
+    const int i = blockIdx.x;
+    const int e = ids[i];
+    const ElemDev &el = A.elems[e];
+    const int w = threadIdx.x >> 6;
+    if (w < el.nip) {
+        const double *last = A.last + 2 * kMaxNipDev * e + 2 * w;
+        commit_section_2d(A.defs[el.def], A.tb, A.mats, A.c, A.elem_inst[e] + w, last[0], last[1]);
+    }
+    if (u6 && el.type == 0 && threadIdx.x == 0) {
+        const double *ug = u6 + 6 * (int64_t)i;
+        const double cs = el.c2, sn = el.s2;
+        double ul[6];
+        ul[0] = cs * ug[0] + sn * ug[1];
+        ul[1] = -sn * ug[0] + cs * ug[1];
+        ul[2] = ug[2];
+        ul[3] = cs * ug[3] + sn * ug[4];
+        ul[4] = -sn * ug[3] + cs * ug[4];
+        ul[5] = ug[5];
+        const double chord = (ul[4] - ul[1]) / el.L2;
+        double *b = A.q + el.q_off + 3 + 2 * el.nip;
+        b[0] = ul[3] - ul[0];
+        b[1] = ul[2] - chord;
+        b[2] = ul[5] - chord;
+    }
+}
+
 // K_ff band (gather in element order) into `band`, F_int into `fint`.
 __device__ void gather(const Frame2dArgs &A, const InstDev &I, const FrameDev &F, double *band, double *fint) {
     const double *out = A.scratch + I.scratch_base;
@@ -1532,6 +1590,19 @@ hipError_t launch_frame2d_queue(hipStream_t stream, const Frame2dArgs &args, con
 hipError_t launch_frame2d_persistent(hipStream_t stream, const Frame2dArgs &args, int n_inst, int max_steps,
                                      size_t lds_bytes) {
     hipLaunchKernelGGL(frame2d_inst_kernel, dim3(n_inst), dim3(kElemThreads), lds_bytes, stream, args, max_steps);
+    return hipGetLastError();
+}
+
+// The sgpu_elem2d entry points' launches (n listed elements, one block each).
+hipError_t launch_elem2d_state(hipStream_t stream, const Frame2dArgs &args, int n, const int32_t *ids,
+                               const double *u6, double *out42, int32_t *status) {
+    hipLaunchKernelGGL(elem2d_state_kernel, dim3(n), dim3(kElemThreads), 0, stream, args, ids, u6, out42, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_elem2d_commit(hipStream_t stream, const Frame2dArgs &args, int n, const int32_t *ids,
+                                const double *u6) {
+    hipLaunchKernelGGL(elem2d_commit_kernel, dim3(n), dim3(64 * kMaxNipDev), 0, stream, args, ids, u6);
     return hipGetLastError();
 }
 

@@ -376,6 +376,7 @@ int sgpu_ctx_destroy(sgpu_ctx *ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     sgpu::frame2d_host_free(ctx->frames);
     sgpu::elem3d_host_free(ctx->elems3d);
+    sgpu::elem2d_host_free(ctx->elems2d);
     sgpu::mpc_host_free(ctx->mpc);
     sgpu::frame3d_host_free(ctx->frame3d);
     delete ctx;
