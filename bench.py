@@ -176,6 +176,8 @@ def _cpu_task(name: str, *a):
         out = frame3d_cpu_baseline(*a)
     elif name == "ensemble_record":
         out = _oracle_record(*a)
+    elif name == "start_sensitivity":
+        out = _start_sensitivity(*a)
     else:
         raise ValueError(name)
     out["cpu"] = _CPU_SLOT
@@ -200,6 +202,28 @@ def _oracle_record(k: int, seed: int, max_steps: int, step_cap: int) -> dict:
             "iters": [int(v) for v in r["iters"][:n]],
             "newton_iters": int(r["iters"][:fc.steps].sum()), "section_evals": int(r["section_evals"]),
             "seconds": dt}
+
+
+def _start_sensitivity(seed: int, max_steps: int) -> dict:
+    """Whether the oracle's own run of a 4 x 3 frame record changes its stop
+    or moves its peak drift beyond 1e-6 relative when every step's Newton
+    start is scaled by 1 +- 1e-4 or 1 +- 1e-2 on alternate DOFs."""
+    from oracle import oracle
+    from strut_amd import cases
+
+    fc = _motion_case(cases.config2_frame_case(4, 3, steps=100), seed, max_steps)
+    base = oracle.run_transient2d(fc)
+    n = base["steps_done"]
+    for eps in (1e-4, 1e-2):
+        r = oracle.run_transient2d(fc, start_eps=eps)
+        if r["status"] != base["status"] or r["steps_done"] != n:
+            return {"evidence": f"start x(1+-{eps:g}): stop {r['status']} instead of {base['status']}"}
+        if n:
+            d0 = cases.peak_story_drift(base["u"][:n], 4, 3)
+            e = float(np.max(np.abs(cases.peak_story_drift(r["u"][:n], 4, 3) - d0) / d0))
+            if e > 1e-6:
+                return {"evidence": f"start x(1+-{eps:g}): peak drift moves {e:.2e}"}
+    return {"evidence": ""}
 
 
 def cpu_baseline(mats, cells, budget_s: float = 12.0) -> dict:
@@ -601,7 +625,8 @@ def ensemble_leg(n_motions: int, max_steps: int, n_single: int = 4, cpu_workers:
     return out
 
 
-def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
+def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int, sens_records: int = 16,
+                cpu_workers: int = 16) -> dict:
     """Learned predictor on the 4 x 3 frame: train on one seed range, compare
     with the plain predictor on held-out records."""
     from strut_amd import _native, cases
@@ -639,19 +664,28 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
     it_l = sum(int(b["iters"][:b["steps_done"]].sum()) for b in rl)
     steps_p = sum(a["steps_done"] for a in rp)
     steps_l = sum(b["steps_done"] for b in rl)
-    errs, errs_stalled, over = [], [], []
-    for a, b, ok in zip(rp, rl, same):
-        n = a["steps_done"]
-        if not ok or n == 0:
-            continue
-        dp = cases.peak_story_drift(a["u"][:n], 4, 3)
-        dl = cases.peak_story_drift(b["u"][:n], 4, 3)
-        e = float(np.max(np.abs(dl - dp) / dp))
-        stalled = int(a["iters"][:n].max()) > 10
-        (errs_stalled if stalled else errs).append(e)
-        if e > 1e-6 and not stalled:
-            over.append({"status": int(a["status"]), "steps": int(n), "max_iters": int(a["iters"][:n].max()),
-                         "drift_rel_err": e})
+    # every record: within the bar (same stop, peak drift within 1e-6), or
+    # checked for the reference's own sensitivity to where Newton starts
+    errs, flagged = [], []
+    for j, (a, b, ok) in enumerate(zip(rp, rl, same)):
+        n = min(a["steps_done"], b["steps_done"])
+        e = 0.0
+        if n:
+            dp = cases.peak_story_drift(a["u"][:n], 4, 3)
+            dl = cases.peak_story_drift(b["u"][:n], 4, 3)
+            e = float(np.max(np.abs(dl - dp) / dp))
+        errs.append(e)
+        if not ok or e > 1e-6:
+            flagged.append({"seed": 300000 + j, "stop_plain": int(a["status"]), "stop_learned": int(b["status"]),
+                            "max_iters": int(a["iters"][:max(n, 1)].max()), "drift_rel_err": e})
+    checked = flagged[:sens_records]
+    if checked and sens_records > 0:
+        pool, _ = cpu_pool(cpu_workers)
+        with pool:
+            futs = [pool.submit(_cpu_task, "start_sensitivity", f["seed"], max_steps) for f in checked]
+            for f, fu in zip(checked, futs):
+                f["oracle_start_sensitivity"] = fu.result()["evidence"]
+    within = sum(1 for e, ok in zip(errs, same) if ok and e <= 1e-6)
     out.update({
         "schedule": "queue",
         "newton_iters_per_step_plain": it_p / steps_p, "newton_iters_per_step_learned": it_l / steps_l,
@@ -659,11 +693,10 @@ def learned_leg(n_train: int, n_test: int, max_steps: int, epochs: int) -> dict:
         "gm_steps_per_s_plain": steps_p / sp, "gm_steps_per_s_learned": steps_l / sl,
         "proposals": int(sum(s["proposals"] for s in stl)), "fallbacks": int(sum(s["fallbacks"] for s in stl)),
         "records_same_stop": int(sum(same)),
-        "peak_drift_rel_err_max": max(errs) if errs else None,
-        "records_within_1e-6": int(sum(1 for e in errs if e <= 1e-6)), "records_compared": len(errs),
-        "records_over_1e-6": over,
-        "records_with_stalled_step": len(errs_stalled),
-        "stalled_records_drift_rel_err_max": max(errs_stalled) if errs_stalled else None,
+        "records_within_bar": within, "records_outside_bar_count": len(flagged),
+        "outside_bar_checked_on_oracle": len([f for f in checked if "oracle_start_sensitivity" in f]),
+        "outside_bar_oracle_also_sensitive": len([f for f in checked if f.get("oracle_start_sensitivity")]),
+        "records_outside_bar": flagged,
     })
     return out
 
@@ -750,8 +783,13 @@ def main() -> int:
                       f"{e['cpu_baseline']['cores']} processes; high-PGA checks {e['high_pga_stop_checks']}")
     if args.learned_train > 0:
         config["learned"] = learned_leg(args.learned_train, args.learned_test, args.learned_steps,
-                                        args.learned_epochs)
-        _progress(f"learned leg: iteration reduction {config['learned']['newton_iter_reduction']:.3g}")
+                                        args.learned_epochs, sens_records=0 if args.no_cpu_baseline else 16,
+                                        cpu_workers=args.cpu_workers)
+        ll = config["learned"]
+        _progress(f"learned leg: iteration reduction {ll['newton_iter_reduction']:.3g}, GM-steps/s "
+                  f"{ll['gm_steps_per_s_learned']:.4g} vs {ll['gm_steps_per_s_plain']:.4g}, within bar "
+                  f"{ll['records_within_bar']}, outside {ll['records_outside_bar_count']} (oracle as sensitive: "
+                  f"{ll['outside_bar_oracle_also_sensitive']} of {ll['outside_bar_checked_on_oracle']} checked)")
     cpu_main = None
     if not args.no_cpu_baseline:  # the single-core baselines, one after another in one pinned process
         pool, _ = cpu_pool(1)

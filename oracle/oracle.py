@@ -234,7 +234,7 @@ class _Tr2dCase(ctypes.Structure):
         ("max_iters", c_int), ("alpha_m", c_double), ("beta_k", c_double), ("beta_k_init", c_double),
         ("exc_dir", c_int), ("ts_n", c_int), ("ts_values", POINTER(c_double)), ("ts_dt", c_double),
         ("ts_factor", c_double), ("ts_start", c_double), ("ts_use_last", c_int),
-        ("elem_type", POINTER(c_int)),
+        ("elem_type", POINTER(c_int)), ("start_eps", c_double),
     ]
 
 
@@ -246,10 +246,11 @@ class _Tr2dOut(ctypes.Structure):
     ]
 
 
-def run_transient2d(fc, record_history: bool = True) -> dict:
+def run_transient2d(fc, record_history: bool = True, start_eps: float = 0.0) -> dict:
     """Run a loaded 2D fiber-frame case (strut_amd.frame2d.Frame2dCase) on
     the oracle; returns the displacement history and per-step Newton
-    iteration counts."""
+    iteration counts.  start_eps != 0 (not the reference) scales every
+    step's Newton start by 1 + start_eps (-1)^i on free DOF i."""
     lib = load()
     lib.or_tr2d_run.restype = c_int
     lib.or_tr2d_run.argtypes = [POINTER(_Tr2dCase), POINTER(_Tr2dOut)]
@@ -291,6 +292,7 @@ def run_transient2d(fc, record_history: bool = True) -> dict:
     c.ts_values = _d_out(keep["ts"]) if len(keep["ts"]) else None
     c.ts_dt, c.ts_factor, c.ts_start, c.ts_use_last = fc.ts_dt, fc.ts_factor, fc.ts_start, fc.ts_use_last
     c.elem_type = keep["et"].ctypes.data_as(POINTER(c_int))
+    c.start_eps = start_eps
     o = _Tr2dOut()
     u_hist = np.zeros((fc.steps, 3 * fc.n_nodes)) if record_history else None
     iters = np.zeros(fc.steps, dtype=np.int32)

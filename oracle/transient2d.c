@@ -410,6 +410,8 @@ int or_tr2d_run(const or_tr2d_case *c, or_tr2d_out *out) {
         memcpy(an, af, sizeof(double) * n);
         double coeff = dt * dt * (0.5 - beta); /* :553-590 */
         for (int i = 0; i < n; ++i) uf[i] = un[i] + dt * vn[i] + coeff * an[i];
+        if (c->start_eps != 0.0)
+            for (int i = 0; i < n; ++i) uf[i] *= 1.0 + ((i & 1) ? -c->start_eps : c->start_eps);
         for (int i = 0; i < n; ++i) u[free_list[i]] = uf[i];
         double t = (double)(step + 1) * dt;
         memset(Fext, 0, sizeof(double) * total);

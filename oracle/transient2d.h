@@ -28,6 +28,11 @@ typedef struct {
     double ts_dt, ts_factor, ts_start;
     int ts_use_last;
     const int *elem_type;    /* 0 forceBeamColumn2d, 1 dispBeamColumn2d per element; NULL: all 0 */
+    /* Not in the reference (0 restates it): every step's Newton start, the
+     * Newmark predictor, scaled by 1 + start_eps (-1)^i on free DOF i -- a
+     * probe of how far the converged solution depends on where Newton
+     * starts, as a learned proposal moves it. */
+    double start_eps;
 } or_tr2d_case;
 
 typedef struct {

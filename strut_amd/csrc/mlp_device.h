@@ -7,9 +7,12 @@
 // rows and columns of every weight matrix are zero and stay zero (their
 // gradients are zero).
 //
-// Matrix-core tile (v_mfma_f32_16x16x16_bf16): for a 16 x 16 x 16 product
-// D = A B, lane l holds A[l % 16][4 (l / 16) + e] and B[4 (l / 16) + e][l % 16]
-// (e = 0..3) and receives D[4 (l / 16) + e][l % 16].
+// Matrix-core tiles.  v_mfma_f32_16x16x32_bf16 (gfx950) takes 32 of the
+// inner dimension per instruction: for D = A B (16 x 16 x 32), lane l holds
+// A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16] (e = 0..7); an
+// inner dimension that ends in 16 takes one v_mfma_f32_16x16x16_bf16 step,
+// where lane l holds A[l % 16][4 (l / 16) + e] and B[4 (l / 16) + e][l % 16]
+// (e = 0..3).  Both leave D[4 (l / 16) + e][l % 16] (e = 0..3) in lane l.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,7 +21,21 @@
 namespace sgpu {
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x8s_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// One 16 x 16 x 32 step from two 4-element halves of each operand (8-byte
+// loads: activation rows in LDS are only 8-byte aligned).
+__device__ __forceinline__ f32x4_t mfma_16x16x32(bf16x4_t a_lo, bf16x4_t a_hi, bf16x4_t b_lo, bf16x4_t b_hi,
+                                                 f32x4_t acc) {
+    const bf16x8s_t a = __builtin_shufflevector(a_lo, a_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    const bf16x8s_t b = __builtin_shufflevector(b_lo, b_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                  acc, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4_t load4(const uint16_t *p) { return *reinterpret_cast<const bf16x4_t *>(p); }
 
 constexpr int kMlpHidden = 256;
 constexpr int kMlpLayers = 5;
@@ -64,32 +81,39 @@ __device__ __forceinline__ void mlp_tile_gemm(const uint16_t *A, int lda, int ro
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     const bool arow = r < rows;
-    const uint16_t *ap = A + r * lda + 4 * g;
+    const uint16_t *ap = A + r * lda + 8 * g;  // 16 x 16 x 32 steps
+    const uint16_t *ap16 = A + r * lda + 4 * g;  // the 16 x 16 x 16 tail
     for (int jt = wave; jt < N / 16; jt += nw) {
-        const uint16_t *wp = W + (size_t)(jt * 16 + r) * K + 4 * g;
+        const uint16_t *wp = W + (size_t)(jt * 16 + r) * K + 8 * g;
+        const uint16_t *wp16 = W + (size_t)(jt * 16 + r) * K + 4 * g;
         f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
         int kb = 0;
         for (; kb + 64 <= K; kb += 64) {  // four independent weight loads in flight
-            bf16x4_t b0 = *reinterpret_cast<const bf16x4_t *>(wp + kb);
-            bf16x4_t b1 = *reinterpret_cast<const bf16x4_t *>(wp + kb + 16);
-            bf16x4_t b2 = *reinterpret_cast<const bf16x4_t *>(wp + kb + 32);
-            bf16x4_t b3 = *reinterpret_cast<const bf16x4_t *>(wp + kb + 48);
+            const bf16x4_t b0 = load4(wp + kb), b1 = load4(wp + kb + 4);
+            const bf16x4_t b2 = load4(wp + kb + 32), b3 = load4(wp + kb + 36);
             bf16x4_t a0 = {0, 0, 0, 0}, a1 = a0, a2 = a0, a3 = a0;
             if (arow) {
-                a0 = *reinterpret_cast<const bf16x4_t *>(ap + kb);
-                a1 = *reinterpret_cast<const bf16x4_t *>(ap + kb + 16);
-                a2 = *reinterpret_cast<const bf16x4_t *>(ap + kb + 32);
-                a3 = *reinterpret_cast<const bf16x4_t *>(ap + kb + 48);
+                a0 = load4(ap + kb);
+                a1 = load4(ap + kb + 4);
+                a2 = load4(ap + kb + 32);
+                a3 = load4(ap + kb + 36);
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b2, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, b3, acc, 0, 0, 0);
+            acc = mfma_16x16x32(a0, a1, b0, b1, acc);
+            acc = mfma_16x16x32(a2, a3, b2, b3, acc);
         }
-        for (; kb < K; kb += 16) {
-            bf16x4_t b0 = *reinterpret_cast<const bf16x4_t *>(wp + kb);
+        for (; kb + 32 <= K; kb += 32) {
+            const bf16x4_t b0 = load4(wp + kb), b1 = load4(wp + kb + 4);
+            bf16x4_t a0 = {0, 0, 0, 0}, a1 = a0;
+            if (arow) {
+                a0 = load4(ap + kb);
+                a1 = load4(ap + kb + 4);
+            }
+            acc = mfma_16x16x32(a0, a1, b0, b1, acc);
+        }
+        if (kb < K) {  // K = 16 mod 32
+            const bf16x4_t b0 = load4(wp16 + kb);
             bf16x4_t a0 = {0, 0, 0, 0};
-            if (arow) a0 = *reinterpret_cast<const bf16x4_t *>(ap + kb);
+            if (arow) a0 = load4(ap16 + kb);
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc, 0, 0, 0);
         }
 #pragma unroll
@@ -107,21 +131,35 @@ __device__ __forceinline__ void mlp_row_gemm(const uint16_t *A, const uint16_t *
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int r = lane & 15, g = lane >> 4;
     for (int jt = wave; jt < N / 16; jt += nw) {
-        const uint16_t *wp = W + (size_t)(jt * 16 + r) * K + 4 * g;
+        const uint16_t *wp = W + (size_t)(jt * 16 + r) * K + 8 * g;
+        const uint16_t *wp16 = W + (size_t)(jt * 16 + r) * K + 4 * g;
         f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int kc = 0; kc < K; kc += 256) {
-            const int nk = (K - kc) / 16 < 16 ? (K - kc) / 16 : 16;
-            bf16x4_t b[16];
+            const int nk = (K - kc) / 32 < 8 ? (K - kc) / 32 : 8;  // 16 x 16 x 32 steps of this chunk
+            bf16x4_t bl[8], bh[8];
 #pragma unroll
-            for (int t = 0; t < 16; ++t)
-                if (t < nk) b[t] = *reinterpret_cast<const bf16x4_t *>(wp + kc + 16 * t);
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
+            for (int t = 0; t < 8; ++t)
                 if (t < nk) {
-                    bf16x4_t a = {0, 0, 0, 0};
-                    if (r == 0) a = *reinterpret_cast<const bf16x4_t *>(A + 4 * g + kc + 16 * t);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc, 0, 0, 0);
+                    bl[t] = load4(wp + kc + 32 * t);
+                    bh[t] = load4(wp + kc + 32 * t + 4);
                 }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                if (t < nk) {
+                    bf16x4_t al = {0, 0, 0, 0}, ah = al;
+                    if (r == 0) {
+                        al = load4(A + 8 * g + kc + 32 * t);
+                        ah = load4(A + 8 * g + kc + 32 * t + 4);
+                    }
+                    acc = mfma_16x16x32(al, ah, bl[t], bh[t], acc);
+                }
+            }
+            const int kt = kc + 32 * nk;
+            if (nk < 8 && kt < K) {  // K = 16 mod 32: one 16 x 16 x 16 step
+                const bf16x4_t b0 = load4(wp16 + kt);
+                bf16x4_t a0 = {0, 0, 0, 0};
+                if (r == 0) a0 = load4(A + 4 * g + kt);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc, 0, 0, 0);
             }
         }
         if (g == 0) epi(jt * 16 + r, acc[0]);  // row 0 of D: lanes 0..15, element 0

@@ -143,13 +143,23 @@ __global__ __launch_bounds__(256) void mlp_wgrad_kernel(MlpTrainArgs T, MlpTiles
     const int nt = tt / (K / 16), kt = tt % (K / 16);
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
     const int B = T.batch;
-    const uint16_t *dz = T.dzt[l] + (int64_t)(nt * 16 + r) * B + 4 * g;
-    const uint16_t *hh = (l == 0 ? T.xt : T.ht[l - 1]) + (int64_t)(kt * 16 + r) * B + 4 * g;
+    const uint16_t *dz_row = T.dzt[l] + (int64_t)(nt * 16 + r) * B;
+    const uint16_t *hh_row = (l == 0 ? T.xt : T.ht[l - 1]) + (int64_t)(kt * 16 + r) * B;
     f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
     float bs = 0.0f;
-    for (int mb = 0; mb < B; mb += 16) {
-        const bf16x4_t a = *reinterpret_cast<const bf16x4_t *>(dz + mb);
-        const bf16x4_t b = *reinterpret_cast<const bf16x4_t *>(hh + mb);
+    int mb = 0;
+    for (; mb + 32 <= B; mb += 32) {  // the batch is the inner dimension: 16 x 16 x 32 steps
+        const bf16x4_t a0 = load4(dz_row + mb + 8 * g), a1 = load4(dz_row + mb + 8 * g + 4);
+        const bf16x4_t b0 = load4(hh_row + mb + 8 * g), b1 = load4(hh_row + mb + 8 * g + 4);
+        acc = mfma_16x16x32(a0, a1, b0, b1, acc);
+        if (kt == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bs += bf16_to_f32((uint16_t)a0[e]) + bf16_to_f32((uint16_t)a1[e]);
+        }
+    }
+    if (mb < B) {  // B = 16 mod 32
+        const bf16x4_t a = load4(dz_row + mb + 4 * g);
+        const bf16x4_t b = load4(hh_row + mb + 4 * g);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         if (kt == 0) {
 #pragma unroll

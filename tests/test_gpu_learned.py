@@ -1,24 +1,27 @@
 """The learned Newton predictor in the device transient (a reduced form of
 the learned-predictor configuration): the 4-story x 3-bay frame, a network
 trained on the device from trajectories the solver records on one seed
-range, then held-out records of another seed range run with the plain
-Newmark predictor and with the learned one.
+range, then held-out records of another seed range (every PGA of the
+ensemble's grid) run with the plain Newmark predictor and with the learned
+one.
 
-Checked: the learned runs stop where the plain runs stop, use fewer Newton
-iterations in total, and on records without a stalled Newton step their peak
-inter-story drifts stay within 1e-6 relative of the plain FP64 solution;
-proposals replaced by the Newmark predictor are counted.
-
-A stalled step (more than 10 Newton iterations) converges to a point that
-depends on where Newton starts: on held-out seed 9007 step 374 takes 19
-iterations in the oracle, and the oracle itself, started 10% closer to the
-converged displacement, moves that step's solution by 1.7e-7 relative and
-the record's peak drift by 1.8e-6.  Such records are reported, not held to
-the 1e-6 bar."""
+Checked: the learned runs use fewer Newton iterations in total; proposals
+replaced by the Newmark predictor are counted; on every record the learned
+run stops where the plain run stops and its peak inter-story drifts stay
+within 1e-6 relative of the plain FP64 solution -- unless the reference
+itself is that sensitive to where Newton starts.  Newton converges to a
+point within the residual tolerance that depends on its start; near a
+stalled step or an approaching stop, path-dependent hysteresis carries that
+difference on.  The oracle shows it without any network: started every step
+from the Newmark predictor scaled by 1 +- 1e-4 or 1 +- 1e-2 on alternate
+DOFs, it moves such records' peak drifts by up to 3e-4 and changes their
+stop steps, while most records stay within 1e-6.  A record outside the bar must show that sensitivity in the
+oracle."""
 
 import numpy as np
 import pytest
 
+from oracle import oracle
 from strut_amd import _native, cases
 from strut_amd.frame2d import load_frame2d_case
 from strut_amd.learned import train_predictor
@@ -52,11 +55,30 @@ def _batch(fcs, record=False, net=None):
     return fb, fid, ids
 
 
+def _oracle_sensitive(fc) -> str:
+    """Whether the oracle's own run of the record changes stop or peak drift
+    (beyond 1e-6 relative) when each step's Newton start is moved by 1e-4 or
+    1e-2 relative; returns the evidence, or '' if it does not."""
+    base = oracle.run_transient2d(fc)
+    n = base["steps_done"]
+    for eps in (1e-4, 1e-2):
+        r = oracle.run_transient2d(fc, start_eps=eps)
+        if r["status"] != base["status"] or r["steps_done"] != n:
+            return f"start moved by {eps:g}: stop {r['status']} instead of {base['status']}"
+        if n:
+            d0 = cases.peak_story_drift(base["u"][:n], 4, 3)
+            d1 = cases.peak_story_drift(r["u"][:n], 4, 3)
+            e = float(np.max(np.abs(d1 - d0) / d0))
+            if e > 1e-6:
+                return f"start moved by {eps:g}: peak drift moves {e:.1e}"
+    return ""
+
+
 def test_gpu_learned_predictor_gain_and_drift():
     frame = cases.config2_frame_case(4, 3, steps=100)
     pga = lambda s: cases.config2_motion_params(s)["pga_g"]
     train = [_case(frame, s) for s in range(7000, 7400) if pga(s) <= 0.6][:64]
-    held = [_case(frame, s) for s in range(9000, 9100) if pga(s) <= 0.6][:6]
+    held = [_case(frame, s) for s in range(9000, 9010)]
     fb, fid, _ = _batch(train, record=True)
     x, y = fb.samples(fid)
     print("training pairs", x.shape[0], flush=True)
@@ -68,26 +90,33 @@ def test_gpu_learned_predictor_gain_and_drift():
     plain, _, pid = _batch(held)
     learned, _, lid = _batch(held, net=net)
     it_p = it_l = 0
-    held_to_bar = 0
+    held_to_bar = sensitive = 0
     for k, (a, b) in enumerate(zip(pid, lid)):
         rp, rl = plain.result(a), learned.result(b)
         sl = learned.inst_stats(b)
-        assert rp["status"] == rl["status"] and rp["steps_done"] == rl["steps_done"]
-        n = rp["steps_done"]
-        it_p += int(rp["iters"][:n].sum())
-        it_l += int(rl["iters"][:n].sum())
-        dp = cases.peak_story_drift(rp["u"][:n], 4, 3)
-        dl = cases.peak_story_drift(rl["u"][:n], 4, 3)
-        err = float(np.max(np.abs(dl - dp) / dp))
-        stalled = int(rp["iters"][:n].max()) > 10
-        print(f"record {k}: iterations plain {rp['iters'][:n].sum()} learned {rl['iters'][:n].sum()} "
-              f"proposals {sl['proposals']} fallbacks {sl['fallbacks']} drift rel err {err:.2e}"
-              f"{' (stalled step: reported only)' if stalled else ''}")
-        assert sl["proposals"] >= n and sl["fallbacks"] < sl["proposals"]
-        if not stalled:
-            assert err <= 1e-6
+        n = min(rp["steps_done"], rl["steps_done"])
+        it_p += int(rp["iters"][:rp["steps_done"]].sum())
+        it_l += int(rl["iters"][:rl["steps_done"]].sum())
+        same_stop = rp["status"] == rl["status"] and rp["steps_done"] == rl["steps_done"]
+        err = 0.0
+        if n:
+            dp = cases.peak_story_drift(rp["u"][:n], 4, 3)
+            dl = cases.peak_story_drift(rl["u"][:n], 4, 3)
+            err = float(np.max(np.abs(dl - dp) / dp))
+        line = (f"record {k} (PGA {pga(9000 + k):.2f} g): stop {rp['status']}/{rl['status']} iterations plain "
+                f"{rp['iters'][:rp['steps_done']].sum()} learned {rl['iters'][:rl['steps_done']].sum()} proposals "
+                f"{sl['proposals']} fallbacks {sl['fallbacks']} drift rel err {err:.2e}")
+        assert sl["proposals"] >= rl["steps_done"] and sl["fallbacks"] < max(sl["proposals"], 1)
+        if same_stop and err <= 1e-6:
             held_to_bar += 1
-    print(f"total iterations plain {it_p} learned {it_l}: {100.0 * (1 - it_l / it_p):.1f}% fewer")
+            print(line, flush=True)
+            continue
+        why = _oracle_sensitive(held[k])
+        print(line + f"; the oracle is as sensitive: {why or 'NO'}", flush=True)
+        assert why, "the learned run leaves the bar on a record the reference holds to it"
+        sensitive += 1
+    print(f"total iterations plain {it_p} learned {it_l}: {100.0 * (1 - it_l / it_p):.1f}% fewer; "
+          f"{held_to_bar} records within the bar, {sensitive} start-sensitive")
     assert it_l < it_p
     assert held_to_bar >= 4
     plain.close()
