@@ -428,6 +428,20 @@ def beam3d_rotation(xyz1, xyz2, vecxz=None) -> np.ndarray:
     return R.reshape(3, 3)
 
 
+def shell4_stiffness(E: float, nu: float, h: float, xyz) -> np.ndarray:
+    """24 x 24 global stiffness of an elastic MITC4 shell (shell4.c:
+    shell4_mindlin_stiffness, shell4.mojo:10-330) with nodes xyz (4 x 3)."""
+    lib = load()
+    lib.or_shell4_stiffness.restype = c_int
+    lib.or_shell4_stiffness.argtypes = [c_double] * 3 + [POINTER(c_double)] * 4
+    p = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(4, 3).T)
+    K = np.zeros(576)
+    rc = lib.or_shell4_stiffness(float(E), float(nu), float(h), _d_out(p[0]), _d_out(p[1]), _d_out(p[2]), _d_out(K))
+    if rc != 0:
+        raise ValueError("degenerate shell geometry")
+    return K.reshape(24, 24)
+
+
 def invert5(a) -> tuple:
     lib = load()
     lib.or_invert5.restype = c_int

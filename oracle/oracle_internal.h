@@ -49,6 +49,11 @@ int or_dbc3d_state(const or_elem3d *el, const double u_global[12], double k_out[
 int or_fbc3d_state(const or_elem3d *el, double *q_state, const double u_global[12], double k_out[144],
                    double f_out[12]);
 
+/* Elastic MITC4 shell stiffness (shell4.c: shell4.mojo:10-330), 24 x 24
+ * row major, DOFs ux uy uz rx ry rz per node */
+int or_shell4_stiffness(double E, double nu, double h, const double x[4], const double y[4], const double z[4],
+                        double K[576]);
+
 /* Transformation-handler collapses (mpc.c) */
 void or_mpc_collapse_matrix(int n, const double *K, const int *row_offsets, const int *dof_pool,
                             const double *coeff_pool, double *out);
