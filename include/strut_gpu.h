@@ -231,6 +231,20 @@ int sgpu_elem3d_commit(sgpu_ctx *ctx, int n, const int32_t *elem_ids);
 int sgpu_elem3d_basic_state(sgpu_ctx *ctx, int32_t elem_id, double *q);
 
 /* ------------------------------------------------------------------
+ * Elastic four-node shells (ShellMITC4 with an ElasticMembranePlateSection)
+ * ------------------------------------------------------------------ */
+
+/* Global stiffness of n elastic MITC4 shells: xyz[12 e + 3 i + c] is
+ * coordinate c of node i of shell e, props[3 e] = (E, nu, h); K[576 e] is
+ * the 24 x 24 matrix, row major, DOFs ux uy uz rx ry rz per node in the
+ * given node order.  Replaces shell4_mindlin_stiffness
+ * (elements/shell4.mojo:10-330), which _assemble_surface_soa_indices calls
+ * for every shell at every assembly
+ * (solver/assembly/stiffness_internal_surface.mojo:238-280).  A degenerate
+ * basis or a zero Jacobian is SGPU_ERR_ARG (the reference aborts). */
+int sgpu_shell4_stiffness(sgpu_ctx *ctx, int n, const double *xyz, const double *props, double *K);
+
+/* ------------------------------------------------------------------
  * 2D fiber beam-column elements outside any frame: the element calls of
  * the 2D frame assembly (stiffness_internal_frame2d.mojo:336-634) for a
  * host that scatters them itself.  The device routines are the 2D frame

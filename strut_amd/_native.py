@@ -88,6 +88,7 @@ EXPORTED = (
     "sgpu_elem3d_state_device",
     "sgpu_elem3d_commit",
     "sgpu_elem3d_basic_state",
+    "sgpu_shell4_stiffness",
     "sgpu_mpc_set",
     "sgpu_mpc_collapse_matrix",
     "sgpu_mpc_collapse_matrix_device",
