@@ -360,6 +360,14 @@ typedef struct {
                        reference (linear_solver_backend.mojo:1481-1506),
                        which drops the collapse's couplings outside it;
                        1: every coupling of the collapsed system */
+    /* elastic MITC4 shells (ElasticMembranePlateSection), assembled after
+     * the beam-columns (stiffness_internal_surface.mojo:238-280); their
+     * stiffness is formed once on the device (sgpu_shell4_stiffness's
+     * kernel) and collapsed once by the Transformation handler */
+    int n_shell;
+    const int32_t *shell_nodes;      /* 4 node indices per shell */
+    const double *shell_xyz;         /* 12 per shell: node i coordinate c at 3 i + c */
+    const double *shell_props;       /* 3 per shell: E, nu, h */
 } sgpu_frame3d_desc;
 
 /* Tables of the frame (after sgpu_sections_finalize; one frame per

@@ -552,6 +552,8 @@ class _Tr3dCase(ctypes.Structure):
         ("slave", POINTER(c_int)), ("has_mpc", c_int),
         ("n_elem", c_int), ("elem_nodes", POINTER(c_int)), ("elem", POINTER(c_void_p)),
         ("elem_kind", POINTER(c_int)), ("elem_q", POINTER(POINTER(c_double))),
+        ("n_shell", c_int), ("shell_nodes", POINTER(c_int)), ("shell_xyz", POINTER(c_double)),
+        ("shell_props", POINTER(c_double)),
         ("dt", c_double), ("steps", c_int), ("gamma", c_double), ("beta", c_double), ("tol", c_double),
         ("max_iters", c_int), ("alpha_m", c_double), ("beta_k", c_double), ("beta_k_init", c_double),
         ("n_patterns", c_int), ("exc_dir", c_int * TR3D_MAX_PATTERNS), ("ts_n", c_int * TR3D_MAX_PATTERNS),
@@ -585,16 +587,10 @@ def frame3d_elements(fc):
     return secs, elems
 
 
-def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: int | None = None,
-                    capture_band: bool = False) -> dict:
-    """Run a loaded 3D fiber-frame case (strut_amd.frame3d.Frame3dCase) on the
-    oracle.  hbw_rule 0 takes the band from the element free map as the
-    reference does (the collapse's couplings outside it are dropped); 1
-    widens it to every coupling the collapse makes."""
+def _tr3d_case(fc, steps: int, hbw_rule: int):
+    """The oracle's case record of a loaded 3D case (fresh element state);
+    returns the record and the objects it points into."""
     lib = load()
-    lib.or_tr3d_run.restype = c_int
-    lib.or_tr3d_run.argtypes = [POINTER(_Tr3dCase), POINTER(_Tr3dOut)]
-    steps = fc.steps if steps is None else steps
     secs, elems = frame3d_elements(fc)
     ptrs = (c_void_p * fc.n_elem)(*[ctypes.addressof(el.e) for el in elems])
     qptrs = (POINTER(c_double) * fc.n_elem)(*[_d_out(el.q) for el in elems])
@@ -605,6 +601,9 @@ def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: i
                 sl=np.ascontiguousarray(fc.slave, dtype=np.int32),
                 en=np.ascontiguousarray(fc.elem_nodes.reshape(-1), dtype=np.int32),
                 kind=np.ascontiguousarray(fc.elem_kind, dtype=np.int32),
+                sn=np.ascontiguousarray(fc.shell_nodes.reshape(-1), dtype=np.int32),
+                sx=np.ascontiguousarray(fc.coords[fc.shell_nodes].reshape(-1), dtype=np.float64),
+                spp=np.ascontiguousarray(fc.shell_props.reshape(-1), dtype=np.float64),
                 ts=[np.ascontiguousarray(pt.values, dtype=np.float64) for pt in fc.patterns])
     c = _Tr3dCase()
     c.n_nodes = fc.n_nodes
@@ -620,6 +619,11 @@ def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: i
     c.elem = ctypes.cast(ptrs, POINTER(c_void_p))
     c.elem_kind = keep["kind"].ctypes.data_as(POINTER(c_int))
     c.elem_q = ctypes.cast(qptrs, POINTER(POINTER(c_double)))
+    c.n_shell = len(fc.shell_nodes)
+    if c.n_shell:
+        c.shell_nodes = keep["sn"].ctypes.data_as(POINTER(c_int))
+        c.shell_xyz = _d_out(keep["sx"])
+        c.shell_props = _d_out(keep["spp"])
     c.dt, c.steps, c.gamma, c.beta, c.tol, c.max_iters = fc.dt, steps, fc.gamma, fc.beta, fc.tol, fc.max_iters
     c.alpha_m, c.beta_k, c.beta_k_init = fc.alpha_m, fc.beta_k, fc.beta_k_init
     c.n_patterns = len(fc.patterns)
@@ -630,6 +634,21 @@ def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: i
         c.ts_dt[k] = pt.dt
         c.ts_factor[k] = pt.factor
     c.hbw_rule = hbw_rule
+    return c, (keep, ptrs, qptrs, secs, elems)
+
+
+def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: int | None = None,
+                    capture_band: bool = False) -> dict:
+    """Run a loaded 3D fiber-frame case (strut_amd.frame3d.Frame3dCase) on the
+    oracle.  hbw_rule 0 takes the band from the element free map as the
+    reference does (the collapse's couplings outside it are dropped); 1
+    widens it to every coupling the collapse makes."""
+    lib = load()
+    lib.or_tr3d_run.restype = c_int
+    lib.or_tr3d_run.argtypes = [POINTER(_Tr3dCase), POINTER(_Tr3dOut)]
+    steps = fc.steps if steps is None else steps
+    c, held = _tr3d_case(fc, steps, hbw_rule)
+    elems, secs = held[4], held[3]
     o = _Tr3dOut()
     u_hist = np.zeros((steps, 6 * fc.n_nodes)) if record_history else None
     iters = np.zeros(steps, dtype=np.int32)
@@ -645,6 +664,27 @@ def run_transient3d(fc, record_history: bool = True, hbw_rule: int = 0, steps: i
     return {"status": status, "u": u_hist, "iters": iters, "steps_done": o.steps_done, "hbw": o.hbw,
             "free_count": o.free_count, "dense_fallbacks": o.dense_fallbacks, "band_first": band,
             "_elems": elems, "_sections": secs}
+
+
+# ------------------------------------------------------------------- modal
+def modal3d(fc, num_modes: int, max_rotations: int = 100, tol: float = 1.0e-12):
+    """run_modal_eigen (modal_eigen.mojo:143-362) on a loaded 3D case:
+    (eigenvalues, modes num_modes x 6 n_nodes, Jacobi rotations made).
+    max_rotations 100 and tol 1e-12 are the reference's
+    jacobi_symmetric_eigen defaults."""
+    lib = load()
+    lib.or_tr3d_modal.restype = c_int
+    lib.or_tr3d_modal.argtypes = [POINTER(_Tr3dCase), c_int, c_int, c_double, POINTER(c_double), POINTER(c_double),
+                                  POINTER(c_int)]
+    c, held = _tr3d_case(fc, 0, 0)
+    eig = np.zeros(num_modes)
+    modes = np.zeros((num_modes, fc.total_dofs))
+    rot = c_int(0)
+    rc = lib.or_tr3d_modal(ctypes.byref(c), num_modes, max_rotations, tol, _d_out(eig), _d_out(modes),
+                           ctypes.byref(rot))
+    if rc != 0:
+        raise ValueError(f"modal analysis failed ({rc})")
+    return eig, modes, rot.value
 
 
 # ---------------------------------------------------------------- recorders

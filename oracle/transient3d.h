@@ -23,6 +23,12 @@ typedef struct {
     void *const *elem;           /* or_elem3d* per element */
     const int *elem_kind;        /* 0 forceBeamColumn3d, 1 dispBeamColumn3d */
     double *const *elem_q;       /* force-based basic state per element */
+    /* elastic MITC4 shells (ElasticMembranePlateSection), assembled after
+     * the beam-columns as _assemble_surface_soa_indices does */
+    int n_shell;
+    const int *shell_nodes;      /* 4 per shell (node indices) */
+    const double *shell_xyz;     /* 12 per shell: node i coordinate c at 3 i + c */
+    const double *shell_props;   /* 3 per shell: E, nu, h */
     double dt;
     int steps;
     double gamma, beta, tol;
@@ -46,5 +52,10 @@ typedef struct {
 } or_tr3d_out;
 
 int or_tr3d_run(const or_tr3d_case *c, or_tr3d_out *out);
+/* run_modal_eigen (modal_eigen.mojo:143-362) on the same model: num_modes
+ * eigenvalues and mode shapes (num_modes x 6 n_nodes); the case's analysis
+ * fields are not read */
+int or_tr3d_modal(const or_tr3d_case *c, int num_modes, int max_rotations, double tol, double *eig, double *modes,
+                  int *rotations);
 
 #endif

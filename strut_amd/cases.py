@@ -609,3 +609,164 @@ def config1_case(steps: int = 4000) -> dict:
     dispBeamColumn3d beams (5 IPs, 268-fiber sections: 12 x 12 core, 12
     bars), rigid diaphragms, two records at 0.5 g and 0.35 g, dt 0.005 s."""
     return rc_frame3d_case(20, 5, 5, steps=steps, dt=0.005, seeds=(0, 1), pgas_g=(0.5, 0.35))
+
+
+# ------------------------------- 3D RC tower with an elastic shell core wall
+def rc_tower_case(stories: int, bays: int, steps: int, dt: float = 0.005, core: Tuple[int, int] = (2, 2),
+                  wall_h_div: int = 4, wall_v_div: int = 2, seeds: Tuple[int, int] = (0, 1),
+                  pgas_g: Tuple[float, float] = (0.5, 0.35), story_h: float = 3.0, bay_w: float = 6.0,
+                  floor_mass: float = 50.0, nip: int = 5, n_core: int = 8, n_bars: int = 8,
+                  wall_E: float = 30.0e6, wall_nu: float = 0.2, wall_t: float = 0.4, strong: float = 6.0,
+                  tol: float = 1.0e-8, max_iters: int = 25, name: str = "") -> dict:
+    """3D RC frame tower with an elastic shell core wall, in Strut's JSON
+    schema.
+
+    Frame: bays x bays plan, forceBeamColumn3d columns (400 x 400 mm, PDelta)
+    at every grid point and dispBeamColumn3d beams (300 x 500 mm) along both
+    grid directions, nip Lobatto points, n_core x n_core confined core and
+    n_bars Steel02 bars (8 and 8: 152 fibers per section).  Core wall: the
+    perimeter of the core[0] x core[1] central bays, 0.4 m thick elastic
+    ``shell`` elements (ElasticMembranePlateSection, E 30 GPa, nu 0.2, no
+    mass), each bay face split into wall_h_div panels along the plan and
+    every story into wall_v_div panels up the height.  Floors: fixed base;
+    per floor a retained node at the plan centre (uz, rx, ry fixed) tied by
+    rigidDiaphragm (perpDirn 3) to every node of the floor level (grid and
+    wall-edge nodes); the wall's nodes between floors keep six DOFs.  Nodes
+    are numbered level by level (the Plain numberer then keeps the band to
+    about one story's DOFs).  Lumped floor_mass on ux, uy, uz of every grid
+    node; two UniformExcitation records (x, y); Rayleigh 5 % on modes 1 and
+    3 from T1 = 0.1 s x stories and T3 = T1 / 5; Newmark 1/2-1/4, Newton,
+    NormUnbalance, BandGeneral, Transformation."""
+    from .constraints import rigid_diaphragm
+
+    i0 = (bays - core[0]) // 2
+    j0 = (bays - core[1]) // 2
+    i1, j1 = i0 + core[0], j0 + core[1]
+    # perimeter points of the core (plan coordinates as multiples of bay_w / wall_h_div), counter-clockwise
+    per = []
+    for i in range(i0 * wall_h_div, i1 * wall_h_div):
+        per.append((i, j0 * wall_h_div))
+    for j in range(j0 * wall_h_div, j1 * wall_h_div):
+        per.append((i1 * wall_h_div, j))
+    for i in range(i1 * wall_h_div, i0 * wall_h_div, -1):
+        per.append((i, j1 * wall_h_div))
+    for j in range(j1 * wall_h_div, j0 * wall_h_div, -1):
+        per.append((i0 * wall_h_div, j))
+    d = bay_w / wall_h_div
+    nodes, at, nid = [], {}, 1
+    masters = {}
+    levels = []  # (z index in units of story_h / wall_v_div, is floor)
+    for k in range(stories + 1):
+        levels.append((k * wall_v_div, True))
+        if k < stories:
+            levels.extend(((k * wall_v_div + m, False) for m in range(1, wall_v_div)))
+    for zi, floor in levels:
+        z = zi * story_h / wall_v_div
+        if floor:
+            for j in range(bays + 1):
+                for i in range(bays + 1):
+                    node = {"id": nid, "x": i * bay_w, "y": j * bay_w, "z": z}
+                    if zi == 0:
+                        node["constraints"] = [1, 2, 3, 4, 5, 6]
+                    nodes.append(node)
+                    at[(i * wall_h_div, j * wall_h_div, zi)] = nid
+                    nid += 1
+        for (pi, pj) in per:
+            if (pi, pj, zi) in at:
+                continue
+            node = {"id": nid, "x": pi * d, "y": pj * d, "z": z}
+            if zi == 0:
+                node["constraints"] = [1, 2, 3, 4, 5, 6]
+            nodes.append(node)
+            at[(pi, pj, zi)] = nid
+            nid += 1
+        if floor and zi > 0:
+            nodes.append({"id": nid, "x": bays * bay_w / 2.0, "y": bays * bay_w / 2.0, "z": z,
+                          "constraints": [3, 4, 5]})
+            masters[zi // wall_v_div] = nid
+            nid += 1
+    g = lambda i, j, k: at[(i * wall_h_div, j * wall_h_div, k * wall_v_div)]  # noqa: E731
+    elements, eid = [], 1
+    for k in range(stories):
+        for j in range(bays + 1):
+            for i in range(bays + 1):
+                elements.append({"id": eid, "type": "forceBeamColumn3d", "nodes": [g(i, j, k), g(i, j, k + 1)],
+                                 "section": 1, "geomTransf": "PDelta", "vecxz": [1.0, 0.0, 0.0],
+                                 "integration": "Lobatto", "num_int_pts": nip})
+                eid += 1
+    for k in range(1, stories + 1):
+        for j in range(bays + 1):
+            for i in range(bays):
+                elements.append({"id": eid, "type": "dispBeamColumn3d", "nodes": [g(i, j, k), g(i + 1, j, k)],
+                                 "section": 2, "geomTransf": "Linear", "vecxz": [0.0, 1.0, 0.0],
+                                 "integration": "Lobatto", "num_int_pts": nip})
+                eid += 1
+        for j in range(bays):
+            for i in range(bays + 1):
+                elements.append({"id": eid, "type": "dispBeamColumn3d", "nodes": [g(i, j, k), g(i, j + 1, k)],
+                                 "section": 2, "geomTransf": "Linear", "vecxz": [1.0, 0.0, 0.0],
+                                 "integration": "Lobatto", "num_int_pts": nip})
+                eid += 1
+    for zi in range(stories * wall_v_div):
+        for a in range(len(per)):
+            pa, pb = per[a], per[(a + 1) % len(per)]
+            elements.append({"id": eid, "type": "shell", "section": 3,
+                             "nodes": [at[(pa[0], pa[1], zi)], at[(pb[0], pb[1], zi)],
+                                       at[(pb[0], pb[1], zi + 1)], at[(pa[0], pa[1], zi + 1)]]})
+            eid += 1
+    masses = [{"node": g(i, j, k), "dof": dd, "value": floor_mass}
+              for k in range(1, stories + 1) for j in range(bays + 1) for i in range(bays + 1) for dd in (1, 2, 3)]
+    coords = {n["id"]: (n["x"], n["y"], n["z"]) for n in nodes}
+    mps = []
+    for k, m in masters.items():
+        floor = [nd["id"] for nd in nodes if nd["z"] == k * story_h and nd["id"] != m]
+        mps += rigid_diaphragm(coords, m, floor, 3)
+    sections = [rc_section(1, dim=3, b=0.4, h=0.4, n_core=n_core, n_bars=n_bars),
+                rc_section(2, dim=3, b=0.3, h=0.5, n_core=n_core, n_bars=n_bars),
+                {"id": 3, "type": "ElasticMembranePlateSection",
+                 "params": {"E": wall_E, "nu": wall_nu, "h": wall_t, "rho": 0.0}}]
+    t1 = 0.1 * stories
+    w1, w3 = 2.0 * math.pi / t1, 2.0 * math.pi / (t1 / 5.0)
+    zeta = 0.05
+    series, patterns = [], []
+    for tag, (seed, pga, direction) in enumerate(zip(seeds, pgas_g, (1, 2)), start=1):
+        acc = ground_motion(seed, dt, steps, pga, strong=strong)
+        series.append({"type": "Path", "tag": tag, "dt": dt, "values": [float(v) for v in acc], "factor": 1.0})
+        patterns.append({"type": "UniformExcitation", "tag": tag, "direction": direction, "accel": tag})
+    line = [g(0, 0, k) for k in range(stories + 1)]
+    recorders = [{"type": "node_displacement", "nodes": [masters[stories]], "dofs": [1, 2, 6], "output": "roof"}]
+    for k in range(1, stories + 1):
+        for dd in (1, 2):
+            recorders.append({"type": "drift", "i_node": line[k - 1], "j_node": line[k], "dof": dd, "perp_dirn": 3,
+                              "output": f"drift_{k}_{dd}"})
+    recorders.append({"type": "envelope_node_displacement", "nodes": [masters[k] for k in range(1, stories + 1)],
+                      "dofs": [1, 2, 6], "output": "floor_envelope"})
+    return {
+        "schema_version": "1.0",
+        "metadata": {"name": name or f"rc_tower_{stories}x{bays}x{bays}_core{core[0]}x{core[1]}", "units": "kN-m-t-s"},
+        "model": {"ndm": 3, "ndf": 6},
+        "nodes": nodes,
+        "materials": rc_materials(),
+        "sections": sections,
+        "elements": elements,
+        "masses": masses,
+        "mp_constraints": mps,
+        "time_series": series,
+        "patterns": patterns,
+        "rayleigh": {"alphaM": 2.0 * zeta * w1 * w3 / (w1 + w3), "betaKInit": 2.0 * zeta / (w1 + w3)},
+        "analysis": {"type": "transient_nonlinear", "steps": steps, "dt": dt, "tol": tol, "max_iters": max_iters,
+                     "system": "BandGeneral", "numberer": "Plain", "constraints": "Transformation",
+                     "algorithm": "Newton", "test_type": "NormUnbalance",
+                     "integrator": {"type": "Newmark", "gamma": 0.5, "beta": 0.25}},
+        "recorders": recorders,
+    }
+
+
+def config4_case(steps: int = 6000, seeds: Tuple[int, int] = (0, 1), pgas_g: Tuple[float, float] = (0.6, 0.42)) -> dict:
+    """The large single-GPU case: 60 stories x 6 x 6 bays (2940 forceBeamColumn3d
+    columns, 5040 dispBeamColumn3d beams, 5 IPs, 152-fiber sections: about
+    6.1 M fibers) around a 2 x 2-bay elastic shell core (each bay face in 4
+    panels, each story in 3: 5760 shells), rigid floors, two records,
+    dt 0.005 s."""
+    return rc_tower_case(60, 6, steps=steps, dt=0.005, core=(2, 2), wall_h_div=4, wall_v_div=3, seeds=seeds,
+                         pgas_g=pgas_g)

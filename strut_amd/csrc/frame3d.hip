@@ -46,8 +46,12 @@ int launch_band_piv(sgpu_ctx *ctx, int mode, int n, int hbw, double *d_band, con
                     int count);
 bool band_piv_reg_fits(int n, int hbw);
 size_t band_piv_work_doubles(int n, int hbw);
+// shell4.hip: elastic MITC4 shell stiffness, one wavefront per shell
+int launch_shell4_stiffness(sgpu_ctx *ctx, int n, const double *d_xyz, const double *d_props, double *d_K,
+                            int32_t *d_status);
 
 constexpr int kF3MaxRet = 12;       // free retained DOFs one element reaches
+constexpr int kF3ShellMaxRet = 24;  // free retained DOFs one shell reaches
 constexpr int kF3SolveThreads = 1024;
 
 struct Frame3dHost {
@@ -78,6 +82,12 @@ struct Frame3dHost {
     int32_t *d_m_off = nullptr, *d_m_col = nullptr;        // reduced mass, CSR
     double *d_m_val = nullptr;
     double *d_load = nullptr;                              // n_patterns x n: T^T (-M iota_p)
+    // elastic shells: stiffness formed and collapsed once; per record the
+    // element-level products k_hat u and k_hat v of every iteration
+    int n_sh = 0;
+    int32_t *d_sh_ret = nullptr, *d_sh_ret_n = nullptr;    // n_sh x 24 free retained DOFs
+    double *d_sh_khat = nullptr;                           // n_sh x 24 x 24 collapsed stiffness
+    double *d_sh_f = nullptr, *d_sh_y = nullptr;           // R x n_sh x 24
     // device state
     double *d_u = nullptr;  // total
     double *d_uf, *d_vf, *d_af, *d_un, *d_vn, *d_an, *d_vt, *d_at, *d_R, *d_du;
@@ -210,10 +220,13 @@ __global__ __launch_bounds__(256) void f3_elem_collapse_kernel(const int32_t *re
 // every band slot: K from its k_hat entries, then (:2302-2335, :2400-2411)
 // band = K; band += a1 C with C = alphaM M + betaK K + betaKInit K_init;
 // band += a0 M.  raw != 0 writes K alone (the initial tangent).
+// A source index q >= 0 is a beam-column's k_hat entry of the record, q < 0
+// the shells' collapsed stiffness entry -q - 1 (the same for every record);
+// beam-columns come first, then shells, each in element order.
 __global__ void f3_band_kernel(const int32_t *recs, int64_t slots, int64_t khat_stride, const int32_t *off,
-                               const int32_t *src, const double *khat, const double *mass_band,
-                               const double *kinit_band, double *band, int raw, double a0, double a1, double alpha_m,
-                               double beta_k, double beta_k_init, int has_damping) {
+                               const int32_t *src, const double *khat, const double *sh_khat,
+                               const double *mass_band, const double *kinit_band, double *band, int raw, double a0,
+                               double a1, double alpha_m, double beta_k, double beta_k_init, int has_damping) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= slots) return;
     const int r = recs[blockIdx.y];
@@ -221,7 +234,10 @@ __global__ void f3_band_kernel(const int32_t *recs, int64_t slots, int64_t khat_
     kinit_band += (size_t)r * slots;
     band += (size_t)r * slots;
     double kv = 0.0;
-    for (int t = off[s]; t < off[s + 1]; ++t) kv += khat[src[t]];
+    for (int t = off[s]; t < off[s + 1]; ++t) {
+        const int q = src[t];
+        kv += q >= 0 ? khat[q] : sh_khat[-(int64_t)q - 1];
+    }
     if (raw) {
         band[s] = kv;
         return;
@@ -240,11 +256,15 @@ __global__ void f3_band_kernel(const int32_t *recs, int64_t slots, int64_t khat_
 }
 
 // :2337-2356 R = P - F_int - C v - M a on the free DOFs
-__global__ void f3_residual_kernel(const int32_t *recs, int n, int n_el, int n_patterns, const double *load,
+// A source index s >= 0 is a beam-column's (element, retained position)
+// entry, s < 0 the shells' entry -s - 1: their k_hat u and k_hat v, the
+// elastic shell's tangent and initial tangent being the same matrix.
+__global__ void f3_residual_kernel(const int32_t *recs, int n, int n_el, int n_sh, int n_patterns, const double *load,
                                    const double *ag_all, const int32_t *dof_off, const int32_t *dof_src,
-                                   const double *fhat, const double *yk, const double *yi, const int32_t *m_off,
-                                   const int32_t *m_col, const double *m_val, const double *vt, const double *at,
-                                   double alpha_m, double beta_k, double beta_k_init, int has_damping, double *R) {
+                                   const double *fhat, const double *yk, const double *yi, const double *sh_f,
+                                   const double *sh_y, const int32_t *m_off, const int32_t *m_col,
+                                   const double *m_val, const double *vt, const double *at, double alpha_m,
+                                   double beta_k, double beta_k_init, int has_damping, double *R) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int r = recs[blockIdx.y];
@@ -252,6 +272,8 @@ __global__ void f3_residual_kernel(const int32_t *recs, int n, int n_el, int n_p
     fhat += (size_t)r * n_el * 12;
     yk += (size_t)r * n_el * 12;
     yi += (size_t)r * n_el * 12;
+    sh_f += (size_t)r * n_sh * 24;
+    sh_y += (size_t)r * n_sh * 24;
     vt += (size_t)r * n;
     at += (size_t)r * n;
     R += (size_t)r * n;
@@ -260,9 +282,16 @@ __global__ void f3_residual_kernel(const int32_t *recs, int n, int n_el, int n_p
     double fi = 0.0, ky = 0.0, kiy = 0.0;
     for (int t = dof_off[i]; t < dof_off[i + 1]; ++t) {
         const int s = dof_src[t];
-        fi += fhat[s];
-        ky += yk[s];
-        kiy += yi[s];
+        if (s >= 0) {
+            fi += fhat[s];
+            ky += yk[s];
+            kiy += yi[s];
+        } else {
+            const double y = sh_y[-(int64_t)s - 1];
+            fi += sh_f[-(int64_t)s - 1];
+            ky += y;
+            kiy += y;
+        }
     }
     double mv = 0.0, ma = 0.0;
     for (int t = m_off[i]; t < m_off[i + 1]; ++t) {
@@ -272,6 +301,52 @@ __global__ void f3_residual_kernel(const int32_t *recs, int n, int n_el, int n_p
     double damping = 0.0;
     if (has_damping) damping = alpha_m * mv + beta_k * ky + beta_k_init * kiy;
     R[i] = p - fi - 0.0 - damping - ma;
+}
+
+// one 576-thread workgroup per shell, once: k_hat = T^T k T (24 x 24; the
+// positions past the shell's retained count stay zero)
+__global__ __launch_bounds__(576) void f3_shell_collapse_kernel(const double *T, const double *k, double *khat) {
+    __shared__ double sT[576], sK[576], sKT[576];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    sT[tid] = T[(size_t)e * 576 + tid];
+    sK[tid] = k[(size_t)e * 576 + tid];
+    __syncthreads();
+    const int a = tid / 24, bp = tid % 24;
+    double s = 0.0;
+    for (int b = 0; b < 24; ++b) s += sK[a * 24 + b] * sT[b * 24 + bp];
+    sKT[tid] = s;
+    __syncthreads();
+    const int ap = tid / 24;
+    s = 0.0;
+    for (int a2 = 0; a2 < 24; ++a2) s += sT[a2 * 24 + ap] * sKT[a2 * 24 + bp];
+    khat[(size_t)e * 576 + tid] = s;
+}
+
+// every listed record's shells: k_hat u and k_hat v over the retained free
+// DOFs (u = T u_ret exactly, the slaves following their retained DOFs), one
+// 64-lane workgroup per (shell, record), one row per lane
+__global__ __launch_bounds__(64) void f3_shell_force_kernel(const int32_t *recs, int n, int n_sh, const int32_t *ret,
+                                                            const int32_t *ret_n, const double *khat,
+                                                            const double *uf, const double *vt, double *sh_f,
+                                                            double *sh_y) {
+    __shared__ double su[24], sv[24];
+    const int e = blockIdx.x, r = recs[blockIdx.y], lane = threadIdx.x;
+    const int m = ret_n[e];
+    if (lane < 24) {
+        su[lane] = lane < m ? uf[(size_t)r * n + ret[e * 24 + lane]] : 0.0;
+        sv[lane] = lane < m ? vt[(size_t)r * n + ret[e * 24 + lane]] : 0.0;
+    }
+    __syncthreads();
+    if (lane >= 24) return;
+    const double *row = khat + (size_t)e * 576 + lane * 24;
+    double f = 0.0, y = 0.0;
+    for (int b = 0; b < m; ++b) {
+        f += row[b] * su[b];
+        y += row[b] * sv[b];
+    }
+    const size_t o = ((size_t)r * n_sh + e) * 24 + lane;
+    sh_f[o] = f;
+    sh_y[o] = y;
 }
 
 // one workgroup: sum of R^2 and the largest element status
@@ -1014,6 +1089,16 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
         for (int a = 0; a < 2; ++a)
             if (d->elem_nodes[2 * e + a] < 0 || d->elem_nodes[2 * e + a] >= nn)
                 return fail(ctx, SGPU_ERR_ARG, "element %d: node index out of range", e);
+    const int n_sh = d->n_shell;
+    if (n_sh < 0 || (n_sh > 0 && (!d->shell_nodes || !d->shell_xyz || !d->shell_props)))
+        return fail(ctx, SGPU_ERR_ARG, "bad shell description");
+    for (int e = 0; e < n_sh; ++e) {
+        for (int a = 0; a < 4; ++a)
+            if (d->shell_nodes[4 * e + a] < 0 || d->shell_nodes[4 * e + a] >= nn)
+                return fail(ctx, SGPU_ERR_ARG, "shell %d: node index out of range", e);
+        const double nu = d->shell_props[3 * e + 1];
+        if (!(nu > -1.0 && nu < 1.0)) return fail(ctx, SGPU_ERR_ARG, "shell %d: Poisson's ratio %g", e, nu);
+    }
     {  // record 0's element ids, checked before anything is built (claimed below)
         Frame3dHost probe;
         int rc0 = claim_elements(ctx, probe, d->elem_ids, n_el);
@@ -1071,6 +1156,42 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
                 for (size_t b = a + 1; b < R.size(); ++b) hbw = std::max(hbw, std::abs(R[a] - R[b]));
         }
     }
+    // per shell: free retained DOFs and T_s (24 x 24), the band from its
+    // 24 DOFs as for the beam-columns
+    std::vector<int32_t> sh_ret((size_t)n_sh * 24, 0), sh_ret_n(n_sh, 0);
+    std::vector<double> sh_T((size_t)n_sh * 576, 0.0);
+    for (int e = 0; e < n_sh; ++e) {
+        std::vector<int32_t> R;
+        int fmap[24];
+        for (int a = 0; a < 24; ++a) {
+            const int g = 6 * d->shell_nodes[4 * e + a / 6] + a % 6;
+            fmap[a] = free_idx[g];
+            const bool expand = d->has_mpc && d->slave[g];
+            const int t0 = expand ? ro[g] : 0, t1 = expand ? ro[g + 1] : (free_idx[g] >= 0 ? 1 : 0);
+            for (int t = t0; t < t1; ++t) {
+                const int r = expand ? free_idx[dp[t]] : free_idx[g];
+                const double c = expand ? cp[t] : 1.0;
+                if (r < 0) continue;
+                int pos = (int)(std::find(R.begin(), R.end(), r) - R.begin());
+                if (pos == (int)R.size()) {
+                    if ((int)R.size() == kF3ShellMaxRet)
+                        return fail(ctx, SGPU_ERR_ARG, "shell %d reaches more than %d free DOFs", e, kF3ShellMaxRet);
+                    R.push_back(r);
+                }
+                sh_T[(size_t)e * 576 + a * 24 + pos] += c;
+            }
+        }
+        sh_ret_n[e] = (int32_t)R.size();
+        for (size_t q = 0; q < R.size(); ++q) sh_ret[(size_t)e * 24 + q] = R[q];
+        if (d->hbw_rule == 0) {
+            for (int a = 0; a < 24; ++a)
+                for (int b = a + 1; b < 24; ++b)
+                    if (fmap[a] >= 0 && fmap[b] >= 0) hbw = std::max(hbw, std::abs(fmap[a] - fmap[b]));
+        } else {
+            for (size_t a = 0; a < R.size(); ++a)
+                for (size_t b = a + 1; b < R.size(); ++b) hbw = std::max(hbw, std::abs(R[a] - R[b]));
+        }
+    }
     // helpers.mojo:3304-3342: reduced mass, in the reference's addition order
     std::vector<std::vector<std::pair<int, double>>> mrow(n);
     auto madd = [&](int r, int c, double v) {
@@ -1106,9 +1227,6 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
     const int W = 2 * hbw + 1;
     const int64_t slots = (int64_t)n * W;
     if (slots > (int64_t)1 << 31) return fail(ctx, SGPU_ERR_ARG, "band of %d x %d is too large", n, W);
-    if ((size_t)(n + 2 * hbw) * sizeof(double) > 56 * 1024)
-        return fail(ctx, SGPU_ERR_ARG, "%d free DOFs: the solve keeps the right-hand side in LDS (at most %zu)", n,
-                    56 * 1024 / sizeof(double) - 2 * hbw);
     std::vector<int32_t> moff(n + 1, 0), mcol;
     std::vector<double> mval, mass_band(slots, 0.0);
     for (int r = 0; r < n; ++r) {
@@ -1129,6 +1247,12 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
                 const int r = ret[(size_t)e * 12 + a], s = ret[(size_t)e * 12 + b];
                 if (std::abs(r - s) <= hbw) cnt[(size_t)r * W + (s - r + hbw) + 1]++;
             }
+    for (int e = 0; e < n_sh; ++e)
+        for (int a = 0; a < sh_ret_n[e]; ++a)
+            for (int b = 0; b < sh_ret_n[e]; ++b) {
+                const int r = sh_ret[(size_t)e * 24 + a], s = sh_ret[(size_t)e * 24 + b];
+                if (std::abs(r - s) <= hbw) cnt[(size_t)r * W + (s - r + hbw) + 1]++;
+            }
     for (int64_t s = 0; s < slots; ++s) cnt[s + 1] += cnt[s];
     std::vector<int32_t> band_src(cnt[slots]), fill(cnt.begin(), cnt.end() - 1);
     for (int e = 0; e < n_el; ++e)
@@ -1137,12 +1261,21 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
                 const int r = ret[(size_t)e * 12 + a], s = ret[(size_t)e * 12 + b];
                 if (std::abs(r - s) <= hbw) band_src[fill[(size_t)r * W + (s - r + hbw)]++] = e * 144 + a * 12 + b;
             }
+    for (int e = 0; e < n_sh; ++e)  // shells after the beam-columns, negative sources
+        for (int a = 0; a < sh_ret_n[e]; ++a)
+            for (int b = 0; b < sh_ret_n[e]; ++b) {
+                const int r = sh_ret[(size_t)e * 24 + a], s = sh_ret[(size_t)e * 24 + b];
+                if (std::abs(r - s) <= hbw)
+                    band_src[fill[(size_t)r * W + (s - r + hbw)]++] = -(e * 576 + a * 24 + b) - 1;
+            }
     // per free DOF gather lists
     std::vector<int32_t> doff(n + 1, 0), dsrc;
     {
         std::vector<std::vector<int32_t>> lists(n);
         for (int e = 0; e < n_el; ++e)
             for (int a = 0; a < ret_n[e]; ++a) lists[ret[(size_t)e * 12 + a]].push_back(e * 12 + a);
+        for (int e = 0; e < n_sh; ++e)
+            for (int a = 0; a < sh_ret_n[e]; ++a) lists[sh_ret[(size_t)e * 24 + a]].push_back(-(e * 24 + a) - 1);
         for (int r = 0; r < n; ++r) {
             doff[r] = (int32_t)dsrc.size();
             dsrc.insert(dsrc.end(), lists[r].begin(), lists[r].end());
@@ -1187,6 +1320,7 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
     h->beta_k_init = d->beta_k_init;
     h->max_iters = d->max_iters;
     h->n_patterns = d->n_patterns;
+    h->n_sh = n_sh;
     std::vector<int32_t> slave(total), rov(ro, ro + total + 1), dpv(dp, dp + ro[total]), en(d->elem_nodes,
                                                                                            d->elem_nodes + 2 * n_el);
     std::vector<double> cpv(cp, cp + ro[total]);
@@ -1202,6 +1336,26 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
         (rc = upload(ctx, &h->d_m_off, moff)) || (rc = upload(ctx, &h->d_m_col, mcol)) ||
         (rc = upload(ctx, &h->d_m_val, mval)) || (rc = upload(ctx, &h->d_load, load)))
         return rc;
+    if (n_sh > 0) {  // shell stiffness on the device, once, and its collapse
+        std::vector<double> xyz(d->shell_xyz, d->shell_xyz + 12 * (size_t)n_sh),
+            props(d->shell_props, d->shell_props + 3 * (size_t)n_sh);
+        double *d_xyz = nullptr, *d_props = nullptr, *d_T = nullptr, *d_k = nullptr;
+        int32_t *d_st = nullptr;
+        if ((rc = upload(ctx, &h->d_sh_ret, sh_ret)) || (rc = upload(ctx, &h->d_sh_ret_n, sh_ret_n)) ||
+            (rc = upload(ctx, &d_xyz, xyz)) || (rc = upload(ctx, &d_props, props)) || (rc = upload(ctx, &d_T, sh_T)) ||
+            (rc = dalloc(ctx, &d_k, (size_t)n_sh * 576)) || (rc = dalloc(ctx, &h->d_sh_khat, (size_t)n_sh * 576)) ||
+            (rc = dalloc(ctx, &d_st, (size_t)n_sh)))
+            return rc;
+        if ((rc = launch_shell4_stiffness(ctx, n_sh, d_xyz, d_props, d_k, d_st))) return rc;
+        std::vector<int32_t> st(n_sh);
+        HIP_OK(ctx, hipMemcpyAsync(st.data(), d_st, sizeof(int32_t) * n_sh, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+        for (int e = 0; e < n_sh; ++e)
+            if (st[e])
+                return fail(ctx, SGPU_ERR_ARG, "shell %d: %s", e, st[e] == 1 ? "degenerate basis" : "singular Jacobian");
+        hipLaunchKernelGGL(f3_shell_collapse_kernel, dim3(n_sh), dim3(576), 0, ctx->stream, d_T, d_k, h->d_sh_khat);
+        HIP_OK(ctx, hipGetLastError());
+    }
     int32_t rec0;
     if ((rc = sgpu_frame3d_add_record(ctx, d->elem_ids, d->ts_values, d->ts_n, d->ts_dt, d->ts_factor, &rec0)))
         return rc;
@@ -1256,6 +1410,8 @@ int f3_allocate(sgpu_ctx *ctx, Frame3dHost &h) {
         (rc = dalloc(ctx, &h.d_ag, 3 * R)))
         return rc;
     if (!band_piv_reg_fits(h.n, h.hbw) && (rc = dalloc(ctx, &h.d_bwork, R * band_piv_work_doubles(h.n, h.hbw))))
+        return rc;
+    if (h.n_sh > 0 && ((rc = dalloc(ctx, &h.d_sh_f, R * h.n_sh * 24)) || (rc = dalloc(ctx, &h.d_sh_y, R * h.n_sh * 24))))
         return rc;
     for (double **p : vecs) HIP_OK(ctx, hipMemset(*p, 0, sizeof(double) * R * n));
     HIP_OK(ctx, hipMemset(h.d_u, 0, sizeof(double) * R * h.total));
@@ -1318,8 +1474,8 @@ int f3_initialize(sgpu_ctx *ctx, Frame3dHost &h) {
                                hipMemcpyDeviceToDevice, st));
     const int64_t slots = (int64_t)h.n * h.W;
     hipLaunchKernelGGL(f3_band_kernel, grid2(slots, h.R), dim3(256), 0, st, h.d_list, slots, (int64_t)h.n_el * 144,
-                       h.d_band_off, h.d_band_src, h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_kinit_band, 1, 0.0,
-                       0.0, 0.0, 0.0, 0.0, 0);
+                       h.d_band_off, h.d_band_src, h.d_khat, h.d_sh_khat, h.d_mass_band, h.d_kinit_band,
+                       h.d_kinit_band, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0);
     HIP_OK(ctx, hipGetLastError());
     if ((rc = sgpu_sections_revert_all(ctx))) return rc;
     h.initialized = true;
@@ -1367,10 +1523,13 @@ int f3_run(sgpu_ctx *ctx, Frame3dHost &h, int steps, double *u_hist, int32_t *it
             hipLaunchKernelGGL(f3_trial_kernel, grid2(n, ni), dim3(256), 0, st, h.d_list, n, h.d_uf, h.d_un, h.d_vn,
                                h.d_an, h.d_at, h.d_vt, a0, a2, a3, gamma, dt);
             if ((rc = f3_elements(ctx, h, ni))) return rc;
-            hipLaunchKernelGGL(f3_residual_kernel, grid2(n, ni), dim3(256), 0, st, h.d_list, n, h.n_el, h.n_patterns,
-                               h.d_load, h.d_ag, h.d_dof_off, h.d_dof_src, h.d_fhat, h.d_yk, h.d_yi, h.d_m_off,
-                               h.d_m_col, h.d_m_val, h.d_vt, h.d_at, h.alpha_m, h.beta_k, h.beta_k_init, has_damping,
-                               h.d_R);
+            if (h.n_sh > 0)
+                hipLaunchKernelGGL(f3_shell_force_kernel, dim3(h.n_sh, ni), dim3(64), 0, st, h.d_list, n, h.n_sh,
+                                   h.d_sh_ret, h.d_sh_ret_n, h.d_sh_khat, h.d_uf, h.d_vt, h.d_sh_f, h.d_sh_y);
+            hipLaunchKernelGGL(f3_residual_kernel, grid2(n, ni), dim3(256), 0, st, h.d_list, n, h.n_el, h.n_sh,
+                               h.n_patterns, h.d_load, h.d_ag, h.d_dof_off, h.d_dof_src, h.d_fhat, h.d_yk, h.d_yi,
+                               h.d_sh_f, h.d_sh_y, h.d_m_off, h.d_m_col, h.d_m_val, h.d_vt, h.d_at, h.alpha_m,
+                               h.beta_k, h.beta_k_init, has_damping, h.d_R);
             hipLaunchKernelGGL(f3_norm_kernel, dim3(ni), dim3(1024), 0, st, h.d_list, n, h.d_R, h.n_el, h.d_status,
                                h.d_scalar);
             HIP_OK(ctx, hipGetLastError());
@@ -1392,8 +1551,9 @@ int f3_run(sgpu_ctx *ctx, Frame3dHost &h, int steps, double *u_hist, int32_t *it
             if ((rc = f3_set_list(ctx, h, iter))) return rc;
             const int nu = (int)iter.size();
             hipLaunchKernelGGL(f3_band_kernel, grid2(slots, nu), dim3(256), 0, st, h.d_list, slots,
-                               (int64_t)h.n_el * 144, h.d_band_off, h.d_band_src, h.d_khat, h.d_mass_band,
-                               h.d_kinit_band, h.d_band, 0, a0, a1, h.alpha_m, h.beta_k, h.beta_k_init, has_damping);
+                               (int64_t)h.n_el * 144, h.d_band_off, h.d_band_src, h.d_khat, h.d_sh_khat,
+                               h.d_mass_band, h.d_kinit_band, h.d_band, 0, a0, a1, h.alpha_m, h.beta_k,
+                               h.beta_k_init, has_damping);
             {
                 ProfScope ps(ctx, SGPU_KERNEL_F3_SOLVE);
                 if ((rc = launch_band_solve(ctx, h.solver == 0 ? 0 : 7, n, h.hbw, h.d_band, h.d_R, h.d_du,
@@ -1492,6 +1652,10 @@ int sgpu_frame3d_run_records(sgpu_ctx *ctx, int steps, int32_t *iters, int32_t *
 int sgpu_frame3d_set_band_solver(sgpu_ctx *ctx, int solver) {
     if (!ctx || (solver != 0 && solver != 1)) return SGPU_ERR_ARG;
     if (!ctx->frame3d) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
+    const Frame3dHost &h = *ctx->frame3d;
+    if (solver == 1 && (size_t)(h.n + 2 * h.hbw) * sizeof(double) > 56 * 1024)
+        return fail(ctx, SGPU_ERR_ARG, "%d free DOFs: the elimination without swaps keeps the right-hand side in LDS "
+                    "(at most %zu)", h.n, 56 * 1024 / sizeof(double) - 2 * h.hbw);
     ctx->frame3d->solver = solver;
     return SGPU_OK;
 }
@@ -1516,8 +1680,8 @@ int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band_out) {
                        h.d_at, h.d_vt, a0, a2, a3, gamma, dt);
     if ((rc = f3_elements(ctx, h, 1))) return rc;
     hipLaunchKernelGGL(f3_band_kernel, grid2(slots, 1), dim3(256), 0, st, h.d_list, slots, (int64_t)h.n_el * 144,
-                       h.d_band_off, h.d_band_src, h.d_khat, h.d_mass_band, h.d_kinit_band, h.d_band, 0, a0, a1,
-                       h.alpha_m, h.beta_k, h.beta_k_init, has_damping);
+                       h.d_band_off, h.d_band_src, h.d_khat, h.d_sh_khat, h.d_mass_band, h.d_kinit_band, h.d_band, 0,
+                       a0, a1, h.alpha_m, h.beta_k, h.beta_k_init, has_damping);
     HIP_OK(ctx, hipGetLastError());
     HIP_OK(ctx, hipMemcpyAsync(band_out, h.d_band, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
     HIP_OK(ctx, hipStreamSynchronize(st));

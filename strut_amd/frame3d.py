@@ -5,8 +5,10 @@ numbering (6 DOFs per node), single-point constraints, lumped masses, the
 Transformation handler's rigidDiaphragm / equalDOF expansion tables
 (loader.mojo:2230-2372), forceBeamColumn3d / dispBeamColumn3d elements on
 FiberSection3d sections (geomTransf Linear or PDelta, optional vecxz, elastic
-torsion G J from the section's params) -- and the analysis / pattern /
-Rayleigh records of input_types.mojo:1821-2290.
+torsion G J from the section's params), elastic four-node ``shell`` elements
+on ElasticMembranePlateSection sections (E, nu, h; rho lumps rho h area / 4
+on ux, uy, uz of each node, loader.mojo:2060-2100) -- and the analysis /
+pattern / Rayleigh records of input_types.mojo:1821-2290.
 
 Excitation: one ``pattern`` (Strut's schema) or a ``patterns`` list of
 UniformExcitation records whose inertia loads are summed in order (the
@@ -76,6 +78,8 @@ class Frame3dCase:
     record_dofs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     envelope_dofs: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=np.int32))
     drift: List[tuple] = field(default_factory=list)  # (i_dof, j_dof, dx) per drift recorder
+    shell_nodes: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), dtype=np.int32))
+    shell_props: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))  # E, nu, h per shell
 
     @property
     def total_dofs(self) -> int:
@@ -102,7 +106,8 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
         mass[NDF * idx[int(m["node"])] + int(m["dof"]) - 1] += float(m["value"])
     mats = MaterialTable(case["materials"])
     sections = {int(s["id"]): s for s in case["sections"]}
-    elems = case["elements"]
+    shells = [e for e in case["elements"] if e["type"] == "shell"]
+    elems = [e for e in case["elements"] if e["type"] != "shell"]
     used: List[int] = []
     for e in elems:
         if e["type"] not in ELEMENT_TYPES_3D:
@@ -114,6 +119,30 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
             raise ValueError(f"{e['type']} needs a FiberSection3d on the 3D fiber-frame path")
         if sid not in used:
             used.append(sid)
+    shell_nodes = np.zeros((len(shells), 4), dtype=np.int32)
+    shell_props = np.zeros((len(shells), 3))
+    for k, e in enumerate(shells):
+        sec = sections[int(e["section"])]
+        if sec["type"] != "ElasticMembranePlateSection":  # layered shells are outside the path
+            raise ValueError(f"shell {e.get('id')} needs an ElasticMembranePlateSection on the 3D frame path")
+        if len(e["nodes"]) != 4:
+            raise ValueError("shell requires 4 nodes")
+        pr = sec["params"]
+        shell_nodes[k] = [idx[int(v)] for v in e["nodes"]]
+        shell_props[k] = [float(pr.get("E", 0.0)), float(pr.get("nu", 0.0)), float(pr.get("h", 0.0))]
+        rho_area = float(pr.get("rho", 0.0)) * float(pr.get("h", 0.0))
+        if rho_area != 0.0:  # loader.mojo:2060-2100
+            p = coords[shell_nodes[k]]
+            a = (p[1] - p[0]) + (p[2] - p[3])
+            b = (p[3] - p[0]) + (p[2] - p[1])
+            cr = (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+            area = 0.25 * float(np.sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]))
+            if area <= 0.0:
+                raise ValueError("shell area must be > 0")
+            lumped = rho_area * area / 4.0
+            for d in range(3):
+                for nd in shell_nodes[k]:
+                    mass[NDF * nd + d] += lumped
     cells = {sid: discretize_section(sections[sid], mats) for sid in used}
     gj = {sid: (float(sections[sid]["params"].get("G", 0.0)), float(sections[sid]["params"].get("J", 0.0)))
           for sid in used}
@@ -194,7 +223,8 @@ def load_frame3d_case(case: dict, steps: int | None = None) -> Frame3dCase:
         gamma=float(integ.get("gamma", 0.5)), beta=float(integ.get("beta", 0.25)), tol=float(an.get("tol", 1.0e-10)),
         max_iters=int(an.get("max_iters", 20)), alpha_m=float(ray.get("alphaM", 0.0)),
         beta_k=float(ray.get("betaK", 0.0)), beta_k_init=float(ray.get("betaKInit", 0.0)), patterns=patterns,
-        record_dofs=np.array(rec, dtype=np.int32), envelope_dofs=np.array(env, dtype=np.int32), drift=drift)
+        record_dofs=np.array(rec, dtype=np.int32), envelope_dofs=np.array(env, dtype=np.int32), drift=drift,
+        shell_nodes=shell_nodes, shell_props=shell_props)
 
 
 def recorder_plan(fc: Frame3dCase):

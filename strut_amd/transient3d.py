@@ -36,6 +36,8 @@ class Frame3dDesc(ctypes.Structure):
         ("beta_k_init", c_double), ("n_patterns", c_int), ("exc_dir", c_int * MAX_PATTERNS),
         ("ts_n", c_int * MAX_PATTERNS), ("ts_values", POINTER(c_double) * MAX_PATTERNS),
         ("ts_dt", c_double * MAX_PATTERNS), ("ts_factor", c_double * MAX_PATTERNS), ("hbw_rule", c_int),
+        ("n_shell", c_int), ("shell_nodes", POINTER(c_int32)), ("shell_xyz", POINTER(c_double)),
+        ("shell_props", POINTER(c_double)),
     ]
 
 
@@ -117,7 +119,10 @@ class Frame3dRun:
                  cp=np.ascontiguousarray(fc.coeff_pool if len(fc.coeff_pool) else np.zeros(1), dtype=np.float64),
                  sl=np.ascontiguousarray(fc.slave, dtype=np.int32), ids=np.array(ids, dtype=np.int32),
                  en=np.ascontiguousarray(fc.elem_nodes.reshape(-1), dtype=np.int32),
-                 ts=[np.ascontiguousarray(p.values, dtype=np.float64) for p in fc.patterns])
+                 ts=[np.ascontiguousarray(p.values, dtype=np.float64) for p in fc.patterns],
+                 sn=np.ascontiguousarray(fc.shell_nodes.reshape(-1), dtype=np.int32),
+                 sx=np.ascontiguousarray(fc.coords[fc.shell_nodes].reshape(-1), dtype=np.float64),
+                 spp=np.ascontiguousarray(fc.shell_props.reshape(-1), dtype=np.float64))
         self._keep = k
         d = Frame3dDesc()
         d.n_nodes = fc.n_nodes
@@ -133,6 +138,9 @@ class Frame3dRun:
             d.exc_dir[q], d.ts_n[q], d.ts_values[q] = p.direction, len(k["ts"][q]), dptr(k["ts"][q])
             d.ts_dt[q], d.ts_factor[q] = p.dt, p.factor
         d.hbw_rule = hbw_rule
+        d.n_shell = len(fc.shell_nodes)
+        if d.n_shell:
+            d.shell_nodes, d.shell_xyz, d.shell_props = iptr(k["sn"]), dptr(k["sx"]), dptr(k["spp"])
         hbw, nf = c_int32(), c_int32()
         self.rt._check(self.rt.lib.sgpu_frame3d_create(self.rt.ctx, ctypes.byref(d), ctypes.byref(hbw),
                                                        ctypes.byref(nf)))

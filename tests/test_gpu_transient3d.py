@@ -288,3 +288,54 @@ def test_gpu_records_that_stop_early_leave_the_others_unchanged():
             scale = np.abs(ref["u"][:k]).max()
             assert np.abs(hists[r] - ref["u"][:k]).max() <= 1e-8 * scale
     assert stopped == 2 and got["status"][1] < 0 and got["status"][3] > 0
+
+
+def _tower(steps, **kw):
+    args = dict(dt=0.005, core=(1, 1), wall_h_div=2, wall_v_div=2, n_core=4, n_bars=4, pgas_g=(1.2, 0.84),
+                wall_E=3.0e6)
+    args.update(kw)
+    return cases.rc_tower_case(2, 2, steps=steps, **args)
+
+
+def test_gpu_tower_effective_band_matches_oracle():
+    """A tower with an elastic shell core wall: beam-column and shell
+    stiffness, collapse and band of the first Newton solve."""
+    fc = load_frame3d_case(_tower(2))
+    ref = oracle.run_transient3d(fc, hbw_rule=0, steps=1, capture_band=True)
+    dev = Frame3dRun(fc, hbw_rule=0)
+    got = dev.effective_band()
+    dev.close()
+    want = ref["band_first"]
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("hbw_rule", [0, 1])
+def test_gpu_tower_with_a_shell_wall_matches_oracle(hbw_rule):
+    """RC frame around a shell core wall (wall nodes on the floors slaved to
+    the diaphragms, wall nodes between floors free), two records, well into
+    the frame's nonlinear range."""
+    ref = _compare(load_frame3d_case(_tower(100)), hbw_rule, 100)
+    assert ref["status"] == 0 and np.abs(ref["u"]).max() > 5e-3
+
+
+def test_gpu_tower_records_run_together_match_each_oracle_run():
+    from strut_amd.frame3d import Pattern
+
+    steps = 50
+    fc = load_frame3d_case(_tower(steps))
+    fcs, recs = [], []
+    for seeds, pgas in (((5, 6), (0.9, 1.3)), ((7, 8), (0.4, 0.5))):
+        f = load_frame3d_case(_tower(steps, seeds=seeds, pgas_g=pgas))
+        fcs.append(f)
+        recs.append([Pattern(p.direction, p.values, p.dt, p.factor) for p in f.patterns])
+    run = Frame3dRun(fc, records=recs)
+    run.set_recorder(np.arange(fc.total_dofs, dtype=np.int32), steps)
+    got = run.run_records(steps)
+    for r, f in enumerate([fc] + fcs):
+        ref = oracle.run_transient3d(f)
+        k = ref["steps_done"]
+        assert got["status"][r] == ref["status"] and got["steps_done"][r] == k
+        np.testing.assert_array_equal(got["iters"][r][:k], ref["iters"][:k])
+        scale = np.abs(ref["u"][:k]).max()
+        assert np.abs(run.recorded(r) - ref["u"][:k]).max() <= 1e-8 * scale
+    run.close()

@@ -141,11 +141,47 @@ def test_elastic_diaphragm_frame_is_the_collapsed_linear_newmark_solution():
     the expansion matrix of the diaphragm tables (K from the oracle's element
     stiffness at rest, which test_oracle_elem3d pins against the textbook
     frame stiffness)."""
-    from strut_amd.constraints import expansion_matrix
-
     fc = load_frame3d_case(_elastic(_small(steps=100)))
     r = oracle.run_transient3d(fc, hbw_rule=1)
     assert r["status"] == 0
+    want = _linear_newmark(fc)
+    scale = np.abs(want).max()
+    assert scale > 1e-3
+    assert np.abs(r["u"] - want).max() <= 1e-7 * scale
+
+
+def _tower(**kw):
+    args = dict(steps=80, dt=0.005, core=(1, 1), wall_h_div=2, wall_v_div=2, n_core=4, n_bars=4)
+    args.update(kw)
+    return cases.rc_tower_case(2, 2, **args)
+
+
+def test_elastic_tower_with_a_shell_wall_is_the_collapsed_linear_newmark_solution():
+    """The same with elastic shells in the wall (their stiffness from
+    oracle.shell4_stiffness, pinned by test_oracle_shell4), some of their
+    nodes slaved to the floors and some free between floors."""
+    fc = load_frame3d_case(_elastic(_tower()))
+    assert len(fc.shell_nodes) == 32
+    r = oracle.run_transient3d(fc, hbw_rule=1)
+    assert r["status"] == 0
+    want = _linear_newmark(fc)
+    scale = np.abs(want).max()
+    assert scale > 1e-4
+    assert np.abs(r["u"] - want).max() <= 1e-7 * scale
+
+
+def test_the_shell_wall_stiffens_the_tower():
+    r_wall = oracle.run_transient3d(load_frame3d_case(_elastic(_tower())))
+    r_soft = oracle.run_transient3d(load_frame3d_case(_elastic(_tower(wall_E=30.0))))  # a wall a million times softer
+    assert r_wall["status"] == 0 and r_soft["status"] == 0
+    assert np.abs(r_wall["u"]).max() < 0.5 * np.abs(r_soft["u"]).max()
+
+
+def _linear_newmark(fc):
+    """numpy's linear Newmark on T^T K T (beam-columns at rest and shells),
+    T^T M T, Rayleigh (alphaM, betaKInit) and T^T P."""
+    from strut_amd.constraints import expansion_matrix
+
     _, elems = oracle.frame3d_elements(fc)
     nt = fc.total_dofs
     K = np.zeros((nt, nt))
@@ -153,6 +189,9 @@ def test_elastic_diaphragm_frame_is_the_collapsed_linear_newmark_solution():
         k, _ = el.state(np.zeros(12))
         dofs = list(range(6 * i, 6 * i + 6)) + list(range(6 * j, 6 * j + 6))
         K[np.ix_(dofs, dofs)] += k
+    for nodes, props in zip(fc.shell_nodes, fc.shell_props):
+        dofs = [6 * nd + k for nd in nodes for k in range(6)]
+        K[np.ix_(dofs, dofs)] += oracle.shell4_stiffness(*props, fc.coords[nodes])
     free = np.flatnonzero(~fc.constrained)
     T = expansion_matrix(fc.row_offsets, fc.dof_pool, fc.coeff_pool)[:, free]
     Kr, Mr = T.T @ K @ T, T.T @ np.diag(fc.mass) @ T
@@ -177,6 +216,4 @@ def test_elastic_diaphragm_frame_is_the_collapsed_linear_newmark_solution():
         v = v + dt * ((1 - g) * a + g * an)
         u, a = un, an
         want[step] = T @ u
-    scale = np.abs(want).max()
-    assert scale > 1e-3
-    assert np.abs(r["u"] - want).max() <= 1e-7 * scale
+    return want
