@@ -421,6 +421,21 @@ int sgpu_frame3d_recorded(sgpu_ctx *ctx, double *hist, int32_t *steps);
 int sgpu_frame3d_envelopes(sgpu_ctx *ctx, int n_drift, const int32_t *col_i, const int32_t *col_j, const double *dx,
                            double *disp_env, double *drift_env);
 
+/* Modal analysis of the frame at rest: run_modal_eigen
+ * (solver/run_case/analysis/modal_eigen.mojo:143-362) with
+ * jacobi_symmetric_eigen (solver/eigen.mojo:15-79).  K_ff from the
+ * beam-columns' initial tangents and the shells' stiffness, collapsed; M_ff
+ * the reduced mass with the reference's floors; A = L^-1 K L^-T (M = L L^T);
+ * classical Jacobi, one rotation per pass on the first largest off-diagonal
+ * magnitude in row-major order, at most max_rotations passes (<= 0: the
+ * reference's 100), stopping at or below tol (the reference's 1e-12).
+ * eigenvalues[num_modes] ascending (the reference's selection sort),
+ * modes[num_modes x 6 n_nodes] = L^-T v expanded to every DOF
+ * (_enforce_mpc_values), rotations = passes that rotated.  Needs 24 n^2
+ * bytes of device memory for the call (n free DOFs). */
+int sgpu_frame3d_modal(sgpu_ctx *ctx, int num_modes, int max_rotations, double tol, double *eigenvalues,
+                       double *modes, int32_t *rotations);
+
 /* Another record (ground-motion instance) of the same frame, before the
  * first run and before the recorder: its own elements (sgpu_elem3d_add, so
  * their own section states; element i of this list plays element i of the

@@ -89,6 +89,7 @@ EXPORTED = (
     "sgpu_elem3d_commit",
     "sgpu_elem3d_basic_state",
     "sgpu_shell4_stiffness",
+    "sgpu_frame3d_modal",
     "sgpu_mpc_set",
     "sgpu_mpc_collapse_matrix",
     "sgpu_mpc_collapse_matrix_device",

@@ -37,6 +37,7 @@
 
 #include "../../include/strut_gpu.h"
 #include "ctx.h"
+#include "frame3d_host.h"
 
 namespace sgpu {
 
@@ -50,59 +51,6 @@ size_t band_piv_work_doubles(int n, int hbw);
 int launch_shell4_stiffness(sgpu_ctx *ctx, int n, const double *d_xyz, const double *d_props, double *d_K,
                             int32_t *d_status);
 
-constexpr int kF3MaxRet = 12;       // free retained DOFs one element reaches
-constexpr int kF3ShellMaxRet = 24;  // free retained DOFs one shell reaches
-constexpr int kF3SolveThreads = 1024;
-
-struct Frame3dHost {
-    int n_nodes = 0, total = 0, n = 0, hbw = 0, W = 0, n_el = 0;
-    double dt = 0, gamma = 0, beta = 0, tol = 0, alpha_m = 0, beta_k = 0, beta_k_init = 0;
-    int max_iters = 0, n_patterns = 0;
-    // records: ground-motion instances of the frame, each with its own
-    // elements (section states) and series; per-record device arrays hold
-    // R slices
-    int R = 0;
-    std::vector<int32_t> rec_elem_ids;                 // R x n_el
-    std::vector<std::vector<double>> rec_ts;           // R x n_patterns series
-    std::vector<double> rec_ts_dt, rec_ts_factor;      // R x n_patterns
-    std::vector<int32_t> rec_steps, rec_status;        // completed steps; 0 running, > 0 stopped, < 0 failed
-    std::vector<uint8_t> elem_used;                    // per sgpu_elem3d element: claimed by a record
-    std::vector<int32_t> list_on_device;               // the record list the kernels read now
-    bool allocated = false, initialized = false;
-    int32_t *d_list = nullptr, *d_list_ids = nullptr, *d_row_of = nullptr;
-    double *d_ag = nullptr;  // R x 3 ground accelerations of the step
-    // device tables
-    int32_t *d_free_idx = nullptr, *d_row_offsets = nullptr, *d_dof_pool = nullptr, *d_slave = nullptr;
-    double *d_coeff_pool = nullptr;
-    int32_t *d_elem_nodes = nullptr, *d_ret = nullptr, *d_ret_n = nullptr;
-    double *d_T = nullptr;                                 // n_el x 12 x 12: T_e[a][a']
-    int32_t *d_band_off = nullptr, *d_band_src = nullptr;  // per band slot: k_hat entries
-    double *d_mass_band = nullptr, *d_kinit_band = nullptr;
-    int32_t *d_dof_off = nullptr, *d_dof_src = nullptr;    // per free DOF: (e, a') entries
-    int32_t *d_m_off = nullptr, *d_m_col = nullptr;        // reduced mass, CSR
-    double *d_m_val = nullptr;
-    double *d_load = nullptr;                              // n_patterns x n: T^T (-M iota_p)
-    // elastic shells: stiffness formed and collapsed once; per record the
-    // element-level products k_hat u and k_hat v of every iteration
-    int n_sh = 0;
-    int32_t *d_sh_ret = nullptr, *d_sh_ret_n = nullptr;    // n_sh x 24 free retained DOFs
-    double *d_sh_khat = nullptr;                           // n_sh x 24 x 24 collapsed stiffness
-    double *d_sh_f = nullptr, *d_sh_y = nullptr;           // R x n_sh x 24
-    // device state
-    double *d_u = nullptr;  // total
-    double *d_uf, *d_vf, *d_af, *d_un, *d_vn, *d_an, *d_vt, *d_at, *d_R, *d_du;
-    double *d_u12 = nullptr, *d_k = nullptr, *d_f = nullptr, *d_khat = nullptr, *d_kinit_hat = nullptr,
-           *d_fhat = nullptr, *d_yk = nullptr, *d_yi = nullptr;
-    int32_t *d_status = nullptr;
-    double *d_band = nullptr;
-    double *d_bwork = nullptr;  // R x band_piv_work_doubles when the register solve cannot hold the band
-    int solver = 0;             // 0: the reference's row swaps (band LU / dense fallback); 1: no row swaps
-    double *d_scalar = nullptr;  // [0] residual norm^2, [1] max element status, [2] solve status
-    // recorder: chosen DOFs after every converged step, kept on the device
-    int n_rec = 0, rec_cap = 0;
-    int32_t *d_rec_dofs = nullptr;
-    double *d_rec = nullptr;
-};
 
 // ------------------------------------------------------------------ kernels
 
@@ -1321,6 +1269,17 @@ int sgpu_frame3d_create(sgpu_ctx *ctx, const sgpu_frame3d_desc *d, int32_t *hbw_
     h->max_iters = d->max_iters;
     h->n_patterns = d->n_patterns;
     h->n_sh = n_sh;
+    h->h_free_list = free_list;
+    h->h_constrained.assign(total, 0);
+    for (int i = 0; i < total; ++i) h->h_constrained[i] = free_idx[i] < 0 ? 1 : 0;
+    h->h_ro.assign(ro, ro + total + 1);
+    h->h_dp.assign(dp, dp + ro[total]);
+    h->h_cp.assign(cp, cp + ro[total]);
+    h->h_slave.assign(total, 0);
+    for (int i = 0; i < total; ++i) h->h_slave[i] = d->has_mpc && d->slave[i] ? 1 : 0;
+    h->h_moff = moff;
+    h->h_mcol = mcol;
+    h->h_mval = mval;
     std::vector<int32_t> slave(total), rov(ro, ro + total + 1), dpv(dp, dp + ro[total]), en(d->elem_nodes,
                                                                                            d->elem_nodes + 2 * n_el);
     std::vector<double> cpv(cp, cp + ro[total]);
@@ -1612,6 +1571,10 @@ int f3_run(sgpu_ctx *ctx, Frame3dHost &h, int steps, double *u_hist, int32_t *it
 }
 
 }  // namespace
+
+namespace sgpu {
+int frame3d_initialize(sgpu_ctx *ctx, Frame3dHost &h) { return f3_initialize(ctx, h); }
+}  // namespace sgpu
 
 int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, int32_t *steps_done,
                      int32_t *status, int32_t *dense_fallbacks) {

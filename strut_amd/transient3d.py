@@ -74,6 +74,9 @@ def _declare(lib):
                                                   POINTER(c_double), POINTER(c_double), POINTER(c_double)]
     lib.sgpu_frame3d_set_band_solver.restype = c_int
     lib.sgpu_frame3d_set_band_solver.argtypes = [c_void_p, c_int]
+    lib.sgpu_frame3d_modal.restype = c_int
+    lib.sgpu_frame3d_modal.argtypes = [c_void_p, c_int, c_int, c_double, POINTER(c_double), POINTER(c_double),
+                                       POINTER(c_int32)]
     lib._f3_declared = True
 
 
@@ -171,6 +174,18 @@ class Frame3dRun:
         self.steps_done += done.value
         return {"status": status.value, "u": u, "iters": iters, "steps_done": done.value, "hbw": self.hbw,
                 "free_count": self.free_count, "dense_fallbacks": fallbacks.value}
+
+    def modal(self, num_modes: int, max_rotations: int = 100, tol: float = 1.0e-12):
+        """run_modal_eigen (modal_eigen.mojo:143-362) on the device: the
+        frame at rest, (eigenvalues ascending, modes num_modes x 6 n_nodes,
+        Jacobi rotations made).  max_rotations 100 and tol 1e-12 are the
+        reference's jacobi_symmetric_eigen defaults."""
+        eig = np.zeros(num_modes)
+        modes = np.zeros((num_modes, self.fc.total_dofs))
+        rot = c_int32(0)
+        self.rt._check(self.rt.lib.sgpu_frame3d_modal(self.rt.ctx, num_modes, max_rotations, tol, dptr(eig),
+                                                      dptr(modes), ctypes.byref(rot)))
+        return eig, modes, rot.value
 
     def effective_band(self) -> np.ndarray:
         """K + a1 C + a0 M in the band at the current displacement."""
