@@ -395,8 +395,10 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
  * 2 hbw wide); *info: 0 solved without a swap; 1 solved with swaps (where
  * the reference took its dense fallback); 2 a column's largest magnitude
  * at or below 1e-14 (the reference aborts), nothing solved.  0 picks the
- * register kernel (half bandwidth <= 127, n <= 3072) or else the
- * global-memory one, 5 the register kernel, 6 the global-memory one.
+ * register kernel (half bandwidth <= 127, n <= 3072), else the blocked
+ * kernels (half bandwidth <= 1264: sixteen-column panels, trailing updates
+ * over many workgroups), else the global-memory one; 5 the register kernel,
+ * 6 the global-memory one, 8 the blocked kernels.
  * Variants 1-4 and 7 eliminate without row swaps (*info 1: the reference
  * would have swapped; the solution is the unpivoted one): 1 global memory,
  * 2 register window, 3 blocked, 4 blocked with 8-row blocks on 512
@@ -464,10 +466,13 @@ int sgpu_frame3d_envelopes_record(sgpu_ctx *ctx, int32_t record, int n_drift, co
  * state; before the first step: at rest). */
 int sgpu_frame3d_effective_band(sgpu_ctx *ctx, double *band);
 
-/* The band solver of the 3D driver: 0 (default) the reference's row swaps;
+/* The band solver of the 3D driver: 0 (default) the reference's row swaps
+ * (the register kernel when the band fits it, else the blocked kernels);
  * 1 the band elimination without swaps (an opt-in for comparison; it counts
  * where the reference would have swapped, and its solution differs from the
- * reference's there by rounding, or more on an indefinite tangent). */
+ * reference's there by rounding, or more on an indefinite tangent); 2 the
+ * reference's row swaps by the blocked kernels at any band width they
+ * hold. */
 int sgpu_frame3d_set_band_solver(sgpu_ctx *ctx, int solver);
 
 /* ------------------------------------------------------------------
@@ -482,6 +487,7 @@ int sgpu_frame3d_set_band_solver(sgpu_ctx *ctx, int solver);
  * ------------------------------------------------------------------ */
 typedef struct sgpu_band sgpu_band;
 int sgpu_band_create(sgpu_ctx *ctx, int n, int hbw, sgpu_band **out);
+/* Destroy a band before its context (the band uses the context's stream). */
 int sgpu_band_destroy(sgpu_band *b);
 /* layout 0: n x (2 hbw + 1) band values; 1: a dense n x n row-major matrix
  * whose entries outside the band are dropped (:413-426).  add 0: the band

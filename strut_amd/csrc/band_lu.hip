@@ -589,6 +589,12 @@ __global__ void band_axpy_kernel(int64_t count, double *y, const double *src, do
     y[i] = add ? y[i] + scale * src[i] : (scale == 1.0 ? src[i] : scale * src[i]);
 }
 
+// band_blk.hip: the blocked factorization and solve for wide bands
+bool band_blk_fits(int n, int hbw);
+size_t band_blk_work_doubles(int n, int hbw);
+int launch_band_blk(sgpu_ctx *ctx, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
+                    double *d_status, double *d_L, int32_t *d_ipiv, double *d_work, const int32_t *recs, int count);
+
 // LDS the register kernel needs beyond its static arrays (the forward values).
 bool band_piv_reg_fits(int n, int hbw) { return hbw + 1 <= kBpMaxP && (size_t)n * sizeof(double) <= 24 * 1024; }
 
@@ -597,12 +603,15 @@ size_t band_piv_work_doubles(int n, int hbw) { return (size_t)n * (3 * (size_t)h
 // Pivoted factor (+ solve when rhs is set) of `count` systems on the stream.
 // mode 0: the register kernel when the band fits it, else the global one
 // (which needs `work`, band_piv_work_doubles per system); 1: the register
-// kernel; 2: the global kernel.
+// kernel; 2: the global kernel; 3: the blocked kernels (`work` of
+// band_blk_work_doubles per system; mode 0 takes them when the register
+// kernel cannot hold the band).
 int launch_band_piv(sgpu_ctx *ctx, int mode, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
                     double *d_status, double *d_L, int32_t *d_ipiv, double *d_work, const int32_t *recs,
                     int count) {
     const bool fits = band_piv_reg_fits(n, hbw);
-    if (mode == 0) mode = fits ? 1 : 2;
+    if (mode == 0) mode = fits ? 1 : band_blk_fits(n, hbw) ? 3 : 2;
+    if (mode == 3) return launch_band_blk(ctx, n, hbw, d_band, d_rhs, d_x, d_status, d_L, d_ipiv, d_work, recs, count);
     if (mode == 1 && !fits)
         return fail(ctx, SGPU_ERR_ARG, "band %d x %d exceeds the register solve (half bandwidth <= %d, n <= 3072)", n,
                     2 * hbw + 1, kBpMaxP - 1);
@@ -658,7 +667,7 @@ int sgpu_band_create(sgpu_ctx *ctx, int n, int hbw, sgpu_band **out) {
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_vec, sizeof(double) * 2 * (size_t)n);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_status, sizeof(double) * 4);
     if (e == hipSuccess && !band_piv_reg_fits(n, hbw))
-        e = hipMalloc((void **)&b->d_work, sizeof(double) * band_piv_work_doubles(n, hbw));
+        e = hipMalloc((void **)&b->d_work, sizeof(double) * band_blk_work_doubles(n, hbw));
     if (e != hipSuccess) {
         sgpu_band_destroy(b);
         return fail(ctx, SGPU_ERR_HIP, "band of %d x %d: %s", n, 2 * hbw + 1, hipGetErrorString(e));

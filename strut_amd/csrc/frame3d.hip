@@ -47,6 +47,7 @@ int launch_band_piv(sgpu_ctx *ctx, int mode, int n, int hbw, double *d_band, con
                     int count);
 bool band_piv_reg_fits(int n, int hbw);
 size_t band_piv_work_doubles(int n, int hbw);
+size_t band_blk_work_doubles(int n, int hbw);
 // shell4.hip: elastic MITC4 shell stiffness, one wavefront per shell
 int launch_shell4_stiffness(sgpu_ctx *ctx, int n, const double *d_xyz, const double *d_props, double *d_K,
                             int32_t *d_status);
@@ -943,9 +944,9 @@ int claim_elements(sgpu_ctx *ctx, Frame3dHost &h, const int32_t *ids, int n_el) 
 
 int launch_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, double *d_band, const double *d_rhs, double *d_x,
                       double *d_status, const int32_t *recs = nullptr, int count = 1, double *d_work = nullptr) {
-    if (variant == 0 || variant == 5 || variant == 6)  // the reference's row swaps
-        return launch_band_piv(ctx, variant == 0 ? 0 : variant - 4, n, hbw, d_band, d_rhs, d_x, d_status, nullptr,
-                               nullptr, d_work, recs, count);
+    if (variant == 0 || variant == 5 || variant == 6 || variant == 8)  // the reference's row swaps
+        return launch_band_piv(ctx, variant == 0 ? 0 : variant == 8 ? 3 : variant - 4, n, hbw, d_band, d_rhs, d_x,
+                               d_status, nullptr, nullptr, d_work, recs, count);
     if (variant == 7) variant = 0;  // no row swaps, kernel picked by size
     const size_t blk_lds = sizeof(double) * ((size_t)kF3BlkRing + n);
     const int P = hbw + 1, Bc = (P + 3) / 4;
@@ -989,12 +990,12 @@ using namespace sgpu;
 
 int sgpu_band_solve(sgpu_ctx *ctx, int variant, int n, int hbw, const double *band, const double *rhs, double *x,
                     int32_t *info) {
-    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 7)
+    if (!ctx || !band || !rhs || !x || !info || n <= 0 || hbw < 0 || hbw > n - 1 || variant < 0 || variant > 8)
         return SGPU_ERR_ARG;
     HIP_OK(ctx, hipSetDevice(ctx->device));
     const size_t slots = (size_t)n * (2 * hbw + 1);
-    const bool global_piv = variant == 6 || (variant == 0 && !band_piv_reg_fits(n, hbw));
-    const size_t work = global_piv ? band_piv_work_doubles(n, hbw) : 0;
+    const bool global_piv = variant == 6 || variant == 8 || (variant == 0 && !band_piv_reg_fits(n, hbw));
+    const size_t work = global_piv ? band_blk_work_doubles(n, hbw) : 0;
     double *d = nullptr;
     HIP_OK(ctx, hipMalloc((void **)&d, sizeof(double) * (slots + 2 * (size_t)n + 4 + work)));
     double *d_rhs = d + slots, *d_x = d_rhs + n, *d_st = d_x + n, *d_work = global_piv ? d_st + 4 : nullptr;
@@ -1368,8 +1369,7 @@ int f3_allocate(sgpu_ctx *ctx, Frame3dHost &h) {
         (rc = dalloc(ctx, &h.d_list_ids, R * n_el)) || (rc = dalloc(ctx, &h.d_row_of, R)) ||
         (rc = dalloc(ctx, &h.d_ag, 3 * R)))
         return rc;
-    if (!band_piv_reg_fits(h.n, h.hbw) && (rc = dalloc(ctx, &h.d_bwork, R * band_piv_work_doubles(h.n, h.hbw))))
-        return rc;
+    if ((rc = dalloc(ctx, &h.d_bwork, R * band_blk_work_doubles(h.n, h.hbw)))) return rc;
     if (h.n_sh > 0 && ((rc = dalloc(ctx, &h.d_sh_f, R * h.n_sh * 24)) || (rc = dalloc(ctx, &h.d_sh_y, R * h.n_sh * 24))))
         return rc;
     for (double **p : vecs) HIP_OK(ctx, hipMemset(*p, 0, sizeof(double) * R * n));
@@ -1515,7 +1515,8 @@ int f3_run(sgpu_ctx *ctx, Frame3dHost &h, int steps, double *u_hist, int32_t *it
                                h.beta_k_init, has_damping);
             {
                 ProfScope ps(ctx, SGPU_KERNEL_F3_SOLVE);
-                if ((rc = launch_band_solve(ctx, h.solver == 0 ? 0 : 7, n, h.hbw, h.d_band, h.d_R, h.d_du,
+                if ((rc = launch_band_solve(ctx, h.solver == 0 ? 0 : h.solver == 1 ? 7 : 8, n, h.hbw, h.d_band, h.d_R,
+                                            h.d_du,
                                             h.d_scalar + 2, h.d_list, nu, h.d_bwork)))
                     return rc;
             }
@@ -1613,7 +1614,7 @@ int sgpu_frame3d_run_records(sgpu_ctx *ctx, int steps, int32_t *iters, int32_t *
 }
 
 int sgpu_frame3d_set_band_solver(sgpu_ctx *ctx, int solver) {
-    if (!ctx || (solver != 0 && solver != 1)) return SGPU_ERR_ARG;
+    if (!ctx || solver < 0 || solver > 2) return SGPU_ERR_ARG;
     if (!ctx->frame3d) return fail(ctx, SGPU_ERR_STATE, "sgpu_frame3d_create has not run");
     const Frame3dHost &h = *ctx->frame3d;
     if (solver == 1 && (size_t)(h.n + 2 * h.hbw) * sizeof(double) > 56 * 1024)

@@ -101,7 +101,8 @@ class BandSystem:
 
     def close(self) -> None:
         if self.h:
-            self.rt.lib.sgpu_band_destroy(self.h)
+            if self.rt.ctx:  # a band never outlives its context (its stream and device are the context's)
+                self.rt.lib.sgpu_band_destroy(self.h)
             self.h = c_void_p()
 
     def __del__(self):

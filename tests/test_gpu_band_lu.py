@@ -3,8 +3,10 @@ against the oracle's restatement of the reference backend
 (linear_solver_backend.mojo:631-679 band LU, :652-657 and :175-214 dense
 partial-pivot fallback, :735-769 and :216-232 solves).
 
-The device's pivoted band LU makes the reference's pivot choices and
-divisions, so its factors are the reference's; its back substitution sums
+The device's pivoted band LU (the register kernel, the global-memory one
+and the blocked one: sixteen-column panels and trailing updates over many
+workgroups) makes the reference's pivot choices and divisions in the
+reference's order, so its factors are the reference's; its back substitution sums
 by columns where the reference sums by rows, so solutions agree to
 rounding: within 1e-12 of the largest solution entry on these bands (the
 measured figure is printed).  The bands are indefinite, nonsymmetric, with
@@ -68,7 +70,7 @@ def _general_band(n, hbw, seed, kind):
 
 
 CASES = [(1, 0), (5, 1), (9, 3), (40, 39), (130, 64), (300, 110), (2300, 110), (700, 119), (520, 127), (200, 150),
-         (60, 59)]
+         (60, 59), (17, 2), (900, 300), (1200, 560)]
 
 
 @pytest.mark.parametrize("n,hbw", CASES)
@@ -84,7 +86,7 @@ def test_pivoted_band_solve_matches_the_reference_backend(rt, n, hbw, kind):
     ok, want, fallback = oracle.band_solve(band, hbw, rhs)
     assert ok
     fits = hbw <= 127 and n <= 3072
-    for variant in ((0, 5, 6) if fits else (0, 6)):
+    for variant in ((0, 5, 6, 8) if fits else (0, 6, 8)):
         x, info = band_solve(rt, band, rhs, variant=variant)
         err = np.abs(x - want).max() / np.abs(want).max()
         print(f"n {n} hbw {hbw} {kind} variant {variant}: info {info} reference fallback {fallback} err {err:.2e}",
@@ -119,7 +121,7 @@ def test_singular_band_is_reported(rt):
     rhs = np.ones(n)
     ok, _, _ = oracle.band_solve(band, hbw, rhs)
     assert not ok
-    for variant in (0, 5, 6):
+    for variant in (0, 5, 6, 8):
         _, info = band_solve(rt, band, rhs, variant=variant)
         assert info == 2
 
@@ -128,7 +130,9 @@ def test_singular_band_is_reported(rt):
 def test_band_system_factor_once_solve_many(rt, n, hbw):
     """create / load (dense, out-of-band entries dropped) / add scaled /
     factor / several solves, against the oracle on the in-band matrix; the
-    split solve gives the one-shot solve's bits."""
+    split solve gives the one-shot solve's bits where both take the register
+    kernel (wider bands: the blocked factorization, whose one-shot back
+    substitution sums by rows where the split solve sums by columns)."""
     rng = np.random.default_rng(n)
     K = _general_band(n, hbw, seed=n, kind="indefinite")
     M = np.diag(1.0 + rng.random(n))
@@ -147,7 +151,10 @@ def test_band_system_factor_once_solve_many(rt, n, hbw):
         x = sysm.solve(rhs)
         assert np.abs(x - want).max() <= TOL * np.abs(want).max()
         one_shot, _ = band_solve(rt, eff, rhs, variant=0)
-        assert np.array_equal(x, one_shot)
+        if hbw <= 127:
+            assert np.array_equal(x, one_shot)
+        else:
+            assert np.abs(x - one_shot).max() <= TOL * np.abs(want).max()
     sysm.close()
 
 
@@ -168,12 +175,14 @@ def test_band_system_refuses_out_of_order_calls(rt):
         BandSystem(rt, 10, 10)
 
 
-@pytest.mark.parametrize("n,hbw", [(5, 1), (9, 3), (40, 39), (130, 64), (300, 110), (2300, 110), (200, 150)])
+@pytest.mark.parametrize("n,hbw", [(5, 1), (9, 3), (40, 39), (130, 64), (300, 110), (2300, 110), (200, 150),
+                                   (700, 300), (640, 200)])
 @pytest.mark.parametrize("kind", ["indefinite", "nonsym", "zero_diag", "tiny_pivot"])
 def test_factors_are_the_reference_factors(rt, n, hbw, kind):
     """The U rows of the device factorization equal, bit for bit, those of
     the oracle's reference backend: its band LU's where it needs no swap,
-    its dense partial-pivot LU's rows where it falls back."""
+    its dense partial-pivot LU's rows where it falls back.  Bands wider than
+    the register kernel holds go through the blocked kernels."""
     A = _general_band(n, hbw, seed=11 * n + hbw, kind=kind)
     band = _to_band(A, hbw)
     ok, want, fallback = oracle.band_factor_rows(band, hbw)

@@ -68,6 +68,15 @@ def test_gpu_diaphragm_frame_bidirectional_matches_oracle(hbw_rule):
     assert ref["status"] == 0 and np.abs(ref["u"]).max() > 1e-2  # well into the nonlinear range
 
 
+@pytest.mark.parametrize("hbw_rule", [0, 1])
+def test_gpu_blocked_band_solver_matches_oracle(hbw_rule):
+    """The blocked row-swap kernels (sixteen-column panels, trailing updates
+    over many workgroups) in the 3D driver: the same bar as the register
+    kernel."""
+    case = cases.rc_frame3d_case(2, 2, 1, steps=60, dt=0.005, n_core=4, n_bars=4, pgas_g=(0.5, 0.35))
+    _compare(load_frame3d_case(case), hbw_rule, 60, band_solver=2)
+
+
 def test_gpu_frame_without_constraints_matches_oracle():
     case = cases.rc_frame3d_case(2, 1, 1, steps=60, dt=0.005, n_core=4, n_bars=4, diaphragms=False,
                                  bidirectional=False, pgas_g=(0.6, 0.0))
