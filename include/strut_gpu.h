@@ -396,7 +396,7 @@ int sgpu_frame3d_run(sgpu_ctx *ctx, int steps, double *u_hist, int32_t *iters, i
  * the reference took its dense fallback); 2 a column's largest magnitude
  * at or below 1e-14 (the reference aborts), nothing solved.  0 picks the
  * register kernel (half bandwidth <= 127, n <= 3072), else the blocked
- * kernels (half bandwidth <= 1264: sixteen-column panels, trailing updates
+ * kernels (half bandwidth <= 1008: sixteen-column panels, trailing updates
  * over many workgroups), else the global-memory one; 5 the register kernel,
  * 6 the global-memory one, 8 the blocked kernels.
  * Variants 1-4 and 7 eliminate without row swaps (*info 1: the reference

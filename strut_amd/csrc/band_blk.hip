@@ -9,24 +9,31 @@
 // product and per difference, steps in ascending k.  Here the steps are
 // taken sixteen columns at a time (a panel) and every entry still receives
 // the same operations in the same order:
-//   * panel: one workgroup per system, one wavefront per panel column, the
-//     panel's rows (k0 .. k0 + 15 + hbw) in registers; per column, the
-//     owning wavefront searches, swaps, divides and publishes the
-//     multipliers in LDS (one barrier), every other wavefront swaps the two
-//     rows in its column and the later ones update theirs.  The multipliers
-//     are stored where they were formed (the reference's band LU keeps them
-//     so, and its solves swap the right-hand side step by step); the
-//     trailing work takes them as the rows stand after the panel's last
-//     swap (rows moved by a later swap carry their multipliers along), which
-//     gives each row the same products in the same order;
-//   * the panel's swaps applied in order to the trailing columns (k0 + 16 ..
-//     k0 + 15 + 2 hbw) and the right-hand side, then the panel's pivot rows
-//     finished there (u_kj -= l_kk' u_k'j, k' ascending), in the same
-//     launch, one thread per column;
+//   * panel: one workgroup per system, one panel row per thread (rows k0 ..
+//     k0 + 15 + hbw, sixteen entries each in registers); per column one
+//     barrier: before it each wavefront holding rows publishes its first
+//     row of largest magnitude (DPP maximum, ballot) with the row's
+//     entries, after it every thread picks the pivot from the candidates,
+//     the two rows swap from LDS and the rows below update themselves.  The
+//     multipliers are stored where they were formed (the reference's band
+//     LU keeps them so, and its solves swap the right-hand side step by
+//     step); the trailing work takes them as the rows stand after the
+//     panel's last swap (rows carry their multipliers when they move),
+//     which gives each row the same products in the same order;
+//   * the panel's swaps applied to the trailing columns (k0 + 16 .. k0 + 15
+//     + 2 hbw) and the right-hand side as one permutation of the rows they
+//     touch (each slot loaded from the row whose entry ends there), then the
+//     panel's pivot rows finished there (u_kj -= l_kk' u_k'j, k' ascending),
+//     in registers, one thread per column, in the same launch;
 //   * the trailing update a_ij -= l_ik u_kj over the panel's sixteen k in
 //     ascending order, tiles of 64 x 64 entries over as many workgroups as
 //     the window holds (the right-hand side is one more column, so the
 //     forward substitution rides along);
+//   * bands with hbw <= 127 run all of it in one launch, one workgroup
+//     per system: the trailing columns one thread each (swaps and pivot rows
+//     in registers, the finished rows kept in LDS), then the trailing update
+//     over the workgroup's threads; wider bands take one launch for the
+//     panel and trailing columns and one for the trailing update per panel;
 //   * the back substitution by blocks of 64 rows from the bottom: the
 //     block's sums over the solved unknowns in parallel, then the block's
 //     triangle in one wavefront (its sums run in another order than the
@@ -49,8 +56,10 @@ namespace {
 
 constexpr int kBkNb = 16;                 // panel columns, one wavefront each
 constexpr int kBkThreads = 64 * kBkNb;    // 1024
-constexpr int kBkMaxM = 20;               // panel rows per lane: hbw + 16 <= 1280
+constexpr int kBkMaxRows = kBkThreads;    // panel rows, one per thread: hbw + 16 <= 1024
 constexpr int kBkTile = 64;               // trailing update tile (rows and columns)
+constexpr int kBkStage = 256;             // trailing columns per LDS stage of the pivot rows
+constexpr int kBkFusedMaxHbw = (kBkStage - 2) / 2;  // one-launch path: the window and the rhs in one stage
 constexpr int kBkBack = 64;               // back substitution block rows
 constexpr double kBkPivotTol = 1.0e-14;
 
@@ -80,21 +89,298 @@ __device__ __forceinline__ double lane_read(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// value of panel row i (relative index q = i - k0) held in a[q / 64] of lane q % 64
-__device__ __forceinline__ double bk_get(const double (&a)[kBkMaxM], int q) {
-    const int m = q >> 6, l = q & 63;
-    double v = 0.0;
-#pragma unroll
-    for (int t = 0; t < kBkMaxM; ++t)
-        if (t == m) v = lane_read(a[t], l);
-    return v;
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// LDS of one panel
+struct BkPanel {
+    double cand_v[2][kBkNb];          // per wavefront: largest magnitude of the column (double buffered)
+    int cand_i[2][kBkNb];             //   its first row
+    double cand_row[2][kBkNb][kBkNb]; //   and that row's entries
+    double krow[2][kBkNb];            // row k's entries (the row a swap sends down)
+    double l11[kBkNb][kBkNb];         // the pivot rows' multipliers as the rows finally stand
+    double rows[2][kBkNb];            // rows exchanged while the formed multipliers are restored
+    int p[kBkNb];                     // row swapped with k0 + q
+    int slot_row[2 * kBkNb];          // rows the swaps touch: the pivot rows, then other swapped rows
+    int slot_src[2 * kBkNb];          // slot whose row's entries end in slot t after all the swaps
+    int nslot;
+};
+
+template <int CTRL>
+__device__ __forceinline__ double bk_dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ void bk_set(double (&a)[kBkMaxM], int q, double v, int lane) {
-    const int m = q >> 6, l = q & 63;
+// largest value over the wavefront, in every lane
+__device__ __forceinline__ double bk_wave_max(double v) {
+    v = fmax(v, bk_dpp<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fmax(v, bk_dpp<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fmax(v, bk_dpp<0x141>(v));  // row_half_mirror
+    v = fmax(v, bk_dpp<0x140>(v));  // row_mirror
+    return fmax(fmax(lane_read(v, 0), lane_read(v, 16)), fmax(lane_read(v, 32), lane_read(v, 48)));
+}
+
+// Factor panel k0 with one panel row per thread (rows k0 .. r_end, at most
+// 1024): per column, a block-wide search for the first row of largest
+// magnitude (wavefront reductions, then the sixteen candidates in LDS), the
+// two rows exchanged through LDS, and every row below updating itself
+// against the pivot row.  Rows carry their multipliers when they move, so
+// on return r[] holds the panel's multipliers as the rows finally stand
+// (and U for the pivot rows).  Writes the U rows, the pivot rows, the slot
+// table of the trailing swaps, and with kStoreL the multipliers where they
+// were formed (the interleaved-swap convention of the reference's band LU
+// and of the later solves).  False (after setting status 2) for a singular
+// column.  Every thread of the workgroup calls it.
+template <bool kStoreL>
+__device__ __forceinline__ bool bk_panel(const BkSys &s, int n, int hbw, int k0, BkPanel &L, double (&r)[kBkNb]) {
+    const int V = 3 * hbw + 1;
+    auto at = [&](int i, int j) -> double & { return s.work[(size_t)i * V + (j - i + hbw)]; };
+    const int tid = threadIdx.x, lane = tid & 63, w = uniform(tid >> 6);
+    const int kb = min(kBkNb, n - k0);
+    const int r_end = min(k0 + kb - 1 + hbw, n - 1);
+    const int row = k0 + tid;
+    const bool active = row <= r_end;
 #pragma unroll
-    for (int t = 0; t < kBkMaxM; ++t)
-        if (t == m && lane == l) a[t] = v;
+    for (int c = 0; c < kBkNb; ++c) {
+        const int j = k0 + c;
+        r[c] = (active && c < kb && row - j <= hbw && j - row <= 2 * hbw) ? at(row, j) : 0.0;
+    }
+    bool swapped = false;
+    const int nw = uniform((r_end - k0 + 64) / 64);  // wavefronts holding panel rows
+    // Per column one barrier: before it, every wavefront holding rows
+    // publishes its first row of largest magnitude in the column, with the
+    // row's entries, and wavefront 0 row k's entries; after it, every thread
+    // picks the pivot from the candidates, the two rows swap from LDS, and
+    // the rows below update against the pivot row.  jj stays a loop
+    // variable: a row's entries keep fixed registers, column jj is picked
+    // out by selects.
+#pragma unroll 1
+    for (int jj = 0; jj < kb; ++jj) {  // kb is uniform: every thread takes the same path to the barrier
+        const int k = k0 + jj, kmax = min(k + hbw, n - 1), buf = jj & 1;
+        if (w < nw) {
+            double rj = 0.0;
+#pragma unroll
+            for (int c = 0; c < kBkNb; ++c) rj = c == jj ? r[c] : rj;
+            const double v = (active && row >= k && row <= kmax) ? fabs(rj) : -1.0;
+            const double m = bk_wave_max(v);
+            const uint64_t hit = __ballot(v == m);
+            const int f = (int)__builtin_ctzll(hit);  // rows ascend with the lane: the first of equals
+            if (lane == f) {
+                L.cand_v[buf][w] = m;
+                L.cand_i[buf][w] = row;
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) L.cand_row[buf][w][c] = r[c];
+            }
+            if (w == 0 && lane == jj)
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) L.krow[buf][c] = r[c];
+        }
+        __syncthreads();
+        double best = L.cand_v[buf][0];
+        int p = L.cand_i[buf][0], wb = 0;
+        for (int q = 1; q < nw; ++q) {
+            const double cv = L.cand_v[buf][q];
+            if (cv > best) {  // waves ascend with the rows: the first of equals stays
+                best = cv;
+                p = L.cand_i[buf][q];
+                wb = q;
+            }
+        }
+        p = uniform(p);
+        wb = uniform(wb);
+        if (!(best > kBkPivotTol)) {
+            if (tid == 0) s.status[0] = 2.0;
+            return false;  // uniform: every thread reduced the same candidates
+        }
+        if (tid == 0) L.p[jj] = p;
+        if (w < nw) {
+            if (p != k) {
+                swapped = true;
+                if (row == k)
+#pragma unroll
+                    for (int c = 0; c < kBkNb; ++c) r[c] = L.cand_row[buf][wb][c];
+                if (row == p)
+#pragma unroll
+                    for (int c = 0; c < kBkNb; ++c) r[c] = L.krow[buf][c];
+            }
+            if (active && row > k && row <= kmax) {
+                double aj = 0.0;
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) aj = c == jj ? r[c] : aj;
+                double pj = 0.0;
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) pj = c == jj ? L.cand_row[buf][wb][c] : pj;
+                const double l = aj / pj;
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) {
+                    const double pc = L.cand_row[buf][wb][c];
+                    if (c == jj) r[c] = l;
+                    if (c > jj && c < kb) r[c] = r[c] - l * pc;
+                }
+            }
+        }
+    }
+    // U rows, and the multipliers as the rows stand for the trailing work
+    if (active && row < k0 + kb) {
+#pragma unroll
+        for (int c = 0; c < kBkNb; ++c) {
+            const int j = k0 + c;
+            if (c < kb && j >= row && j - row <= 2 * hbw) at(row, j) = r[c];
+            L.l11[tid][c] = (c < kb && j < row) ? r[c] : 0.0;
+        }
+    }
+    if (tid < kb) s.piv[k0 + tid] = L.p[tid];
+    if (tid == 0) {  // the rows the swaps touch, and where their entries end up
+        int ns = kb;
+        for (int q = 0; q < kb; ++q) L.slot_row[q] = k0 + q;
+        int sl_p[kBkNb];
+        for (int q = 0; q < kb; ++q) {
+            const int pq = L.p[q];
+            int sl = pq - k0;
+            if (pq >= k0 + kb) {
+                sl = -1;
+                for (int t = kb; t < ns; ++t)
+                    if (L.slot_row[t] == pq) sl = t;
+                if (sl < 0) {
+                    sl = ns++;
+                    L.slot_row[sl] = pq;
+                }
+            }
+            sl_p[q] = sl;
+        }
+        for (int t = 0; t < ns; ++t) L.slot_src[t] = t;
+        for (int q = 0; q < kb; ++q) {
+            const int t2 = sl_p[q], tmp = L.slot_src[q];
+            L.slot_src[q] = L.slot_src[t2];
+            L.slot_src[t2] = tmp;
+        }
+        L.nslot = ns;
+        if (swapped && s.status[0] == 0.0) s.status[0] = 1.0;
+    }
+    __syncthreads();
+    if (kStoreL) {  // later swaps undone in reverse on a copy, then written once
+        double b[kBkNb];
+#pragma unroll
+        for (int c = 0; c < kBkNb; ++c) b[c] = r[c];
+        for (int q = kb - 1; q > 0; --q) {
+            const int pq = L.p[q], kq = k0 + q;
+            if (pq == kq) continue;  // uniform
+            if (row == kq)
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) L.rows[0][c] = b[c];
+            if (row == pq)
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c) L.rows[1][c] = b[c];
+            __syncthreads();
+            if (row == kq)
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c)
+                    if (c < q) b[c] = L.rows[1][c];
+            if (row == pq)
+#pragma unroll
+                for (int c = 0; c < kBkNb; ++c)
+                    if (c < q) b[c] = L.rows[0][c];
+            __syncthreads();
+        }
+        if (active)
+#pragma unroll
+            for (int c = 0; c < kBkNb; ++c) {
+                const int j = k0 + c;
+                if (c < kb && row > j && row - j <= hbw) at(row, j) = b[c];
+            }
+    }
+    return true;
+}
+
+// The trailing columns j0 + c (c < ncr; c == nc is the right-hand side),
+// kBkStage at a time through LDS: every slot (a row the panel's swaps touch)
+// loaded from the row whose entry ends there (the swaps' composition), the
+// pivot rows finished per column (u_kj -= l_kk' u_k'j, k' ascending), every
+// slot stored back.  The swaps move only entries the band holds or
+// structural zeros: a row swapped into a pivot row at a column past that
+// pivot row's window is zero there.  On return (with a single chunk)
+// stage[q][c] is pivot row k0 + q's finished entry (0 where the band holds
+// none).  Every thread of the workgroup calls it.
+__device__ __forceinline__ void bk_pivot_rows(const BkSys &s, int hbw, int k0, int kb, int j0, int nc, int ncr,
+                                              const BkPanel &L, double (*stage)[kBkStage]) {
+    const int V = 3 * hbw + 1, tid = threadIdx.x;
+    const int ns = uniform(L.nslot);
+    for (int cs = 0; cs < ncr; cs += kBkStage) {
+        const int cw = min(kBkStage, ncr - cs);
+        for (int e = tid; e < ns * cw; e += kBkThreads) {
+            const int t = e / cw, c = e % cw, j = j0 + cs + c;
+            const int src = L.slot_row[L.slot_src[t]], o = j - src + hbw;
+            double v = 0.0;
+            if (cs + c == nc)
+                v = s.x[src];
+            else if (o >= 0 && o <= 3 * hbw)
+                v = s.work[(size_t)src * V + o];
+            stage[t][c] = v;
+        }
+        __syncthreads();
+        if (tid < cw) {
+            const int c = tid, j = j0 + cs + c;
+            const bool rhs = cs + c == nc;
+            double u[kBkNb];
+#pragma unroll
+            for (int q = 0; q < kBkNb; ++q) u[q] = (q < kb && (rhs || j <= k0 + q + 2 * hbw)) ? stage[q][c] : 0.0;
+#pragma unroll
+            for (int q = 1; q < kBkNb; ++q) {
+                if (q < kb && (rhs || j <= k0 + q + 2 * hbw)) {
+                    double t = u[q];
+#pragma unroll
+                    for (int q2 = 0; q2 < q; ++q2)
+                        if (rhs || j <= k0 + q2 + 2 * hbw) t = t - L.l11[q][q2] * u[q2];
+                    u[q] = t;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kBkNb; ++q)
+                if (q < kb) stage[q][c] = u[q];
+        }
+        __syncthreads();
+        for (int e = tid; e < ns * cw; e += kBkThreads) {
+            const int t = e / cw, c = e % cw, j = j0 + cs + c;
+            const int dst = L.slot_row[t], o = j - dst + hbw;
+            if (cs + c == nc)
+                s.x[dst] = stage[t][c];
+            else if (o >= 0 && o <= 3 * hbw)
+                s.work[(size_t)dst * V + o] = stage[t][c];
+        }
+        __syncthreads();
+    }
+}
+
+// x_i = (y_i - sum_{j > i} u_ij x_j) / u_ii by blocks of 64 rows from the
+// bottom; U row i at work columns i .. i + 2 hbw.  Every thread calls it.
+__device__ __forceinline__ void bk_back(const BkSys &s, int n, int hbw, double *s_s) {
+    const int V = 3 * hbw + 1;
+    auto at = [&](int i, int j) -> double { return s.work[(size_t)i * V + (j - i + hbw)]; };
+    const int tid = threadIdx.x, lane = tid & 63, w = uniform(tid >> 6);
+    for (int r1 = n; r1 > 0; r1 -= kBkBack) {
+        const int r0 = max(r1 - kBkBack, 0);
+        for (int i = r0 + w; i < r1; i += kBkNb) {  // the block's sums over the solved unknowns
+            const int jend = min(i + 2 * hbw, n - 1);
+            double part = 0.0;
+            for (int j = r1 + lane; j <= jend; j += 64) part += at(i, j) * s.x[j];
+            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+            if (lane == 0) s_s[i - r0] = part;
+        }
+        __syncthreads();
+        if (w == 0) {  // the block's triangle, by columns
+            const int q = r0 + lane;
+            double t = q < r1 ? s.x[q] - s_s[lane] : 0.0;
+            for (int i = r1 - 1; i >= r0; --i) {
+                double xi = 0.0;
+                if (lane == i - r0) xi = t / at(i, i);
+                xi = lane_read(xi, i - r0);
+                if (lane == i - r0) s.x[i] = xi;
+                if (q < i && i <= q + 2 * hbw) t = t - at(q, i) * xi;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace
@@ -116,150 +402,28 @@ __global__ void bk_init_kernel(const int32_t *recs, int n, int hbw, const double
     if (q == 0) s.status[0] = 0.0;
 }
 
-// Panel k0 .. k0 + kb - 1 of every listed system, then its swaps and pivot
-// rows over the trailing columns and the right-hand side.
+// Panel k0 of every listed system, then its trailing columns' swaps and
+// pivot rows (one thread per column); the multipliers as the rows finally
+// stand go to the system's scratch for bk_update_kernel.
+template <bool kStoreL>
 __global__ __launch_bounds__(kBkThreads) void bk_panel_kernel(const int32_t *recs, int n, int hbw, int k0,
                                                               double *work, double *x, double *status) {
     const BkSys s = bk_sys(recs, n, hbw, work, x, status);
     if (s.status[0] == 2.0) return;  // singular earlier: uniform exit
-    const int V = 3 * hbw + 1;
-    auto at = [&](int i, int j) -> double & { return s.work[(size_t)i * V + (j - i + hbw)]; };
-    __shared__ double s_l[2][64 * kBkMaxM];
-    __shared__ double s_l11[kBkNb][kBkNb];
-    __shared__ int s_p[kBkNb];
-    __shared__ int s_swapped;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int kb = min(kBkNb, n - k0);
-    const int r_end = min(k0 + kb - 1 + hbw, n - 1);
-    const int col = k0 + w;
-    if (tid == 0) s_swapped = 0;
-    double a[kBkMaxM];
+    __shared__ BkPanel L;
+    double r[kBkNb];
+    if (!bk_panel<kStoreL>(s, n, hbw, k0, L, r)) return;
+    const int tid = threadIdx.x;
+    const int kb = min(kBkNb, n - k0), r_end = min(k0 + kb - 1 + hbw, n - 1);
+    if (tid >= kb && k0 + tid <= r_end)
 #pragma unroll
-    for (int m = 0; m < kBkMaxM; ++m) {
-        const int i = k0 + lane + 64 * m;
-        a[m] = (w < kb && i <= r_end && i - col <= hbw && col - i <= 2 * hbw) ? at(i, col) : 0.0;
-    }
-    __syncthreads();
-    for (int jj = 0; jj < kb; ++jj) {
-        const int k = k0 + jj, kmax = min(k + hbw, n - 1), buf = jj & 1;
-        if (w == jj) {
-            double best = -1.0;
-            int bi = 0x7fffffff;
-#pragma unroll
-            for (int m = 0; m < kBkMaxM; ++m) {
-                const int i = k0 + lane + 64 * m;
-                if (i >= k && i <= kmax) {
-                    const double v = fabs(a[m]);
-                    if (v > best) {  // rows ascend within a lane: the first of equals stays
-                        best = v;
-                        bi = i;
-                    }
-                }
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(best, off, 64);
-                const int oi = __shfl_xor(bi, off, 64);
-                if (ov > best || (ov == best && oi < bi)) {
-                    best = ov;
-                    bi = oi;
-                }
-            }
-            if (!(best > kBkPivotTol)) {
-                if (lane == 0) s_p[jj] = -1;
-            } else {
-                const int p = bi;
-                const double vk = bk_get(a, k - k0), vp = bk_get(a, p - k0);
-                if (p != k) {
-                    bk_set(a, k - k0, vp, lane);
-                    bk_set(a, p - k0, vk, lane);
-                }
-                const double diag = vp;
-#pragma unroll
-                for (int m = 0; m < kBkMaxM; ++m) {
-                    const int i = k0 + lane + 64 * m;
-                    if (i > k && i <= kmax) {
-                        a[m] = a[m] / diag;
-                        s_l[buf][i - k0] = a[m];
-                        at(i, k) = a[m];  // kept where it was formed (the interleaved-swap convention)
-                    }
-                }
-                if (lane == 0) {
-                    s_p[jj] = p;
-                    if (p != k) s_swapped = 1;
-                }
-            }
-        }
-        __syncthreads();
-        const int p = s_p[jj];
-        if (p < 0) {
-            if (tid == 0) s.status[0] = 2.0;
-            return;  // uniform: every thread read the same s_p
-        }
-        if (w != jj && w < kb && p != k) {  // earlier columns too: their multipliers follow the rows
-            const double vk = bk_get(a, k - k0), vp = bk_get(a, p - k0);
-            bk_set(a, k - k0, vp, lane);
-            bk_set(a, p - k0, vk, lane);
-        }
-        if (w > jj && w < kb) {
-            const double u = bk_get(a, k - k0);
-#pragma unroll
-            for (int m = 0; m < kBkMaxM; ++m) {
-                const int i = k0 + lane + 64 * m;
-                if (i > k && i <= kmax) a[m] = a[m] - s_l[buf][i - k0] * u;
-            }
-        }
-    }
-    // U rows of the panel back to memory (the later swaps never reach them);
-    // the multipliers as the rows finally stand, for the trailing work
-#pragma unroll
-    for (int m = 0; m < kBkMaxM; ++m) {
-        const int i = k0 + lane + 64 * m;
-        if (w < kb && i <= col && col - i <= 2 * hbw) at(i, col) = a[m];
-        if (w < kb && i > col && i <= r_end) s.lt[(size_t)w * (hbw + kBkNb) + (i - k0)] = a[m];
-    }
-    if (w < kb && lane < kBkNb) s_l11[lane][w] = (lane < kb && lane > w) ? a[0] : 0.0;
-    if (tid < kb) s.piv[k0 + tid] = s_p[tid];
-    __syncthreads();
-    if (tid == 0 && s_swapped && s.status[0] == 0.0) s.status[0] = 1.0;
-    // trailing columns k0 + kb .. jmax and the right-hand side (c == nc)
+        for (int c = 0; c < kBkNb; ++c)
+            if (c < kb) s.lt[(size_t)c * (hbw + kBkNb) + tid] = r[c];
     const int j0 = k0 + kb, jmax = min(k0 + kb - 1 + 2 * hbw, n - 1);
     const int nc = jmax >= j0 ? jmax - j0 + 1 : 0;
     const int ncr = nc + (s.x ? 1 : 0);  // the right-hand side is column nc
-    for (int c = tid; c < ncr; c += kBkThreads) {
-        const bool rhs = c == nc;
-        const int j = j0 + c;
-        for (int q = 0; q < kb; ++q) {  // the swaps in order (row k holds columns up to k + 2 hbw)
-            const int k = k0 + q, p = s_p[q];
-            if (p == k) continue;
-            if (rhs) {
-                const double t = s.x[k];
-                s.x[k] = s.x[p];
-                s.x[p] = t;
-            } else if (j <= k + 2 * hbw) {
-                const double t = at(k, j);
-                at(k, j) = at(p, j);
-                at(p, j) = t;
-            }
-        }
-        double u[kBkNb];
-#pragma unroll
-        for (int q = 0; q < kBkNb; ++q) {
-            const int k = k0 + q;
-            u[q] = 0.0;
-            if (q < kb && (rhs || j <= k + 2 * hbw)) {
-                double t = rhs ? s.x[k] : at(k, j);
-#pragma unroll
-                for (int q2 = 0; q2 < kBkNb; ++q2)
-                    if (q2 < q && (rhs || j <= k0 + q2 + 2 * hbw)) t = t - s_l11[q][q2] * u[q2];
-                u[q] = t;
-                if (rhs)
-                    s.x[k] = t;
-                else
-                    at(k, j) = t;
-            }
-        }
-    }
+    __shared__ double stage[2 * kBkNb][kBkStage];
+    bk_pivot_rows(s, hbw, k0, kb, j0, s.x ? nc : -1, ncr, L, stage);
 }
 
 // a_ij -= sum over the panel's k (ascending) of l_ik u_kj, rows k0 + kb ..
@@ -321,40 +485,56 @@ __global__ __launch_bounds__(256) void bk_update_kernel(const int32_t *recs, int
     }
 }
 
-// x_i = (y_i - sum_{j > i} u_ij x_j) / u_ii by blocks of 64 rows from the
-// bottom; U row i at work columns i .. i + 2 hbw
 __global__ __launch_bounds__(kBkThreads) void bk_back_kernel(const int32_t *recs, int n, int hbw, double *work,
                                                              double *x, double *status) {
     const BkSys s = bk_sys(recs, n, hbw, work, x, status);
     if (s.status[0] == 2.0) return;
-    const int V = 3 * hbw + 1;
-    auto at = [&](int i, int j) -> double { return s.work[(size_t)i * V + (j - i + hbw)]; };
     __shared__ double s_s[kBkBack];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int r1 = n; r1 > 0; r1 -= kBkBack) {
-        const int r0 = max(r1 - kBkBack, 0);
-        // each wavefront: rows r0 + w, r0 + w + 16, ...; lanes over the solved unknowns j >= r1
-        for (int i = r0 + w; i < r1; i += kBkNb) {
-            const int jend = min(i + 2 * hbw, n - 1);
-            double part = 0.0;
-            for (int j = r1 + lane; j <= jend; j += 64) part += at(i, j) * s.x[j];
-            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-            if (lane == 0) s_s[i - r0] = part;
-        }
-        __syncthreads();
-        if (w == 0) {
-            const int q = r0 + lane;
-            double t = q < r1 ? s.x[q] - s_s[lane] : 0.0;
-            for (int i = r1 - 1; i >= r0; --i) {
-                double xi = 0.0;
-                if (lane == i - r0) xi = t / at(i, i);
-                xi = lane_read(xi, i - r0);
-                if (lane == i - r0) s.x[i] = xi;
-                if (q < i && i <= q + 2 * hbw) t = t - at(q, i) * xi;
-            }
+    bk_back(s, n, hbw, s_s);
+}
+
+// Narrow bands in one launch, one workgroup per system: every panel, its
+// trailing columns (one thread per column: swaps and pivot rows, the
+// results kept in LDS) and its trailing update (the workgroup's threads
+// over the window's entries), then the back substitution.
+template <int M, bool kStoreL>
+__global__ __launch_bounds__(kBkThreads) void bk_fused_kernel(const int32_t *recs, int n, int hbw, double *work,
+                                                              double *x, double *status) {
+    const BkSys s = bk_sys(recs, n, hbw, work, x, status);
+    if (s.status[0] == 2.0) return;
+    const int V = 3 * hbw + 1;
+    auto at = [&](int i, int j) -> double & { return s.work[(size_t)i * V + (j - i + hbw)]; };
+    __shared__ BkPanel L;
+    __shared__ double s_lt[64 * M][kBkNb + 1];   // multipliers as the rows finally stand
+    __shared__ double stage[2 * kBkNb][kBkStage];  // the window's slots; then its finished pivot rows
+    __shared__ double s_s[kBkBack];
+    const int tid = threadIdx.x;
+    const bool has_rhs = s.x != nullptr;
+    double r[kBkNb];
+    for (int k0 = 0; k0 < n; k0 += kBkNb) {
+        if (!bk_panel<kStoreL>(s, n, hbw, k0, L, r)) return;
+        const int kb = min(kBkNb, n - k0), r_end = min(k0 + kb - 1 + hbw, n - 1);
+        if (tid >= kb && k0 + tid <= r_end)
+#pragma unroll
+            for (int c = 0; c < kBkNb; ++c) s_lt[tid][c] = c < kb ? r[c] : 0.0;
+        const int j0 = k0 + kb, jmax = min(k0 + kb - 1 + 2 * hbw, n - 1);
+        const int nc = jmax >= j0 ? jmax - j0 + 1 : 0;
+        const int ncr = nc + (has_rhs ? 1 : 0);  // <= kBkStage: one stage
+        __syncthreads();  // s_lt complete
+        bk_pivot_rows(s, hbw, k0, kb, j0, has_rhs ? nc : -1, ncr, L, stage);
+        const int i0 = k0 + kb, nr = r_end - i0 + 1;
+        for (int e = tid; e < nr * ncr; e += kBkThreads) {
+            const int rr = e / ncr, c = e % ncr, i = i0 + rr;
+            double *tp = (has_rhs && c == nc) ? s.x + i : &at(i, j0 + c);
+            double t = *tp;
+#pragma unroll
+            for (int q = 0; q < kBkNb; ++q)
+                if (q < kb) t = t - s_lt[i - k0][q] * stage[q][c];
+            *tp = t;
         }
         __syncthreads();
     }
+    if (has_rhs) bk_back(s, n, hbw, s_s);
 }
 
 // The factors in the other band kernels' formats: U row k at band slots
@@ -378,7 +558,7 @@ __global__ void bk_store_kernel(const int32_t *recs, int n, int hbw, double *wor
     if (ipiv && o == 0) ipiv[(size_t)r * n + k] = s.piv[k];
 }
 
-bool band_blk_fits(int n, int hbw) { return n > 0 && hbw >= 0 && hbw + kBkNb <= 64 * kBkMaxM; }
+bool band_blk_fits(int n, int hbw) { return n > 0 && hbw >= 0 && hbw + kBkNb <= kBkMaxRows; }
 
 size_t band_blk_work_doubles(int n, int hbw) {
     return (size_t)n * (3 * (size_t)hbw + 1) + ((size_t)n + 1) / 2 + 1 + (size_t)kBkNb * (hbw + kBkNb);
@@ -392,29 +572,40 @@ int launch_band_blk(sgpu_ctx *ctx, int n, int hbw, double *d_band, const double 
                     double *d_status, double *d_L, int32_t *d_ipiv, double *d_work, const int32_t *recs, int count) {
     if (!band_blk_fits(n, hbw))
         return fail(ctx, SGPU_ERR_ARG, "half bandwidth %d exceeds the blocked band solve (%d)", hbw,
-                    64 * kBkMaxM - kBkNb);
+                    kBkMaxRows - kBkNb);
     if (!d_work || (d_rhs && !d_x)) return fail(ctx, SGPU_ERR_ARG, "the blocked band solve needs work (and x)");
     if (!d_rhs) d_x = nullptr;
     hipStream_t st = ctx->stream;
     const int64_t cells = (int64_t)n * (3 * hbw + 1);
     hipLaunchKernelGGL(bk_init_kernel, dim3((unsigned)((cells + 255) / 256), count), dim3(256), 0, st, recs, n, hbw,
                        d_band, d_rhs, d_work, d_x, d_status);
-    for (int k0 = 0; k0 < n; k0 += kBkNb) {
-        hipLaunchKernelGGL(bk_panel_kernel, dim3(1, count), dim3(kBkThreads), 0, st, recs, n, hbw, k0, d_work, d_x,
-                           d_status);
-        const int kb = std::min(kBkNb, n - k0);
-        const int nr = std::min(k0 + kb - 1 + hbw, n - 1) - (k0 + kb) + 1;
-        if (nr > 0) {
-            const int jmax = std::min(k0 + kb - 1 + 2 * hbw, n - 1);
-            const int nc = (jmax >= k0 + kb ? jmax - (k0 + kb) + 1 : 0) + 1;
-            const int tiles = ((nr + kBkTile - 1) / kBkTile) * ((nc + kBkTile - 1) / kBkTile);
-            hipLaunchKernelGGL(bk_update_kernel, dim3(tiles, count), dim3(256), 0, st, recs, n, hbw, k0, d_work, d_x,
+    const int M = (hbw + kBkNb + 63) / 64;
+    const bool keep = d_L != nullptr;  // the multipliers where they were formed, for later solves
+    if (hbw <= kBkFusedMaxHbw) {  // one launch: panels, trailing work and back substitution in one workgroup
+        auto fused = keep ? (M <= 1 ? bk_fused_kernel<1, true> : M == 2 ? bk_fused_kernel<2, true>
+                                                                       : bk_fused_kernel<3, true>)
+                          : (M <= 1 ? bk_fused_kernel<1, false> : M == 2 ? bk_fused_kernel<2, false>
+                                                                        : bk_fused_kernel<3, false>);
+        hipLaunchKernelGGL(fused, dim3(1, count), dim3(kBkThreads), 0, st, recs, n, hbw, d_work, d_x, d_status);
+    } else {
+        auto panel = keep ? bk_panel_kernel<true> : bk_panel_kernel<false>;
+        for (int k0 = 0; k0 < n; k0 += kBkNb) {
+            hipLaunchKernelGGL(panel, dim3(1, count), dim3(kBkThreads), 0, st, recs, n, hbw, k0, d_work, d_x,
                                d_status);
+            const int kb = std::min(kBkNb, n - k0);
+            const int nr = std::min(k0 + kb - 1 + hbw, n - 1) - (k0 + kb) + 1;
+            if (nr > 0) {
+                const int jmax = std::min(k0 + kb - 1 + 2 * hbw, n - 1);
+                const int nc = (jmax >= k0 + kb ? jmax - (k0 + kb) + 1 : 0) + 1;
+                const int tiles = ((nr + kBkTile - 1) / kBkTile) * ((nc + kBkTile - 1) / kBkTile);
+                hipLaunchKernelGGL(bk_update_kernel, dim3(tiles, count), dim3(256), 0, st, recs, n, hbw, k0, d_work,
+                                   d_x, d_status);
+            }
         }
+        if (d_x)
+            hipLaunchKernelGGL(bk_back_kernel, dim3(1, count), dim3(kBkThreads), 0, st, recs, n, hbw, d_work, d_x,
+                               d_status);
     }
-    if (d_x)
-        hipLaunchKernelGGL(bk_back_kernel, dim3(1, count), dim3(kBkThreads), 0, st, recs, n, hbw, d_work, d_x,
-                           d_status);
     if (d_L || !d_x) {
         const int64_t slots = (int64_t)n * (2 * hbw + 1);
         hipLaunchKernelGGL(bk_store_kernel, dim3((unsigned)((slots + 255) / 256), count), dim3(256), 0, st, recs, n,
