@@ -463,6 +463,60 @@ def frame3d_leg(steps: int, warmup: int, hbw_rule: int = 0, records: int = 32) -
             "element_state_ms_avg": elem_ms / max(elem_n, 1), "band_solve_ms_avg": solve_ms / max(solve_n, 1)}
 
 
+def tower_leg(steps: int, warmup: int, records: int, modal_rotations: int) -> dict:
+    """The large single-GPU case (60 stories x 6 x 6 bays around a 2 x 2-bay
+    elastic shell core, 33,480 free DOFs, half bandwidth 560): `records`
+    bidirectional seeded records together (PGA 0.3-0.9 g in x, 0.7 of it in
+    y), the modal analysis of the frame at rest with the reference's
+    Jacobi (`modal_rotations` rotations), then `warmup` and `steps` timed
+    steps of every record; element state and band solve per Newton
+    iteration."""
+    from strut_amd import _native, cases
+    from strut_amd.frame3d import Pattern, load_frame3d_case
+    from strut_amd.transient3d import Frame3dRun
+
+    pga = lambda k: 0.3 + 0.6 * k / max(records - 1, 1)  # noqa: E731
+    n_steps = warmup + steps
+    fc = load_frame3d_case(cases.config4_case(steps=n_steps, seeds=(0, 1), pgas_g=(pga(0), 0.7 * pga(0))))
+    recs = []
+    for k in range(1, records):
+        c = cases.config4_case(steps=n_steps, seeds=(2 * k, 2 * k + 1), pgas_g=(pga(k), 0.7 * pga(k)))
+        recs.append([Pattern(p.direction, p.values, p.dt, p.factor) for p in load_frame3d_case(c).patterns])
+    t0 = time.perf_counter()
+    run = Frame3dRun(fc, records=recs)
+    setup_s = time.perf_counter() - t0
+    modal = None
+    if modal_rotations > 0:
+        t0 = time.perf_counter()
+        eig, _, rot = run.modal(6, max_rotations=modal_rotations)
+        modal = {"rotations": rot, "seconds": time.perf_counter() - t0, "eigenvalues": [float(v) for v in eig],
+                 "note": "the reference's classical Jacobi stops at 100 rotations, far from converged at this size"}
+    run.run_records(warmup)
+    run.rt.profile_enable(True)
+    t0 = time.perf_counter()
+    r = run.run_records(steps)
+    dt = time.perf_counter() - t0
+    elem_ms, elem_n = run.rt.profile_read(_native.KERNEL_F3_ELEM)
+    solve_ms, solve_n = run.rt.profile_read(_native.KERNEL_F3_SOLVE)
+    run.rt.profile_enable(False)
+    run.close()
+    done = int(r["steps_done"].sum())
+    iters = int(r["iters"].sum())
+    lockstep = int(np.max(r["iters"], axis=0).sum())  # the records iterate together
+    n_fib = sum(len(fc.section_cells[int(fc.elem_section[e])]["y"]) * int(fc.elem_nip[e]) for e in range(fc.n_elem))
+    return {"frame": "3D RC tower 60 stories x 6 x 6 bays, 2 x 2-bay elastic ShellMITC4 core",
+            "nodes": fc.n_nodes, "beam_columns": fc.n_elem, "shells": len(fc.shell_nodes), "fibers": n_fib,
+            "free_dofs": run.free_count, "hbw": run.hbw, "records": records, "setup_seconds": setup_s, "modal": modal,
+            "warmup_steps": warmup, "steps": steps, "record_steps": done, "seconds": dt,
+            "record_steps_per_s": done / dt, "newton_iters": iters, "lockstep_iters": lockstep,
+            "ms_per_lockstep_iter": 1e3 * dt / max(lockstep, 1), "status": [int(v) for v in r["status"]],
+            "element_state_ms_avg": elem_ms / max(elem_n, 1), "band_solve_ms_avg": solve_ms / max(solve_n, 1),
+            "band_solver": "blocked row-swap LU (sixteen-column panels), all records in each launch",
+            "cpu_baseline": None,
+            "cpu_baseline_note": "not run: the oracle's driver is dense (K of 33,480^2 doubles, 9 GB, and a "
+                                 "dense collapse per Newton iteration)"}
+
+
 def frame3d_cpu_baseline(steps: int) -> dict:
     from oracle import oracle
     from strut_amd import cases
@@ -729,6 +783,10 @@ def main() -> int:
     ap.add_argument("--f3-warmup", type=int, default=2)
     ap.add_argument("--f3-cpu-steps", type=int, default=6, help="steps of the 3D frame leg's CPU baseline sample")
     ap.add_argument("--f3-records", type=int, default=32, help="record pairs run together in the 3D frame leg (1 = skip)")
+    ap.add_argument("--f4-steps", type=int, default=2, help="timed steps of the tower leg (0 = skip)")
+    ap.add_argument("--f4-warmup", type=int, default=1)
+    ap.add_argument("--f4-records", type=int, default=16, help="records run together in the tower leg")
+    ap.add_argument("--f4-modal", type=int, default=100, help="Jacobi rotations of the tower's modal analysis (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -766,6 +824,15 @@ def main() -> int:
         config["frame3d"] = {"workload": "3D RC frame 20 stories x 5 x 5 bays, forceBeamColumn3d columns (PDelta) and "
                                          "dispBeamColumn3d beams, 5 IPs, 268-fiber sections, rigid diaphragms "
                                          "(Transformation), two seeded records 0.5 g / 0.35 g, dt 0.005 s", **f3}
+    if args.f4_steps > 0:
+        f4 = tower_leg(args.f4_steps, args.f4_warmup, args.f4_records, args.f4_modal)
+        _progress(f"tower leg: {f4['record_steps_per_s']:.3g} record-steps/s over {f4['records']} records, "
+                  f"{f4['ms_per_lockstep_iter']:.4g} ms per lockstep Newton iteration (band solve "
+                  f"{f4['band_solve_ms_avg']:.4g} ms, element state {f4['element_state_ms_avg']:.4g} ms)")
+        config["tower"] = {"workload": "configuration 4: 3D RC tower 60 stories x 6 x 6 bays (forceBeamColumn3d "
+                                       "columns, dispBeamColumn3d beams, 5 IPs, 152-fiber sections) around a 2 x "
+                                       "2-bay elastic ShellMITC4 core wall (0.4 m, E 30 GPa, nu 0.2), rigid "
+                                       "diaphragms, bidirectional seeded records, dt 0.005 s", **f4}
     if args.ens_motions > 0:
         from strut_amd import _native
         sched = {"auto": _native.SCHEDULE_AUTO, "rounds": _native.SCHEDULE_ROUNDS,
