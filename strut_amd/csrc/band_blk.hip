@@ -462,25 +462,48 @@ __global__ __launch_bounds__(256) void bk_update_kernel(const int32_t *recs, int
         s_ut[q][c] = v;
     }
     __syncthreads();
+    // 4 x 4 entries per thread, kept in registers over the panel's columns
+    // q: each q reads four multipliers and four pivot-row entries from LDS
+    // for sixteen updates.  Every entry still sees t - l_iq u_qj for
+    // q = 0 .. kb - 1 in order.
     const int tr = (tid / 16) * 4, tc = (tid % 16) * 4;
+    double t[4][4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int cc = tj * kBkTile + tc + b;
-        if (cc >= nc) continue;
-        const bool rhs = has_rhs && cc == nc - 1;
-        const int j = j0 + cc;
+    for (int a = 0; a < 4; ++a) {
+        const int i = i0 + ti * kBkTile + tr + a;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int i = i0 + ti * kBkTile + tr + a;
-            if (i > r_end) continue;
-            double t = rhs ? s.x[i] : at(i, j);
+        for (int b = 0; b < 4; ++b) {
+            const int cc = tj * kBkTile + tc + b;
+            t[a][b] = 0.0;
+            if (i <= r_end && cc < nc) t[a][b] = (has_rhs && cc == nc - 1) ? s.x[i] : at(i, j0 + cc);
+        }
+    }
 #pragma unroll
-            for (int q = 0; q < kBkNb; ++q)
-                if (q < kb) t = t - s_lt[tr + a][q] * s_ut[q][tc + b];
-            if (rhs)
-                s.x[i] = t;
+    for (int q = 0; q < kBkNb; ++q) {
+        if (q < kb) {
+            double lt[4], ut[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) lt[a] = s_lt[tr + a][q];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) ut[b] = s_ut[q][tc + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) t[a][b] = t[a][b] - lt[a] * ut[b];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = i0 + ti * kBkTile + tr + a;
+        if (i > r_end) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int cc = tj * kBkTile + tc + b;
+            if (cc >= nc) continue;
+            if (has_rhs && cc == nc - 1)
+                s.x[i] = t[a][b];
             else
-                at(i, j) = t;
+                at(i, j0 + cc) = t[a][b];
         }
     }
 }
