@@ -177,18 +177,13 @@ __device__ __forceinline__ bool bk_panel(const BkSys &s, int n, int hbw, int k0,
                 for (int c = 0; c < kBkNb; ++c) L.krow[buf][c] = r[c];
         }
         __syncthreads();
-        double best = L.cand_v[buf][0];
-        int p = L.cand_i[buf][0], wb = 0;
-        for (int q = 1; q < nw; ++q) {
-            const double cv = L.cand_v[buf][q];
-            if (cv > best) {  // waves ascend with the rows: the first of equals stays
-                best = cv;
-                p = L.cand_i[buf][q];
-                wb = q;
-            }
-        }
-        p = uniform(p);
-        wb = uniform(wb);
+        // the wavefronts' candidates side by side in lanes 0..nw-1: the
+        // largest, and among equals the first wavefront (waves ascend with
+        // the rows), as the scan over the rows would pick it
+        const double cv = lane < nw ? L.cand_v[buf][lane] : -2.0;
+        const double best = bk_wave_max(cv);
+        const int wb = uniform((int)__builtin_ctzll(__ballot(cv == best)));
+        const int p = uniform(L.cand_i[buf][wb]);
         if (!(best > kBkPivotTol)) {
             if (tid == 0) s.status[0] = 2.0;
             return false;  // uniform: every thread reduced the same candidates
