@@ -297,7 +297,7 @@ __device__ __forceinline__ bool bk_panel(const BkSys &s, int n, int hbw, int k0,
 // pivot row's window is zero there.  On return (with a single chunk)
 // stage[q][c] is pivot row k0 + q's finished entry (0 where the band holds
 // none).  Every thread of the workgroup calls it.
-__device__ __forceinline__ void bk_pivot_rows(const BkSys &s, int hbw, int k0, int kb, int j0, int nc, int ncr,
+__device__ __forceinline__ void bk_pivot_rows_staged(const BkSys &s, int hbw, int k0, int kb, int j0, int nc, int ncr,
                                               const BkPanel &L, double (*stage)[kBkStage]) {
     const int V = 3 * hbw + 1, tid = threadIdx.x;
     const int ns = uniform(L.nslot);
@@ -344,6 +344,64 @@ __device__ __forceinline__ void bk_pivot_rows(const BkSys &s, int hbw, int k0, i
                 s.work[(size_t)dst * V + o] = stage[t][c];
         }
         __syncthreads();
+    }
+}
+
+// The panel's swaps and pivot rows over the trailing columns k0 + kb ..
+// jmax (and the right-hand side, column nc; nc < 0 without one): one thread
+// per column.  A column's entries in every slot the swaps touch are read
+// into registers (slot t takes the entries of slot_src[t]), the pivot rows
+// are solved against L11 (the reference's order: u_q -= l_qp u_p, ascending
+// p), and every slot is written back.  Columns are independent, so no
+// barrier is needed; all of a column's loads issue before its first use.
+__device__ __forceinline__ void bk_pivot_rows_cols(const BkSys &s, int hbw, int k0, int kb, int j0, int nc,
+                                                   int ncr, const BkPanel &L) {
+    const int V = 3 * hbw + 1, tid = threadIdx.x;
+    const int ns = uniform(L.nslot);
+    // entry (row, o) of the system at 32-bit offset row V + o (a system's
+    // work holds n V doubles, under 2^32)
+    const double *__restrict__ wk = s.work;
+    for (int c = tid; c < ncr; c += kBkThreads) {
+        // keeps the L11 reads in the loop (hoisted, 120 values would take
+        // the registers the column needs)
+        asm volatile("" ::: "memory");
+        const int j = j0 + c;
+        const bool rhs = c == nc;
+        double v[2 * kBkNb];
+#pragma unroll
+        for (int t = 0; t < 2 * kBkNb; ++t) {
+            v[t] = 0.0;
+            if (t < ns) {
+                const int src = L.slot_row[L.slot_src[t]], o = j - src + hbw;
+                if (rhs)
+                    v[t] = s.x[src];
+                else if (o >= 0 && o <= 3 * hbw)
+                    v[t] = wk[(uint32_t)src * (uint32_t)V + (uint32_t)o];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kBkNb; ++q)
+            if (q < kb && !(rhs || j <= k0 + q + 2 * hbw)) v[q] = 0.0;
+#pragma unroll
+        for (int q = 1; q < kBkNb; ++q) {
+            if (q < kb && (rhs || j <= k0 + q + 2 * hbw)) {
+                double t = v[q];
+#pragma unroll
+                for (int q2 = 0; q2 < q; ++q2)
+                    if (rhs || j <= k0 + q2 + 2 * hbw) t = t - L.l11[q][q2] * v[q2];
+                v[q] = t;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2 * kBkNb; ++t) {
+            if (t < ns) {
+                const int dst = L.slot_row[t], o = j - dst + hbw;
+                if (rhs)
+                    s.x[dst] = v[t];
+                else if (o >= 0 && o <= 3 * hbw)
+                    s.work[(uint32_t)dst * (uint32_t)V + (uint32_t)o] = v[t];
+            }
+        }
     }
 }
 
@@ -417,8 +475,7 @@ __global__ __launch_bounds__(kBkThreads) void bk_panel_kernel(const int32_t *rec
     const int j0 = k0 + kb, jmax = min(k0 + kb - 1 + 2 * hbw, n - 1);
     const int nc = jmax >= j0 ? jmax - j0 + 1 : 0;
     const int ncr = nc + (s.x ? 1 : 0);  // the right-hand side is column nc
-    __shared__ double stage[2 * kBkNb][kBkStage];
-    bk_pivot_rows(s, hbw, k0, kb, j0, s.x ? nc : -1, ncr, L, stage);
+    bk_pivot_rows_cols(s, hbw, k0, kb, j0, s.x ? nc : -1, ncr, L);
 }
 
 // a_ij -= sum over the panel's k (ascending) of l_ik u_kj, rows k0 + kb ..
@@ -539,7 +596,7 @@ __global__ __launch_bounds__(kBkThreads) void bk_fused_kernel(const int32_t *rec
         const int nc = jmax >= j0 ? jmax - j0 + 1 : 0;
         const int ncr = nc + (has_rhs ? 1 : 0);  // <= kBkStage: one stage
         __syncthreads();  // s_lt complete
-        bk_pivot_rows(s, hbw, k0, kb, j0, has_rhs ? nc : -1, ncr, L, stage);
+        bk_pivot_rows_staged(s, hbw, k0, kb, j0, has_rhs ? nc : -1, ncr, L, stage);
         const int i0 = k0 + kb, nr = r_end - i0 + 1;
         for (int e = tid; e < nr * ncr; e += kBkThreads) {
             const int rr = e / ncr, c = e % ncr, i = i0 + rr;
