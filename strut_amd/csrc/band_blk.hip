@@ -155,8 +155,9 @@ __device__ __forceinline__ bool bk_panel(const BkSys &s, int n, int hbw, int k0,
     // the rows below update against the pivot row.  jj stays a loop
     // variable: a row's entries keep fixed registers, column jj is picked
     // out by selects.
-#pragma unroll 1
-    for (int jj = 0; jj < kb; ++jj) {  // kb is uniform: every thread takes the same path to the barrier
+#pragma unroll
+    for (int jj = 0; jj < kBkNb; ++jj) {  // kb is uniform: every thread takes the same path to the barrier
+        if (jj >= kb) break;
         const int k = k0 + jj, kmax = min(k + hbw, n - 1), buf = jj & 1;
         if (w < nw) {
             double rj = 0.0;
