@@ -63,21 +63,26 @@ constexpr int kBkFusedMaxHbw = (kBkStage - 2) / 2;  // one-launch path: the wind
 constexpr int kBkBack = 64;               // back substitution block rows
 constexpr double kBkPivotTol = 1.0e-14;
 
+constexpr int kBkTabDoubles = 320;  // per system: the panel's L11 and slot tables for the pivot-rows launch
+
 struct BkSys {
     double *work, *x, *status;
     int32_t *piv;
-    double *lt;  // the panel's multipliers with its later swaps applied: 16 columns x (hbw + 16) rows
+    double *lt;   // the panel's multipliers with its later swaps applied: 16 columns x (hbw + 16) rows
+    double *tab;  // BkTab of the current panel (kBkTabDoubles)
 };
 
 __device__ __forceinline__ BkSys bk_sys(const int32_t *recs, int n, int hbw, double *work, double *x,
                                         double *status) {
     const int r = recs ? recs[blockIdx.y] : 0;
     const size_t V = 3 * (size_t)hbw + 1;
-    const size_t per = (size_t)n * V + ((size_t)n + 1) / 2 + 1 + (size_t)kBkNb * (hbw + kBkNb);  // per system
+    const size_t per =
+        (size_t)n * V + ((size_t)n + 1) / 2 + 1 + (size_t)kBkNb * (hbw + kBkNb) + kBkTabDoubles;  // per system
     BkSys s;
     s.work = work + (size_t)r * per;
     s.piv = (int32_t *)(s.work + (size_t)n * V);
     s.lt = s.work + (size_t)n * V + ((size_t)n + 1) / 2 + 1;
+    s.tab = s.lt + (size_t)kBkNb * (hbw + kBkNb);
     s.x = x ? x + (size_t)r * n : nullptr;  // null: factor only
     s.status = status + 4 * (size_t)r;
     return s;
@@ -104,6 +109,16 @@ struct BkPanel {
     int slot_src[2 * kBkNb];          // slot whose row's entries end in slot t after all the swaps
     int nslot;
 };
+
+// What the pivot-rows launch needs of a panel (kept in the system's tab
+// scratch between the panel launch and it).
+struct BkTab {
+    double l11[kBkNb][kBkNb];
+    int slot_row[2 * kBkNb];
+    int slot_src[2 * kBkNb];
+    int nslot;
+};
+static_assert(sizeof(BkTab) <= kBkTabDoubles * sizeof(double), "BkTab fits its scratch");
 
 template <int CTRL>
 __device__ __forceinline__ double bk_dpp(double v) {
@@ -355,14 +370,15 @@ __device__ __forceinline__ void bk_pivot_rows_staged(const BkSys &s, int hbw, in
 // are solved against L11 (the reference's order: u_q -= l_qp u_p, ascending
 // p), and every slot is written back.  Columns are independent, so no
 // barrier is needed; all of a column's loads issue before its first use.
+template <typename Tab>
 __device__ __forceinline__ void bk_pivot_rows_cols(const BkSys &s, int hbw, int k0, int kb, int j0, int nc,
-                                                   int ncr, const BkPanel &L) {
-    const int V = 3 * hbw + 1, tid = threadIdx.x;
+                                                   int ncr, const Tab &L, int c_first, int c_step) {
+    const int V = 3 * hbw + 1;
     const int ns = uniform(L.nslot);
     // entry (row, o) of the system at 32-bit offset row V + o (a system's
     // work holds n V doubles, under 2^32)
     const double *__restrict__ wk = s.work;
-    for (int c = tid; c < ncr; c += kBkThreads) {
+    for (int c = c_first; c < ncr; c += c_step) {
         // keeps the L11 reads in the loop (hoisted, 120 values would take
         // the registers the column needs)
         asm volatile("" ::: "memory");
@@ -473,10 +489,40 @@ __global__ __launch_bounds__(kBkThreads) void bk_panel_kernel(const int32_t *rec
 #pragma unroll
         for (int c = 0; c < kBkNb; ++c)
             if (c < kb) s.lt[(size_t)c * (hbw + kBkNb) + tid] = r[c];
+    // the tables the pivot-rows launch needs (bk_panel ended with a barrier
+    // after they were formed)
+    BkTab *tab = reinterpret_cast<BkTab *>(s.tab);
+    for (int e = tid; e < kBkNb * kBkNb; e += kBkThreads) tab->l11[e / kBkNb][e % kBkNb] = L.l11[e / kBkNb][e % kBkNb];
+    if (tid < 2 * kBkNb) {
+        tab->slot_row[tid] = L.slot_row[tid];
+        tab->slot_src[tid] = L.slot_src[tid];
+    }
+    if (tid == 0) tab->nslot = L.nslot;
+}
+
+// The panel's swaps and pivot rows over its trailing columns (see
+// bk_pivot_rows_cols), 256 columns per workgroup, from the tables the panel
+// launch left in the system's scratch.
+__global__ __launch_bounds__(256) void bk_pivot_rows_kernel(const int32_t *recs, int n, int hbw, int k0,
+                                                            double *work, double *x, double *status) {
+    const BkSys s = bk_sys(recs, n, hbw, work, x, status);
+    if (s.status[0] == 2.0) return;
+    __shared__ BkTab T;
+    const BkTab *tab = reinterpret_cast<const BkTab *>(s.tab);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kBkNb * kBkNb; e += 256) T.l11[e / kBkNb][e % kBkNb] = tab->l11[e / kBkNb][e % kBkNb];
+    if (tid < 2 * kBkNb) {
+        T.slot_row[tid] = tab->slot_row[tid];
+        T.slot_src[tid] = tab->slot_src[tid];
+    }
+    if (tid == 0) T.nslot = tab->nslot;
+    __syncthreads();
+    const int kb = min(kBkNb, n - k0);
     const int j0 = k0 + kb, jmax = min(k0 + kb - 1 + 2 * hbw, n - 1);
     const int nc = jmax >= j0 ? jmax - j0 + 1 : 0;
     const int ncr = nc + (s.x ? 1 : 0);  // the right-hand side is column nc
-    bk_pivot_rows_cols(s, hbw, k0, kb, j0, s.x ? nc : -1, ncr, L);
+    const int c = blockIdx.x * 256 + tid;
+    if (c < ncr) bk_pivot_rows_cols(s, hbw, k0, kb, j0, s.x ? nc : -1, c + 1, T, c, 256);
 }
 
 // a_ij -= sum over the panel's k (ascending) of l_ik u_kj, rows k0 + kb ..
@@ -637,7 +683,8 @@ __global__ void bk_store_kernel(const int32_t *recs, int n, int hbw, double *wor
 bool band_blk_fits(int n, int hbw) { return n > 0 && hbw >= 0 && hbw + kBkNb <= kBkMaxRows; }
 
 size_t band_blk_work_doubles(int n, int hbw) {
-    return (size_t)n * (3 * (size_t)hbw + 1) + ((size_t)n + 1) / 2 + 1 + (size_t)kBkNb * (hbw + kBkNb);
+    return (size_t)n * (3 * (size_t)hbw + 1) + ((size_t)n + 1) / 2 + 1 + (size_t)kBkNb * (hbw + kBkNb) +
+           kBkTabDoubles;
 }
 
 // Factor and solve `count` systems (recs: their record indices; nullptr for
@@ -669,6 +716,13 @@ int launch_band_blk(sgpu_ctx *ctx, int n, int hbw, double *d_band, const double 
             hipLaunchKernelGGL(panel, dim3(1, count), dim3(kBkThreads), 0, st, recs, n, hbw, k0, d_work, d_x,
                                d_status);
             const int kb = std::min(kBkNb, n - k0);
+            {
+                const int jmax = std::min(k0 + kb - 1 + 2 * hbw, n - 1);
+                const int ncr = (jmax >= k0 + kb ? jmax - (k0 + kb) + 1 : 0) + (d_x ? 1 : 0);
+                if (ncr > 0)
+                    hipLaunchKernelGGL(bk_pivot_rows_kernel, dim3((ncr + 255) / 256, count), dim3(256), 0, st, recs,
+                                       n, hbw, k0, d_work, d_x, d_status);
+            }
             const int nr = std::min(k0 + kb - 1 + hbw, n - 1) - (k0 + kb) + 1;
             if (nr > 0) {
                 const int jmax = std::min(k0 + kb - 1 + 2 * hbw, n - 1);
