@@ -424,15 +424,24 @@ __device__ __forceinline__ void bk_pivot_rows_cols(const BkSys &s, int hbw, int 
 
 // x_i = (y_i - sum_{j > i} u_ij x_j) / u_ii by blocks of 64 rows from the
 // bottom; U row i at work columns i .. i + 2 hbw.  Every thread calls it.
-__device__ __forceinline__ void bk_back(const BkSys &s, int n, int hbw, double *s_s) {
+// s_tri (LDS, kBkBack x (kBkBack + 1)): the block's triangle, staged by every
+// thread while the sums run, so the triangle's sequential steps read LDS
+// instead of waiting on a global load each.
+__device__ __forceinline__ void bk_back(const BkSys &s, int n, int hbw, double *s_s,
+                                        double (*s_tri)[kBkBack + 1]) {
     const int V = 3 * hbw + 1;
     auto at = [&](int i, int j) -> double { return s.work[(size_t)i * V + (j - i + hbw)]; };
     const int tid = threadIdx.x, lane = tid & 63, w = uniform(tid >> 6);
     for (int r1 = n; r1 > 0; r1 -= kBkBack) {
         const int r0 = max(r1 - kBkBack, 0);
+        for (int e = tid; e < kBkBack * kBkBack; e += kBkThreads) {
+            const int a = e / kBkBack, b = e % kBkBack, q = r0 + a, i = r0 + b;
+            s_tri[a][b] = (i < r1 && q <= i && i <= q + 2 * hbw) ? at(q, i) : 0.0;
+        }
         for (int i = r0 + w; i < r1; i += kBkNb) {  // the block's sums over the solved unknowns
             const int jend = min(i + 2 * hbw, n - 1);
             double part = 0.0;
+#pragma unroll 8
             for (int j = r1 + lane; j <= jend; j += 64) part += at(i, j) * s.x[j];
             for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
             if (lane == 0) s_s[i - r0] = part;
@@ -443,10 +452,10 @@ __device__ __forceinline__ void bk_back(const BkSys &s, int n, int hbw, double *
             double t = q < r1 ? s.x[q] - s_s[lane] : 0.0;
             for (int i = r1 - 1; i >= r0; --i) {
                 double xi = 0.0;
-                if (lane == i - r0) xi = t / at(i, i);
+                if (lane == i - r0) xi = t / s_tri[i - r0][i - r0];
                 xi = lane_read(xi, i - r0);
                 if (lane == i - r0) s.x[i] = xi;
-                if (q < i && i <= q + 2 * hbw) t = t - at(q, i) * xi;
+                if (q < i && i <= q + 2 * hbw) t = t - s_tri[lane][i - r0] * xi;
             }
         }
         __syncthreads();
@@ -612,7 +621,8 @@ __global__ __launch_bounds__(kBkThreads) void bk_back_kernel(const int32_t *recs
     const BkSys s = bk_sys(recs, n, hbw, work, x, status);
     if (s.status[0] == 2.0) return;
     __shared__ double s_s[kBkBack];
-    bk_back(s, n, hbw, s_s);
+    __shared__ double s_tri[kBkBack][kBkBack + 1];
+    bk_back(s, n, hbw, s_s, s_tri);
 }
 
 // Narrow bands in one launch, one workgroup per system: every panel, its
@@ -656,7 +666,9 @@ __global__ __launch_bounds__(kBkThreads) void bk_fused_kernel(const int32_t *rec
         }
         __syncthreads();
     }
-    if (has_rhs) bk_back(s, n, hbw, s_s);
+    // the back substitution's triangle takes the stage's LDS (no longer used)
+    static_assert(sizeof(stage) >= sizeof(double) * kBkBack * (kBkBack + 1), "triangle fits the stage");
+    if (has_rhs) bk_back(s, n, hbw, s_s, reinterpret_cast<double (*)[kBkBack + 1]>(&stage[0][0]));
 }
 
 // The factors in the other band kernels' formats: U row k at band slots
