@@ -438,13 +438,32 @@ __device__ __forceinline__ void bk_back(const BkSys &s, int n, int hbw, double *
             const int a = e / kBkBack, b = e % kBkBack, q = r0 + a, i = r0 + b;
             s_tri[a][b] = (i < r1 && q <= i && i <= q + 2 * hbw) ? at(q, i) : 0.0;
         }
-        for (int i = r0 + w; i < r1; i += kBkNb) {  // the block's sums over the solved unknowns
-            const int jend = min(i + 2 * hbw, n - 1);
-            double part = 0.0;
-#pragma unroll 8
-            for (int j = r1 + lane; j <= jend; j += 64) part += at(i, j) * s.x[j];
-            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-            if (lane == 0) s_s[i - r0] = part;
+        // the block's sums over the solved unknowns: a wavefront's rows
+        // i = r0 + w + 16 m side by side, so their loads are in flight together
+        {
+            constexpr int kRows = kBkBack / kBkNb;
+            double part[kRows];
+            int jend[kRows];
+#pragma unroll
+            for (int m = 0; m < kRows; ++m) {
+                const int i = r0 + w + kBkNb * m;
+                part[m] = 0.0;
+                jend[m] = i < r1 ? min(i + 2 * hbw, n - 1) : -1;
+            }
+            const int jmax = min(r0 + w + kBkNb * (kRows - 1) + 2 * hbw, n - 1);
+#pragma unroll 4
+            for (int j = r1 + lane; j <= jmax; j += 64) {
+                const double xj = s.x[j];
+#pragma unroll
+                for (int m = 0; m < kRows; ++m)
+                    if (j <= jend[m]) part[m] += at(r0 + w + kBkNb * m, j) * xj;
+            }
+#pragma unroll
+            for (int m = 0; m < kRows; ++m) {
+                double p = part[m];
+                for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+                if (lane == 0 && jend[m] >= 0) s_s[w + kBkNb * m] = p;
+            }
         }
         __syncthreads();
         if (w == 0) {  // the block's triangle, by columns
