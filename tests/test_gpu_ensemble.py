@@ -117,3 +117,44 @@ def test_gpu_dense_fallback_records_match_oracle():
         total_fallbacks += stats["dense_fallbacks"]
     fb.close()
     assert total_fallbacks >= 10
+
+
+def test_gpu_autotuned_schedule_is_within_a_bound_of_the_best():
+    """The autotuner (rounds or work queue, measured on the batch itself and
+    re-measured when the chosen schedule's rate falls) must not lose much to
+    the better of the two schedules forced.  A mixed ensemble: the three
+    frames x 12 records of uneven length, PGA <= 0.6 g.  Bound: the
+    autotuned run takes at most 1.5 x the faster forced run's wall time
+    (sampling both schedules is part of the autotuned run)."""
+    import time
+
+    frames = [cases.config2_frame_case(s, b, steps=300) for s, b in cases.CONFIG2_FRAMES]
+    seeds = [s for s in range(200) if cases.config2_motion_params(s)["pga_g"] <= 0.6][:36]
+    runs = []
+    for k, fcase in enumerate(frames):
+        for j in range(12):
+            m = cases.config2_motion(seeds[12 * k + j], max_steps=150 + 10 * j)
+            runs.append((k, load_frame2d_case(_motion_case(fcase, m))))
+
+    def timed(schedule):
+        fb = FrameBatch(runs[0][1].materials)
+        frame_ids = {}
+        for k, fc in runs:
+            if k not in frame_ids:
+                frame_ids[k] = fb.add_frame(fc)
+            fb.add_case_motion(frame_ids[k], fc)
+        fb.finalize()
+        fb.set_schedule(schedule)
+        t0 = time.perf_counter()
+        fb.run(rounds=16)
+        dt = time.perf_counter() - t0
+        info = fb.schedule_info()
+        fb.close()
+        return dt, info
+
+    timed(_native.SCHEDULE_QUEUE)  # warm-up: code objects loaded, caches filled
+    t_rounds, _ = timed(_native.SCHEDULE_ROUNDS)
+    t_queue, _ = timed(_native.SCHEDULE_QUEUE)
+    t_auto, info = timed(_native.SCHEDULE_AUTO)
+    print("rounds %.3f s, queue %.3f s, auto %.3f s (%s)" % (t_rounds, t_queue, t_auto, info))
+    assert t_auto <= 1.5 * min(t_rounds, t_queue), (t_rounds, t_queue, t_auto, info)
