@@ -500,9 +500,10 @@ __global__ void bk_init_kernel(const int32_t *recs, int n, int hbw, const double
     if (q == 0) s.status[0] = 0.0;
 }
 
-// Panel k0 of every listed system, then its trailing columns' swaps and
-// pivot rows (one thread per column); the multipliers as the rows finally
-// stand go to the system's scratch for bk_update_kernel.
+// Panel k0 of every listed system; the multipliers as the rows finally
+// stand go to the system's scratch for bk_update_kernel, and L11 with the
+// slot tables for bk_pivot_rows_kernel (the trailing columns' swaps and
+// pivot rows, launched next).
 template <bool kStoreL>
 __global__ __launch_bounds__(kBkThreads) void bk_panel_kernel(const int32_t *recs, int n, int hbw, int k0,
                                                               double *work, double *x, double *status) {
